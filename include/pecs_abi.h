@@ -1,0 +1,147 @@
+/* C ABI of the MI355X-native PECS per-time-step IMEX path.
+ *
+ * One opaque context owns the three meshes, the constant matrices (as factored
+ * SPD Schur / multiplier systems), the device state and the scratch buffers of
+ * one problem instance -- the roles of SOLARCELL::SolarCellProblem's members
+ * (include/SolarCell.hpp:327-444).  All functions return 0 on success and a
+ * negative code on error; pecs_last_error() gives the message (the reference
+ * reports failures by exceptions caught in main, source/main/main.cpp:28-51).
+ * One host thread per context; calls are ordered on the context's HIP stream.
+ *
+ * Vector layouts are the reference's component-wise layouts:
+ *   carrier vector (electrons, holes on the semiconductor mesh; reductants,
+ *   oxidants on the electrolyte mesh): [Jx | Jy | u], each block cell-major in
+ *   deal.II cell order (Morton order of the 2^r x 2^r cells), (k+1)^2
+ *   lexicographic DoFs per cell (CarrierPair.cpp:23-64);
+ *   Poisson vector: [D | Phi] (Poisson.cpp:18-69); D uses a nodal RT basis
+ *   (face DoFs: vertical faces j*(NX+1)+i then horizontal faces j*NX+i, k DoFs
+ *   each; then 2(k-1)k interior DoFs per cell), Phi is FE_DGQ(k-1) per cell.
+ * Host-pointer functions copy in and out; *_dev functions take device
+ * pointers and do not synchronise.
+ */
+#ifndef PECS_ABI_H
+#define PECS_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pecs_ctx pecs_ctx;
+
+/* Geometry of the half cell (Grid_Maker::Grid, source/Grid.cpp:11-44,
+ * 135-336): semiconductor trapezoid (0,0),(r2,0),(0,H),(r1,H); electrolyte
+ * (r2,0),(L,0),(r1,H),(L,H); both refined n_refine times. */
+typedef struct {
+  double r1, r2, H, L;
+  int n_refine;
+  int insulated; /* make_Neumann_boundaries, Grid.cpp:376-430 */
+  int schottky;  /* make_Schottky_boundaries, Grid.cpp:432-461 */
+} pecs_mesh;
+
+/* Scaled parameters (ParameterSpace::Parameters, include/Parameters.hpp:25-80,
+ * after parse_and_scale_parameters). */
+typedef struct {
+  int degree;            /* LDG degree k (main.cpp:11); Poisson RT_{k-1} x DGQ_{k-1} */
+  double dt, theta, penalty;
+  double mu[4];          /* electrons, holes, reductants, oxidants */
+  double eps_s, eps_e, lambda2;
+  double k_et, k_ht, v_rec_n, v_rec_p;
+  double n_e, p_e, r_inf, o_inf;   /* equilibria / doping (InitialConditions.cpp) */
+  double phi_bi, phi_app, phi_sch, schottky_location;
+  int schottky_location_set;
+  int gen_mode;          /* 0: alpha*G0*exp(alpha*(y-H)) (Generation.cpp:29-44); 1: G0*exp(-alpha*x); 2: off */
+  double G0, alpha;
+  int srh;               /* 0: R = 0 (SolarCell.hpp:86-98); 1: SRH formula */
+  double n_i, tau_n, tau_p;
+  int leaf_cells;        /* nested-dissection leaf size (0: default) */
+} pecs_params;
+
+/* Physical inputs of input_file.prm (ParameterReader.cpp:29-208). */
+typedef struct {
+  double char_length, char_time, char_density;
+  double electron_mobility, hole_mobility, reductant_mobility, oxidant_mobility;
+  double electron_recombo_t, hole_recombo_t, electron_recombo_v, hole_recombo_v;
+  double k_et, k_ht, intrinsic_density, photon_flux, absorption_coeff;
+  double applied_bias, built_in_bias, schottky_bias;
+} pecs_physical;
+
+typedef struct {
+  double mobility[4], recombo_t[2], recombo_v[2], k_et, k_ht, intrinsic_density;
+  double photon_flux, absorption_coeff, debye_length, applied_bias, built_in_bias, schottky_bias;
+  double rescaled_k_et, rescaled_k_ht, rescale_current;
+} pecs_scaled;
+
+enum { PECS_ELECTRONS = 0, PECS_HOLES = 1, PECS_REDUCTANTS = 2, PECS_OXIDANTS = 3, PECS_POISSON = 4, PECS_MASS_SEMI = 5,
+       PECS_MASS_ELEC = 6 };
+
+/* Parameters::parse_and_scale_parameters, include/Parameters.hpp:117-287. */
+int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out);
+
+/* SolarCellProblem constructor + run_full_system setup (SolarCell.cpp:15-92,
+ * 1898-1970): meshes, maps, constant matrices, factorisation, device upload. */
+int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out);
+int pecs_destroy(pecs_ctx* ctx);
+const char* pecs_last_error(pecs_ctx* ctx);
+/* Error text of the last failed pecs_create / stateless call on this thread. */
+const char* pecs_global_error(void);
+int pecs_set_stream(pecs_ctx* ctx, void* hip_stream);
+int pecs_sizes(pecs_ctx* ctx, int64_t* n_semi, int64_t* n_elec, int64_t* n_poisson);
+/* Setup statistics: factor sizes (doubles), setup seconds, #supernode levels. */
+int pecs_stats(pecs_ctx* ctx, double* out, int n);
+
+/* Constant matrices as CSR (rows, nnz) in the reference layouts, rebuilt on the
+ * host from the local operators (SolarCell.cpp:377-417, 822-932): call with
+ * rowptr == NULL to get sizes.  which: PECS_ELECTRONS..PECS_OXIDANTS (system
+ * matrices), PECS_POISSON (unconstrained mixed matrix), PECS_MASS_*.
+ * `constrained` (may be NULL, PECS_POISSON only) flags Neumann D.n DoFs. */
+int pecs_export_matrix(const pecs_mesh* mesh, const pecs_params* params, int which, int64_t* n, int64_t* nnz,
+                       int64_t* rowptr, int32_t* col, double* val, int8_t* constrained);
+
+/* Per-step right-hand sides.
+ * assemble_semiconductor_rhs (SolarCell.cpp:1037-1071, worker 1073-1414):
+ *   rhs_n, rhs_p = M u_old + cell drift/generation + Dirichlet, interface and
+ *   Schottky face terms, with the lagged electrolyte traces u_r, u_o. */
+int pecs_rhs_semiconductor(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,
+                           const double* u_r, const double* u_o, double* rhs_n, double* rhs_p);
+/* assemble_electrolyte_rhs (SolarCell.cpp:1417-1449, worker 1451-1726). */
+int pecs_rhs_electrolyte(pecs_ctx* ctx, const double* u_r, const double* u_o, const double* poisson,
+                         const double* u_n, const double* u_p, double* rhs_r, double* rhs_o);
+/* assemble_Poisson_rhs (SolarCell.cpp:430-485, workers 487-815). */
+int pecs_rhs_poisson(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* u_r, const double* u_o,
+                     double* rhs);
+/* Carrier::solve / PoissonData::solve (Carrier.cpp:34-40, Poisson.cpp:98-105):
+ * x = A^{-1} rhs for which = PECS_ELECTRONS..PECS_POISSON. */
+int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x);
+
+/* State (Carrier::solution, PoissonData::solution). poisson may be NULL in
+ * set_state (then it is recomputed from the densities as in SolarCell.cpp:2033). */
+int pecs_set_state(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* u_r, const double* u_o,
+                   const double* poisson);
+int pecs_get_state(pecs_ctx* ctx, double* u_n, double* u_p, double* u_r, double* u_o, double* poisson);
+/* Equilibrium initial data + first Poisson solve (SolarCell.cpp:1996-2034). */
+int pecs_initial_state(pecs_ctx* ctx);
+/* nsteps of the IMEX loop body (SolarCell.cpp:2057-2075), device resident. */
+int pecs_step(pecs_ctx* ctx, int nsteps);
+/* Relative steady residual ||x^{n+1}-x^n|| / (||x^n|| dt) over the five state
+ * vectors after the last pecs_step call (the reference keeps dead diff_* vectors,
+ * SolarCell.cpp:2042-2046). */
+int pecs_last_residual(pecs_ctx* ctx, double* residual);
+
+/* Device-pointer variants (no copies, no synchronisation). */
+int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,
+                               const double* u_r, const double* u_o, double* rhs_n, double* rhs_p);
+int pecs_rhs_electrolyte_dev(pecs_ctx* ctx, const double* u_r, const double* u_o, const double* poisson,
+                             const double* u_n, const double* u_p, double* rhs_r, double* rhs_o);
+int pecs_rhs_poisson_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* u_r,
+                         const double* u_o, double* rhs);
+int pecs_solve_dev(pecs_ctx* ctx, int which, const double* rhs, double* x);
+/* Device pointers of the resident state (valid until pecs_destroy). */
+int pecs_state_dev(pecs_ctx* ctx, double** u_n, double** u_p, double** u_r, double** u_o, double** poisson);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PECS_ABI_H */

@@ -1,0 +1,254 @@
+"""Python binding of the MI355X-native PECS IMEX path (ctypes over include/pecs_abi.h).
+
+The product is the C-ABI library libpecs_amd.so (HIP kernels for gfx950 plus
+the C++ setup code).  This module only marshals numpy arrays to it; every
+right-hand side, solve and time step runs in the library on the GPU.  If the
+library is missing, or no GPU is present, the calls raise -- there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpecs_amd.so")
+_lib = None
+
+ELECTRONS, HOLES, REDUCTANTS, OXIDANTS, POISSON, MASS_SEMI, MASS_ELEC = range(7)
+
+ABI_FUNCTIONS = [
+    "pecs_scale_parameters", "pecs_create", "pecs_destroy", "pecs_last_error", "pecs_global_error",
+    "pecs_set_stream", "pecs_sizes", "pecs_stats", "pecs_export_matrix", "pecs_rhs_semiconductor",
+    "pecs_rhs_electrolyte", "pecs_rhs_poisson", "pecs_solve", "pecs_set_state", "pecs_get_state",
+    "pecs_initial_state", "pecs_step", "pecs_last_residual", "pecs_rhs_semiconductor_dev",
+    "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
+]
+
+
+class CMesh(ctypes.Structure):
+    _fields_ = [("r1", ctypes.c_double), ("r2", ctypes.c_double), ("H", ctypes.c_double), ("L", ctypes.c_double),
+                ("n_refine", ctypes.c_int), ("insulated", ctypes.c_int), ("schottky", ctypes.c_int)]
+
+
+class CParams(ctypes.Structure):
+    _fields_ = [("degree", ctypes.c_int), ("dt", ctypes.c_double), ("theta", ctypes.c_double),
+                ("penalty", ctypes.c_double), ("mu", ctypes.c_double * 4), ("eps_s", ctypes.c_double),
+                ("eps_e", ctypes.c_double), ("lambda2", ctypes.c_double), ("k_et", ctypes.c_double),
+                ("k_ht", ctypes.c_double), ("v_rec_n", ctypes.c_double), ("v_rec_p", ctypes.c_double),
+                ("n_e", ctypes.c_double), ("p_e", ctypes.c_double), ("r_inf", ctypes.c_double),
+                ("o_inf", ctypes.c_double), ("phi_bi", ctypes.c_double), ("phi_app", ctypes.c_double),
+                ("phi_sch", ctypes.c_double), ("schottky_location", ctypes.c_double),
+                ("schottky_location_set", ctypes.c_int), ("gen_mode", ctypes.c_int), ("G0", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("srh", ctypes.c_int), ("n_i", ctypes.c_double),
+                ("tau_n", ctypes.c_double), ("tau_p", ctypes.c_double), ("leaf_cells", ctypes.c_int)]
+
+
+class CPhysical(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "char_length", "char_time", "char_density", "electron_mobility", "hole_mobility", "reductant_mobility",
+        "oxidant_mobility", "electron_recombo_t", "hole_recombo_t", "electron_recombo_v", "hole_recombo_v",
+        "k_et", "k_ht", "intrinsic_density", "photon_flux", "absorption_coeff", "applied_bias",
+        "built_in_bias", "schottky_bias")]
+
+
+class CScaled(ctypes.Structure):
+    _fields_ = [("mobility", ctypes.c_double * 4), ("recombo_t", ctypes.c_double * 2),
+                ("recombo_v", ctypes.c_double * 2)] + [(n, ctypes.c_double) for n in (
+                    "k_et", "k_ht", "intrinsic_density", "photon_flux", "absorption_coeff", "debye_length",
+                    "applied_bias", "built_in_bias", "schottky_bias", "rescaled_k_et", "rescaled_k_ht",
+                    "rescale_current")]
+
+
+def lib():
+    """Load libpecs_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.pecs_last_error.restype = ctypes.c_char_p
+        L.pecs_global_error.restype = ctypes.c_char_p
+        L.pecs_create.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(CParams), ctypes.POINTER(ctypes.c_void_p)]
+        for name in ("pecs_destroy", "pecs_initial_state"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.pecs_last_error.argtypes = [ctypes.c_void_p]
+        L.pecs_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pecs_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int64)] * 3
+        L.pecs_stats.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.pecs_last_residual.argtypes = [ctypes.c_void_p, dp]
+        L.pecs_rhs_semiconductor.argtypes = [ctypes.c_void_p] + [dp] * 7
+        L.pecs_rhs_electrolyte.argtypes = [ctypes.c_void_p] + [dp] * 7
+        L.pecs_rhs_poisson.argtypes = [ctypes.c_void_p] + [dp] * 5
+        L.pecs_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+        L.pecs_set_state.argtypes = [ctypes.c_void_p] + [dp] * 5
+        L.pecs_get_state.argtypes = [ctypes.c_void_p] + [dp] * 5
+        L.pecs_export_matrix.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(CParams), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), dp,
+                                         ctypes.POINTER(ctypes.c_int8)]
+        L.pecs_scale_parameters.argtypes = [ctypes.POINTER(CPhysical), ctypes.POINTER(CScaled)]
+        _lib = L
+    return _lib
+
+
+GEN_MODES = {"ref": 0, "config": 1, "off": 2}
+
+
+def to_c(prm):
+    """Build (CMesh, CParams) from a parameter dict (keys as oracle.solarcell.default_params)."""
+    m = CMesh(prm["r1"], prm["r2"], prm["H"], prm["L"], prm["r"], int(prm["insulated"]), int(prm["schottky"]))
+    p = CParams()
+    p.degree = prm["k"]
+    p.dt, p.theta, p.penalty = prm["dt"], prm.get("theta", 1.0), prm["penalty"]
+    for i, key in enumerate(("mu_n", "mu_p", "mu_r", "mu_o")):
+        p.mu[i] = prm[key]
+    p.eps_s, p.eps_e, p.lambda2 = prm["eps_s"], prm["eps_e"], prm["lam2"]
+    p.k_et, p.k_ht, p.v_rec_n, p.v_rec_p = prm["k_et"], prm["k_ht"], prm["v_rec_n"], prm["v_rec_p"]
+    p.n_e, p.p_e, p.r_inf, p.o_inf = prm["ne"], prm["pe"], prm["rinf"], prm["oinf"]
+    p.phi_bi, p.phi_app, p.phi_sch = prm["phi_bi"], prm["phi_app"], prm["phi_sch"]
+    if "schottky_location" in prm:
+        p.schottky_location = prm["schottky_location"]
+        p.schottky_location_set = 1
+    p.gen_mode = GEN_MODES[prm["gen_mode"]]
+    p.G0, p.alpha = prm["G0"], prm["alpha"]
+    p.srh = int(prm["srh"])
+    p.n_i, p.tau_n, p.tau_p = prm["ni"], prm["tau_n"], prm["tau_p"]
+    p.leaf_cells = prm.get("leaf_cells", 0)
+    return m, p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        L = lib()
+        msg = L.pecs_last_error(ctx) if ctx else L.pecs_global_error()
+        raise RuntimeError("pecs: " + (msg.decode() if msg else f"error {rc}"))
+
+
+def export_matrix(prm, which):
+    """Constant matrix `which` as scipy CSR (+ constrained flags for POISSON); host only."""
+    import scipy.sparse as sp
+    L = lib()
+    m, p = to_c(prm)
+    n = ctypes.c_int64()
+    nnz = ctypes.c_int64()
+    _check(L.pecs_export_matrix(ctypes.byref(m), ctypes.byref(p), which, ctypes.byref(n), ctypes.byref(nnz),
+                                None, None, None, None))
+    rp = np.zeros(n.value + 1, np.int64)
+    col = np.zeros(nnz.value, np.int32)
+    val = np.zeros(nnz.value, np.float64)
+    cons = np.zeros(n.value, np.int8)
+    _check(L.pecs_export_matrix(ctypes.byref(m), ctypes.byref(p), which, None, None,
+                                rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _ptr(val),
+                                cons.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))))
+    A = sp.csr_matrix((val, col, rp), shape=(n.value, n.value))
+    return (A, cons.astype(bool)) if which == POISSON else A
+
+
+def scale_parameters(phys):
+    """Parameters::parse_and_scale_parameters through the library (dict in, dict out)."""
+    c = CPhysical(**phys)
+    out = CScaled()
+    _check(lib().pecs_scale_parameters(ctypes.byref(c), ctypes.byref(out)))
+    d = {name: getattr(out, name) for name, _ in CScaled._fields_}
+    d["mobility"] = list(out.mobility)
+    d["recombo_t"] = list(out.recombo_t)
+    d["recombo_v"] = list(out.recombo_v)
+    return d
+
+
+class SolarCellGPU:
+    """One problem instance on the GPU (SolarCellProblem roles, SolarCell.hpp:105-757)."""
+
+    def __init__(self, prm):
+        L = lib()
+        self.prm = dict(prm)
+        m, p = to_c(prm)
+        ctx = ctypes.c_void_p()
+        _check(L.pecs_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(ctx)))
+        self.ctx = ctx
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.pecs_sizes(ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self.n_semi, self.n_elec, self.n_poisson = a.value, b.value, c.value
+
+    def close(self):
+        if self.ctx:
+            lib().pecs_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(rc, self.ctx)
+
+    def stats(self):
+        out = np.zeros(10)
+        self._c(lib().pecs_stats(self.ctx, _ptr(out), 10))
+        keys = ["factor_semi_doubles", "factor_elec_doubles", "factor_poisson_doubles", "setup_s", "factor_s",
+                "levels_semi", "levels_elec", "levels_poisson", "n_schur_semi", "n_multipliers"]
+        return dict(zip(keys, out.tolist()))
+
+    @staticmethod
+    def _f(a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def rhs_semiconductor(self, un, up, P, ur, uo):
+        bn = np.zeros(self.n_semi)
+        bp = np.zeros(self.n_semi)
+        a = [self._f(x) for x in (un, up, P, ur, uo)]
+        self._c(lib().pecs_rhs_semiconductor(self.ctx, *[_ptr(x) for x in a], _ptr(bn), _ptr(bp)))
+        return bn, bp
+
+    def rhs_electrolyte(self, ur, uo, P, un, up):
+        br = np.zeros(self.n_elec)
+        bo = np.zeros(self.n_elec)
+        a = [self._f(x) for x in (ur, uo, P, un, up)]
+        self._c(lib().pecs_rhs_electrolyte(self.ctx, *[_ptr(x) for x in a], _ptr(br), _ptr(bo)))
+        return br, bo
+
+    def rhs_poisson(self, un, up, ur, uo):
+        b = np.zeros(self.n_poisson)
+        a = [self._f(x) for x in (un, up, ur, uo)]
+        self._c(lib().pecs_rhs_poisson(self.ctx, *[_ptr(x) for x in a], _ptr(b)))
+        return b
+
+    def solve(self, which, b):
+        n = {ELECTRONS: self.n_semi, HOLES: self.n_semi, REDUCTANTS: self.n_elec, OXIDANTS: self.n_elec,
+             POISSON: self.n_poisson}[which]
+        b = self._f(b)
+        assert b.shape == (n,)
+        x = np.zeros(n)
+        self._c(lib().pecs_solve(self.ctx, which, _ptr(b), _ptr(x)))
+        return x
+
+    def set_state(self, st):
+        a = [self._f(st[k]) for k in "npro"]
+        P = _ptr(self._f(st["P"])) if st.get("P") is not None else None
+        self._c(lib().pecs_set_state(self.ctx, *[_ptr(x) for x in a], P))
+
+    def get_state(self):
+        st = {k: np.zeros(self.n_semi if k in "np" else self.n_elec) for k in "npro"}
+        st["P"] = np.zeros(self.n_poisson)
+        self._c(lib().pecs_get_state(self.ctx, *[_ptr(st[k]) for k in "npro"], _ptr(st["P"])))
+        return st
+
+    def initial_state(self):
+        self._c(lib().pecs_initial_state(self.ctx))
+
+    def step(self, nsteps=1):
+        self._c(lib().pecs_step(self.ctx, int(nsteps)))
+
+    def residual(self):
+        r = ctypes.c_double()
+        self._c(lib().pecs_last_residual(self.ctx, ctypes.byref(r)))
+        return r.value

@@ -1,0 +1,767 @@
+// C-ABI of the PECS IMEX path: context, setup, device upload, step loop.
+// Declarations and reference citations: include/pecs_abi.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pecs_abi.h"
+#include "cholesky.hpp"
+#include "fe.hpp"
+#include "kernels.hpp"
+#include "mesh.hpp"
+#include "ops.hpp"
+
+using namespace pecs;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct FactorDev {
+  DevFactor d{};
+  std::vector<int> off;  // per level task offsets (host)
+  int nlevels = 0, n = 0;
+  double nnz = 0.0;
+  std::vector<int> cell_base;  // host copy: grid cell -> permuted base
+};
+
+}  // namespace
+
+struct pecs_ctx {
+  std::string err;
+  hipStream_t stream = nullptr;
+  Params P;
+  int k = 1;
+  SolarCellMesh mesh;
+  PoissonLayout L;
+  std::unique_ptr<RefTables> T;
+  DevTabs tabs{};
+  DevRegion Rs{}, Re{};
+  DevPoisson Pd{};
+  DevPhys ph{};
+  FactorDev Fs[2], Fe[2], FH;
+  bool share_s = false, share_e = false;
+  int64_t Ns = 0, Ne = 0, NP = 0;
+  double* u[4] = {};
+  double* pv = nullptr;
+  double* b[4] = {};
+  double* W[4] = {};
+  double* g[2][2] = {};
+  double* t[2][2] = {};
+  double* yP = nullptr;
+  double* rH = nullptr;
+  double* tH = nullptr;
+  double* old[5] = {};
+  double* acc = nullptr;
+  double* io[4] = {};  // staging for host-pointer calls
+  double last_res = -1.0;
+  double setup_seconds = 0.0, factor_seconds = 0.0;
+  std::vector<void*> allocs;
+
+  ~pecs_ctx() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+#define HIPCHECK(x)                                                                               \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+T* dalloc(pecs_ctx* c, size_t n) {
+  void* p = nullptr;
+  HIPCHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+  c->allocs.push_back(p);
+  return (T*)p;
+}
+
+template <class T>
+T* upload(pecs_ctx* c, const std::vector<T>& v) {
+  T* p = dalloc<T>(c, v.size());
+  if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+Params to_params(const pecs_params* q) {
+  Params P;
+  P.k = q->degree;
+  P.dt = q->dt;
+  P.theta = q->theta;
+  P.penalty = q->penalty;
+  for (int i = 0; i < 4; ++i) P.mu[i] = q->mu[i];
+  P.eps_s = q->eps_s;
+  P.eps_e = q->eps_e;
+  P.lam2 = q->lambda2;
+  P.k_et = q->k_et;
+  P.k_ht = q->k_ht;
+  P.v_rec_n = q->v_rec_n;
+  P.v_rec_p = q->v_rec_p;
+  P.ne = q->n_e;
+  P.pe = q->p_e;
+  P.rinf = q->r_inf;
+  P.oinf = q->o_inf;
+  P.phi_bi = q->phi_bi;
+  P.phi_app = q->phi_app;
+  P.phi_sch = q->phi_sch;
+  P.schottky_location = q->schottky_location;
+  P.schottky_location_set = q->schottky_location_set;
+  P.gen_mode = q->gen_mode;
+  P.G0 = q->G0;
+  P.alpha = q->alpha;
+  P.srh = q->srh;
+  P.ni = q->n_i;
+  P.tau_n = q->tau_n;
+  P.tau_p = q->tau_p;
+  return P;
+}
+
+GeometrySpec to_geo(const pecs_mesh* m) {
+  GeometrySpec g;
+  g.r1 = m->r1;
+  g.r2 = m->r2;
+  g.H = m->H;
+  g.L = m->L;
+  g.insulated = m->insulated != 0;
+  g.schottky = m->schottky != 0;
+  return g;
+}
+
+void validate(const pecs_mesh* m, const pecs_params* q) {
+  if (!m || !q) throw std::runtime_error("null mesh or params");
+  if (q->degree < 1 || q->degree > KMAX) throw std::runtime_error("degree must be 1..3");
+  if (m->n_refine < 1 || m->n_refine > 10) throw std::runtime_error("n_refine must be 1..10");
+  if (!(q->dt > 0)) throw std::runtime_error("dt must be positive");
+  if (!(m->r1 > 0 && m->r2 > 0 && m->L > m->r1 && m->L > m->r2 && m->H > 0)) throw std::runtime_error("bad geometry");
+  for (int i = 0; i < 4; ++i)
+    if (!(q->mu[i] > 0)) throw std::runtime_error("mobilities must be positive");
+  if (!(q->eps_s > 0 && q->eps_e > 0 && q->lambda2 > 0)) throw std::runtime_error("permittivities/lambda2 must be positive");
+}
+
+FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf) {
+  SupernodalFactor F = cholesky_factor(S, leaf);
+  FactorDev D;
+  D.n = F.n;
+  D.nlevels = F.nlevels;
+  D.cell_base = F.cell_base;
+  const int nn = (int)F.nodes.size();
+  std::vector<int> p0(nn), ns(nn), ms(nn);
+  std::vector<int64_t> lo(nn), l2(nn), ro(nn);
+  for (int s = 0; s < nn; ++s) {
+    p0[s] = F.nodes[s].p0;
+    ns[s] = F.nodes[s].ns;
+    ms[s] = F.nodes[s].ms;
+    lo[s] = F.nodes[s].linv_off;
+    l2[s] = F.nodes[s].l21_off;
+    ro[s] = F.nodes[s].rows_off;
+  }
+  std::vector<double> LinvT(F.Linv.size()), L21T(F.L21.size());
+  for (int s = 0; s < nn; ++s) {
+    const int n_ = ns[s], m_ = ms[s];
+    for (int i = 0; i < n_; ++i)
+      for (int j = 0; j < n_; ++j) LinvT[lo[s] + (int64_t)j * n_ + i] = F.Linv[lo[s] + (int64_t)i * n_ + j];
+    for (int r = 0; r < m_; ++r)
+      for (int j = 0; j < n_; ++j) L21T[l2[s] + (int64_t)j * m_ + r] = F.L21[l2[s] + (int64_t)r * n_ + j];
+  }
+  std::vector<int2> tasks;
+  D.off.assign(F.nlevels + 1, 0);
+  for (int l = 0; l < F.nlevels; ++l) {
+    D.off[l] = (int)tasks.size();
+    for (int s : F.level_nodes[l])
+      for (int r = 0; r < ns[s]; ++r) tasks.push_back(make_int2(s, r));
+  }
+  D.off[F.nlevels] = (int)tasks.size();
+  D.nnz = (double)F.Linv.size() + (double)F.L21.size();
+  DevFactor& d = D.d;
+  d.n = F.n;
+  d.nnodes = nn;
+  d.p0 = upload(c, p0);
+  d.ns = upload(c, ns);
+  d.ms = upload(c, ms);
+  d.linv_off = upload(c, lo);
+  d.l21_off = upload(c, l2);
+  d.rows_off = upload(c, ro);
+  d.Linv = upload(c, F.Linv);
+  d.LinvT = upload(c, LinvT);
+  d.L21 = upload(c, F.L21);
+  d.L21T = upload(c, L21T);
+  d.rows = upload(c, F.rows);
+  d.cptr = upload(c, F.contrib_ptr);
+  d.cnode = upload(c, F.contrib_node);
+  d.crow = upload(c, F.contrib_row);
+  d.tasks = upload(c, tasks);
+  return D;
+}
+
+DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, const std::vector<int>& sigma,
+                          const FactorDev& F) {
+  DevRegion d{};
+  d.N = R.N;
+  d.ncell = R.ncell;
+  d.V = upload(c, R.V);
+  std::vector<double> h(R.ncell);
+  for (int i = 0; i < R.ncell; ++i) h[i] = cell_diameter(&R.V[8 * i]);
+  d.h = upload(c, h);
+  d.nbr = upload(c, R.nbr);
+  d.bid = upload(c, R.bid);
+  d.type = upload(c, O.type);
+  d.table = upload(c, O.table);
+  d.stride = O.type_stride;
+  d.sigma_nb = upload(c, sigma);
+  std::vector<int> pb(R.ncell);
+  for (int i = 0; i < R.ncell; ++i) pb[i] = F.cell_base[R.ci[i] + R.N * R.cj[i]];
+  d.perm_base = upload(c, pb);
+  return d;
+}
+
+void solve_region(pecs_ctx* c, int reg, const double* b0, const double* b1, double* x0, double* x1, int nrhs) {
+  const DevRegion& R = reg == 0 ? c->Rs : c->Re;
+  FactorDev* F = reg == 0 ? c->Fs : c->Fe;
+  const bool share = reg == 0 ? c->share_s : c->share_e;
+  const double mu0 = c->P.mu[2 * reg], mu1 = c->P.mu[2 * reg + 1];
+  double* W0 = c->W[2 * reg];
+  double* W1 = c->W[2 * reg + 1];
+  hipStream_t s = c->stream;
+  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, mu0, W0, s));
+  if (nrhs == 2) HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, mu1, W1, s));
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg][0], c->g[reg][1], nrhs, s));
+  if (nrhs == 2 && share) {
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], c->g[reg][1],
+                                           c->t[reg][0], c->t[reg][1], 2, s));
+  } else {
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], nullptr,
+                                           c->t[reg][0], nullptr, 1, s));
+    if (nrhs == 2) {
+      const FactorDev& F1 = share ? F[0] : F[1];
+      HIPCHECK((hipError_t)launch_chol_solve(F1.d, F1.off.data(), F1.nlevels, c->g[reg][1], nullptr, c->t[reg][1],
+                                             nullptr, 1, s));
+    }
+  }
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W0, W1, c->g[reg][0], c->g[reg][1], mu0, mu1, x0, x1, b0, b1,
+                                            nrhs, s));
+}
+
+// single-carrier solve with its own factor
+void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
+  const int reg = which / 2, car = which % 2;
+  const DevRegion& R = reg == 0 ? c->Rs : c->Re;
+  FactorDev* F = reg == 0 ? c->Fs : c->Fe;
+  const bool share = reg == 0 ? c->share_s : c->share_e;
+  const FactorDev& Fc = (car == 1 && !share) ? F[1] : F[0];
+  const double mu = c->P.mu[which];
+  hipStream_t s = c->stream;
+  double* W = c->W[which];
+  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b, mu, W, s));
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b, b, W, W, c->g[reg][0], c->g[reg][0], 1, s));
+  HIPCHECK((hipError_t)launch_chol_solve(Fc.d, Fc.off.data(), Fc.nlevels, c->g[reg][0], nullptr, c->t[reg][0],
+                                         nullptr, 1, s));
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W, W, c->g[reg][0], c->g[reg][0], mu, mu, x, x, b, b, 1, s));
+}
+
+void poisson_solve_from_y(pecs_ctx* c, double* out) {
+  hipStream_t s = c->stream;
+  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.off.data(), c->FH.nlevels, c->rH, nullptr, c->tH, nullptr, 1,
+                                         s));
+  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, out, s));
+}
+
+void poisson_from_densities(pecs_ctx* c, const double* un, const double* up, const double* ur, const double* uo,
+                            double* out) {
+  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->ph, un, up, ur, uo, c->mesh.semi.ncell,
+                                           c->mesh.elec.ncell, c->yP, c->stream));
+  poisson_solve_from_y(c, out);
+}
+
+void step_once(pecs_ctx* c) {
+  hipStream_t s = c->stream;
+  HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->ph, c->u[0], c->u[1],
+                                          c->u[2], c->u[3], c->pv, c->b[0], c->b[1], s));
+  HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->ph, c->u[2], c->u[3],
+                                          c->u[0], c->u[1], c->pv, c->b[2], c->b[3], s));
+  solve_region(c, 0, c->b[0], c->b[1], c->u[0], c->u[1], 2);
+  solve_region(c, 1, c->b[2], c->b[3], c->u[2], c->u[3], 2);
+  poisson_from_densities(c, c->u[0], c->u[1], c->u[2], c->u[3], c->pv);
+}
+
+int fail(pecs_ctx* c, const std::exception& e) {
+  if (c) c->err = e.what();
+  g_error = e.what();
+  return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out) {
+  // Parameters.hpp:164-251 (constants Parameters.hpp:7-12)
+  if (!in || !out) return -1;
+  const double UT = 0.02585, q = 1.62e-19, eps0 = 8.85e-14;
+  const double L = in->char_length, t = in->char_time, C = in->char_density;
+  const double mob = (t * UT) / (L * L);
+  out->mobility[0] = in->electron_mobility * mob;
+  out->mobility[1] = in->hole_mobility * mob;
+  out->mobility[2] = in->reductant_mobility * mob;
+  out->mobility[3] = in->oxidant_mobility * mob;
+  out->recombo_t[0] = in->electron_recombo_t / t;
+  out->recombo_t[1] = in->hole_recombo_t / t;
+  out->recombo_v[0] = in->electron_recombo_v * (t / L);
+  out->recombo_v[1] = in->hole_recombo_v * (t / L);
+  out->intrinsic_density = in->intrinsic_density / C;
+  out->photon_flux = in->photon_flux * (t / C);
+  out->absorption_coeff = in->absorption_coeff * L;
+  out->debye_length = (UT * eps0) / (q * C * L * L);
+  out->rescaled_k_et = q * in->k_et * C * C;
+  out->rescaled_k_ht = q * in->k_ht * C * C;
+  out->k_et = in->k_et * (t * C / L);
+  out->k_ht = in->k_ht * (t * C / L);
+  out->applied_bias = in->applied_bias / UT;
+  out->built_in_bias = in->built_in_bias / UT;
+  out->schottky_bias = in->schottky_bias / UT;
+  out->rescale_current = (q * C * L) / t;
+  return 0;
+}
+
+const char* pecs_global_error(void) { return g_error.c_str(); }
+
+int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  auto c = std::make_unique<pecs_ctx>();
+  try {
+    validate(mesh, params);
+    auto t0 = std::chrono::steady_clock::now();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      throw std::runtime_error("no HIP device: the PECS path runs on the GPU only");
+    c->P = to_params(params);
+    c->k = params->degree;
+    const int leaf = params->leaf_cells > 0 ? params->leaf_cells : 16;
+    c->mesh = make_solarcell_mesh(mesh->n_refine, to_geo(mesh));
+    c->T = std::make_unique<RefTables>(c->k);
+    const RefTables& T = *c->T;
+    c->L = poisson_layout(c->mesh, c->k - 1);
+    const int nb = T.nb;
+    c->Ns = 3LL * nb * c->mesh.semi.ncell;
+    c->Ne = 3LL * nb * c->mesh.elec.ncell;
+    c->NP = c->L.n;
+    LdgRegionOps os = build_ldg_ops(c->mesh.semi, T, c->P);
+    LdgRegionOps oe = build_ldg_ops(c->mesh.elec, T, c->P);
+    PoissonOps op = build_poisson_ops(c->mesh, c->L, T, c->P);
+    auto tf = std::chrono::steady_clock::now();
+    c->share_s = c->P.mu[0] == c->P.mu[1];
+    c->share_e = c->P.mu[2] == c->P.mu[3];
+    c->Fs[0] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[0]), leaf);
+    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[1]), leaf);
+    c->Fe[0] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[2]), leaf);
+    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[3]), leaf);
+    c->FH = make_factor(c.get(), multiplier_matrix(c->mesh, op, T), leaf);
+    c->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
+    // reference tables
+    DevTabs& tb = c->tabs;
+    tb.qx = upload(c.get(), T.qx);
+    tb.qy = upload(c.get(), T.qy);
+    tb.qw = upload(c.get(), T.qw);
+    tb.fw = upload(c.get(), T.fw);
+    tb.phi = upload(c.get(), T.phi);
+    tb.fphi = upload(c.get(), T.fphi);
+    tb.phiP = upload(c.get(), T.phiP);
+    tb.psix = upload(c.get(), T.psix);
+    tb.psiy = upload(c.get(), T.psiy);
+    // regions
+    c->Rs = make_region_dev(c.get(), c->mesh.semi, os, c->mesh.semi_sigma_nb, c->Fs[0]);
+    c->Re = make_region_dev(c.get(), c->mesh.elec, oe, c->mesh.elec_sigma_nb, c->Fe[0]);
+    // Poisson
+    DevPoisson& P = c->Pd;
+    const SolarCellMesh& m = c->mesh;
+    P.ncell = m.ncellP;
+    P.ncell_semi = m.semi.ncell;
+    P.nloc = op.nloc;
+    P.nrt = op.nrt;
+    P.nphi = op.nphi;
+    P.NX = m.NX;
+    P.NY = m.NY;
+    P.V = upload(c.get(), m.pV);
+    P.idx = upload(c.get(), c->L.idx);
+    P.type = upload(c.get(), op.type);
+    P.table = upload(c.get(), op.table);
+    P.stride = op.type_stride;
+    P.pbid = upload(c.get(), m.pbid);
+    P.pI = upload(c.get(), m.pI);
+    P.pJ = upload(c.get(), m.pJ);
+    P.mult_face = upload(c.get(), op.mult_face);
+    P.nmult = op.nmult;
+    std::vector<int> owner_of(op.nmult), mperm(op.nmult), mcell(2 * (size_t)op.nmult, -1), mloc(2 * (size_t)op.nmult, 0);
+    std::vector<double> mcoef(2 * (size_t)op.nmult, 0.0), gx, gw;
+    for (int G = 0; G < m.NX * m.NY; ++G)
+      for (int t = 0; t < op.ndof_owner[G]; ++t) {
+        owner_of[op.owner_base[G] + t] = G;
+        mperm[op.owner_base[G] + t] = c->FH.cell_base[G] + t;
+      }
+    gauss01(c->k, gx, gw);
+    for (int K = 0; K < m.ncellP; ++K)
+      for (int f = 0; f < 4; ++f) {
+        const int base = op.mult_face[4 * K + f];
+        if (base < 0) continue;
+        const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+        for (int mm = 0; mm < c->k; ++mm) {
+          const int id = base + mm;
+          const int e = (mcell[2 * id] < 0) ? 0 : 1;
+          mcell[2 * id + e] = K;
+          mloc[2 * id + e] = T.rt_face_dof(f, mm);
+          mcoef[2 * id + e] = sf * gw[mm];
+        }
+      }
+    P.mult_perm = upload(c.get(), mperm);
+    P.mcell = upload(c.get(), mcell);
+    P.mloc = upload(c.get(), mloc);
+    P.mcoef = upload(c.get(), mcoef);
+    P.gw = upload(c.get(), gw);
+    // physics
+    DevPhys& ph = c->ph;
+    ph.dt = c->P.dt;
+    ph.penalty = c->P.penalty;
+    ph.k_et = c->P.k_et;
+    ph.k_ht = c->P.k_ht;
+    ph.v_rec_n = c->P.v_rec_n;
+    ph.v_rec_p = c->P.v_rec_p;
+    ph.ne = c->P.ne;
+    ph.pe = c->P.pe;
+    ph.rinf = c->P.rinf;
+    ph.oinf = c->P.oinf;
+    ph.eps_s = c->P.eps_s;
+    ph.eps_e = c->P.eps_e;
+    ph.G0 = c->P.G0;
+    ph.alpha = c->P.alpha;
+    ph.H = m.geo.H;
+    ph.ni = c->P.ni;
+    ph.tau_n = c->P.tau_n;
+    ph.tau_p = c->P.tau_p;
+    ph.gen_mode = c->P.gen_mode;
+    ph.srh = c->P.srh;
+    // state and scratch
+    const int64_t nsv[4] = {c->Ns, c->Ns, c->Ne, c->Ne};
+    for (int i = 0; i < 4; ++i) {
+      c->u[i] = dalloc<double>(c.get(), nsv[i]);
+      c->b[i] = dalloc<double>(c.get(), nsv[i]);
+      c->W[i] = dalloc<double>(c.get(), 2LL * nb * (i < 2 ? m.semi.ncell : m.elec.ncell));
+      c->old[i] = dalloc<double>(c.get(), nsv[i]);
+      c->io[i] = dalloc<double>(c.get(), std::max(nsv[i], c->NP));
+      HIPCHECK(hipMemset(c->u[i], 0, nsv[i] * sizeof(double)));
+    }
+    c->pv = dalloc<double>(c.get(), c->NP);
+    c->old[4] = dalloc<double>(c.get(), c->NP);
+    HIPCHECK(hipMemset(c->pv, 0, c->NP * sizeof(double)));
+    for (int r = 0; r < 2; ++r) {
+      const int n = r == 0 ? c->Fs[0].n : c->Fe[0].n;
+      for (int j = 0; j < 2; ++j) {
+        c->g[r][j] = dalloc<double>(c.get(), n);
+        c->t[r][j] = dalloc<double>(c.get(), n);
+      }
+    }
+    c->yP = dalloc<double>(c.get(), (size_t)m.ncellP * op.nloc);
+    c->rH = dalloc<double>(c.get(), c->FH.n);
+    c->tH = dalloc<double>(c.get(), c->FH.n);
+    c->acc = dalloc<double>(c.get(), 2);
+    HIPCHECK(hipDeviceSynchronize());
+    c->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+  *out = c.release();
+  return 0;
+}
+
+int pecs_destroy(pecs_ctx* ctx) {
+  if (!ctx) return -1;
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+const char* pecs_last_error(pecs_ctx* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+int pecs_set_stream(pecs_ctx* ctx, void* s) {
+  if (!ctx) return -1;
+  ctx->stream = (hipStream_t)s;
+  return 0;
+}
+
+int pecs_sizes(pecs_ctx* ctx, int64_t* ns, int64_t* ne, int64_t* np) {
+  if (!ctx) return -1;
+  if (ns) *ns = ctx->Ns;
+  if (ne) *ne = ctx->Ne;
+  if (np) *np = ctx->NP;
+  return 0;
+}
+
+int pecs_stats(pecs_ctx* ctx, double* o, int n) {
+  if (!ctx || !o) return -1;
+  const double v[10] = {ctx->Fs[0].nnz + ctx->Fs[1].nnz, ctx->Fe[0].nnz + ctx->Fe[1].nnz, ctx->FH.nnz,
+                        ctx->setup_seconds, ctx->factor_seconds, (double)ctx->Fs[0].nlevels,
+                        (double)ctx->Fe[0].nlevels, (double)ctx->FH.nlevels, (double)ctx->Fs[0].n,
+                        (double)ctx->FH.n};
+  for (int i = 0; i < n && i < 10; ++i) o[i] = v[i];
+  return 0;
+}
+
+int pecs_export_matrix(const pecs_mesh* mesh, const pecs_params* params, int which, int64_t* n, int64_t* nnz,
+                       int64_t* rowptr, int32_t* col, double* val, int8_t* constrained) {
+  try {
+    validate(mesh, params);
+    Params P = to_params(params);
+    SolarCellMesh m = make_solarcell_mesh(mesh->n_refine, to_geo(mesh));
+    RefTables T(params->degree);
+    std::vector<int64_t> rp;
+    std::vector<int> cl;
+    std::vector<double> vl;
+    std::vector<char> cons;
+    if (which >= PECS_ELECTRONS && which <= PECS_OXIDANTS) {
+      const Region& R = which < 2 ? m.semi : m.elec;
+      LdgRegionOps O = build_ldg_ops(R, T, P);
+      ldg_csr(R, O, P.mu[which], P.dt, rp, cl, vl, false);
+    } else if (which == PECS_MASS_SEMI || which == PECS_MASS_ELEC) {
+      const Region& R = which == PECS_MASS_SEMI ? m.semi : m.elec;
+      LdgRegionOps O = build_ldg_ops(R, T, P);
+      ldg_csr(R, O, 1.0, P.dt, rp, cl, vl, true);
+    } else if (which == PECS_POISSON) {
+      PoissonLayout L = poisson_layout(m, params->degree - 1);
+      PoissonOps O = build_poisson_ops(m, L, T, P);
+      mixed_csr(m, L, O, rp, cl, vl);
+      cons = O.constrained;
+    } else {
+      throw std::runtime_error("unknown matrix id");
+    }
+    if (n) *n = (int64_t)rp.size() - 1;
+    if (nnz) *nnz = (int64_t)cl.size();
+    if (rowptr) {
+      std::memcpy(rowptr, rp.data(), rp.size() * sizeof(int64_t));
+      std::memcpy(col, cl.data(), cl.size() * sizeof(int32_t));
+      std::memcpy(val, vl.data(), vl.size() * sizeof(double));
+      if (constrained && !cons.empty())
+        for (size_t i = 0; i < cons.size(); ++i) constrained[i] = (int8_t)cons[i];
+    }
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+#define PECS_TRY(ctx, ...)                      \
+  if (!ctx) return -1;                          \
+  try {                                         \
+    __VA_ARGS__                                 \
+  } catch (const std::exception& e) {           \
+    return fail(ctx, e);                        \
+  }                                             \
+  return 0;
+
+static void h2d(pecs_ctx* c, double* d, const double* h, int64_t n) {
+  HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+}
+static void d2h(pecs_ctx* c, double* h, const double* d, int64_t n) {
+  HIPCHECK(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+}
+
+int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* un, const double* up, const double* pv,
+                               const double* ur, const double* uo, double* bn, double* bp) {
+  PECS_TRY(ctx, {
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->ph, un, up, ur,
+                                            uo, pv, bn, bp, ctx->stream));
+  })
+}
+
+int pecs_rhs_electrolyte_dev(pecs_ctx* ctx, const double* ur, const double* uo, const double* pv,
+                             const double* un, const double* up, double* br, double* bo) {
+  PECS_TRY(ctx, {
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->ph, ur, uo,
+                                            un, up, pv, br, bo, ctx->stream));
+  })
+}
+
+int pecs_rhs_poisson_dev(pecs_ctx* ctx, const double* un, const double* up, const double* ur, const double* uo,
+                         double* rhs) {
+  PECS_TRY(ctx, {
+    HIPCHECK(hipMemsetAsync(rhs, 0, ctx->NP * sizeof(double), ctx->stream));
+    HIPCHECK((hipError_t)launch_poisson_rhs_assemble(ctx->k, ctx->Pd, ctx->tabs, ctx->ph, un, up, ur, uo,
+                                                     ctx->mesh.semi.ncell, ctx->mesh.elec.ncell, rhs, ctx->stream));
+  })
+}
+
+int pecs_solve_dev(pecs_ctx* ctx, int which, const double* rhs, double* x) {
+  PECS_TRY(ctx, {
+    if (which >= PECS_ELECTRONS && which <= PECS_OXIDANTS) {
+      solve_carrier(ctx, which, rhs, x);
+    } else if (which == PECS_POISSON) {
+      HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, rhs, ctx->yP, ctx->stream));
+      poisson_solve_from_y(ctx, x);
+    } else {
+      throw std::runtime_error("unknown system id");
+    }
+  })
+}
+
+int pecs_rhs_semiconductor(pecs_ctx* ctx, const double* un, const double* up, const double* pv, const double* ur,
+                           const double* uo, double* bn, double* bp) {
+  PECS_TRY(ctx, {
+    h2d(ctx, ctx->io[0], un, ctx->Ns);
+    h2d(ctx, ctx->io[1], up, ctx->Ns);
+    h2d(ctx, ctx->io[2], ur, ctx->Ne);
+    h2d(ctx, ctx->io[3], uo, ctx->Ne);
+    h2d(ctx, ctx->old[4], pv, ctx->NP);
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->ph, ctx->io[0],
+                                            ctx->io[1], ctx->io[2], ctx->io[3], ctx->old[4], ctx->b[0], ctx->b[1],
+                                            ctx->stream));
+    d2h(ctx, bn, ctx->b[0], ctx->Ns);
+    d2h(ctx, bp, ctx->b[1], ctx->Ns);
+  })
+}
+
+int pecs_rhs_electrolyte(pecs_ctx* ctx, const double* ur, const double* uo, const double* pv, const double* un,
+                         const double* up, double* br, double* bo) {
+  PECS_TRY(ctx, {
+    h2d(ctx, ctx->io[0], un, ctx->Ns);
+    h2d(ctx, ctx->io[1], up, ctx->Ns);
+    h2d(ctx, ctx->io[2], ur, ctx->Ne);
+    h2d(ctx, ctx->io[3], uo, ctx->Ne);
+    h2d(ctx, ctx->old[4], pv, ctx->NP);
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->ph, ctx->io[2],
+                                            ctx->io[3], ctx->io[0], ctx->io[1], ctx->old[4], ctx->b[2], ctx->b[3],
+                                            ctx->stream));
+    d2h(ctx, br, ctx->b[2], ctx->Ne);
+    d2h(ctx, bo, ctx->b[3], ctx->Ne);
+  })
+}
+
+int pecs_rhs_poisson(pecs_ctx* ctx, const double* un, const double* up, const double* ur, const double* uo,
+                     double* rhs) {
+  PECS_TRY(ctx, {
+    h2d(ctx, ctx->io[0], un, ctx->Ns);
+    h2d(ctx, ctx->io[1], up, ctx->Ns);
+    h2d(ctx, ctx->io[2], ur, ctx->Ne);
+    h2d(ctx, ctx->io[3], uo, ctx->Ne);
+    HIPCHECK(hipMemsetAsync(ctx->old[4], 0, ctx->NP * sizeof(double), ctx->stream));
+    HIPCHECK((hipError_t)launch_poisson_rhs_assemble(ctx->k, ctx->Pd, ctx->tabs, ctx->ph, ctx->io[0], ctx->io[1],
+                                                     ctx->io[2], ctx->io[3], ctx->mesh.semi.ncell,
+                                                     ctx->mesh.elec.ncell, ctx->old[4], ctx->stream));
+    d2h(ctx, rhs, ctx->old[4], ctx->NP);
+  })
+}
+
+int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x) {
+  PECS_TRY(ctx, {
+    int64_t n = 0;
+    if (which == PECS_ELECTRONS || which == PECS_HOLES) n = ctx->Ns;
+    else if (which == PECS_REDUCTANTS || which == PECS_OXIDANTS) n = ctx->Ne;
+    else if (which == PECS_POISSON) n = ctx->NP;
+    else throw std::runtime_error("unknown system id");
+    h2d(ctx, ctx->io[0], rhs, n);
+    if (which == PECS_POISSON) {
+      HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, ctx->io[0], ctx->yP, ctx->stream));
+      poisson_solve_from_y(ctx, ctx->io[1]);
+    } else {
+      solve_carrier(ctx, which, ctx->io[0], ctx->io[1]);
+    }
+    d2h(ctx, x, ctx->io[1], n);
+  })
+}
+
+int pecs_set_state(pecs_ctx* ctx, const double* un, const double* up, const double* ur, const double* uo,
+                   const double* pv) {
+  PECS_TRY(ctx, {
+    h2d(ctx, ctx->u[0], un, ctx->Ns);
+    h2d(ctx, ctx->u[1], up, ctx->Ns);
+    h2d(ctx, ctx->u[2], ur, ctx->Ne);
+    h2d(ctx, ctx->u[3], uo, ctx->Ne);
+    if (pv) h2d(ctx, ctx->pv, pv, ctx->NP);
+    else poisson_from_densities(ctx, ctx->u[0], ctx->u[1], ctx->u[2], ctx->u[3], ctx->pv);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  })
+}
+
+int pecs_get_state(pecs_ctx* ctx, double* un, double* up, double* ur, double* uo, double* pv) {
+  PECS_TRY(ctx, {
+    if (un) d2h(ctx, un, ctx->u[0], ctx->Ns);
+    if (up) d2h(ctx, up, ctx->u[1], ctx->Ns);
+    if (ur) d2h(ctx, ur, ctx->u[2], ctx->Ne);
+    if (uo) d2h(ctx, uo, ctx->u[3], ctx->Ne);
+    if (pv) d2h(ctx, pv, ctx->pv, ctx->NP);
+  })
+}
+
+int pecs_initial_state(pecs_ctx* ctx) {
+  PECS_TRY(ctx, {
+    // VectorTools::project of the constant equilibria (SolarCell.cpp:1999-2021):
+    // the L2 projection of a constant onto FE_DGQ is that constant, J = 0.
+    const int nb = ctx->T->nb;
+    const double vals[4] = {ctx->P.ne, ctx->P.pe, ctx->P.rinf, ctx->P.oinf};
+    const int64_t nsv[4] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne};
+    for (int i = 0; i < 4; ++i) {
+      std::vector<double> h(nsv[i], 0.0);
+      const int64_t n3 = nsv[i] / 3;
+      for (int64_t j = 2 * n3; j < nsv[i]; ++j) h[j] = vals[i];
+      HIPCHECK(hipMemcpy(ctx->u[i], h.data(), nsv[i] * sizeof(double), hipMemcpyHostToDevice));
+    }
+    (void)nb;
+    poisson_from_densities(ctx, ctx->u[0], ctx->u[1], ctx->u[2], ctx->u[3], ctx->pv);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  })
+}
+
+int pecs_step(pecs_ctx* ctx, int nsteps) {
+  PECS_TRY(ctx, {
+    if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
+    for (int i = 0; i < nsteps; ++i) {
+      if (i == nsteps - 1) {
+        const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
+        for (int j = 0; j < 4; ++j)
+          HIPCHECK(hipMemcpyAsync(ctx->old[j], ctx->u[j], nsv[j] * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        HIPCHECK(hipMemcpyAsync(ctx->old[4], ctx->pv, nsv[4] * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      }
+      step_once(ctx);
+    }
+    if (nsteps > 0) {
+      const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
+      HIPCHECK(hipMemsetAsync(ctx->acc, 0, 2 * sizeof(double), ctx->stream));
+      for (int j = 0; j < 4; ++j)
+        HIPCHECK((hipError_t)launch_diff_norms(ctx->u[j], ctx->old[j], nsv[j], ctx->acc, ctx->stream));
+      HIPCHECK((hipError_t)launch_diff_norms(ctx->pv, ctx->old[4], nsv[4], ctx->acc, ctx->stream));
+      double h[2];
+      HIPCHECK(hipMemcpyAsync(h, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+      ctx->last_res = (h[1] > 0) ? std::sqrt(h[0] / h[1]) / ctx->P.dt : 0.0;
+    }
+  })
+}
+
+int pecs_last_residual(pecs_ctx* ctx, double* r) {
+  if (!ctx || !r) return -1;
+  *r = ctx->last_res;
+  return 0;
+}
+
+int pecs_state_dev(pecs_ctx* ctx, double** un, double** up, double** ur, double** uo, double** pv) {
+  if (!ctx) return -1;
+  if (un) *un = ctx->u[0];
+  if (up) *up = ctx->u[1];
+  if (ur) *ur = ctx->u[2];
+  if (uo) *uo = ctx->u[3];
+  if (pv) *pv = ctx->pv;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,837 @@
+// HIP kernels of the PECS per-time-step IMEX path for gfx950 (CDNA4).
+//
+// Per step (SolarCell.cpp:2057-2075):
+//   k_rhs_carrier   assemble_{semiconductor,electrolyte}_rhs: one thread per
+//                   cell, geometry rebuilt from the 4 vertices (MappingQ1),
+//                   old densities, Poisson field (Piola) and lagged traces of
+//                   the other region evaluated at QGauss(k+2) points.
+//   k_schur_prep    W = A^{-1} f_p (cell-local inverse mass, times mobility).
+//   k_schur_rhs     g = f_v + B^T W (cell + 4 face neighbours).
+//   k_fwd_* / k_bwd_*  supernodal triangular solves with S = L L^T, one
+//                   wavefront per factor row, level by level on the
+//                   elimination tree (deterministic, no atomics).
+//   k_schur_recover q = W - A^{-1} B u; writes [Jx | Jy | u].
+//   k_poisson_*     hybridised mixed Poisson: cell condensation, face
+//                   assembly, multiplier solve, cell recovery.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace pecs {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+  return x;
+}
+
+struct GeoD {
+  double x, y, J00, J01, J10, J11, det;
+};
+
+__device__ __forceinline__ GeoD bilin(const double* v, double xi, double eta) {
+  GeoD g;
+  const double a = (1 - xi) * (1 - eta), b = xi * (1 - eta), c = (1 - xi) * eta, d = xi * eta;
+  g.x = v[0] * a + v[2] * b + v[4] * c + v[6] * d;
+  g.y = v[1] * a + v[3] * b + v[5] * c + v[7] * d;
+  g.J00 = (v[2] - v[0]) * (1 - eta) + (v[6] - v[4]) * eta;
+  g.J10 = (v[3] - v[1]) * (1 - eta) + (v[7] - v[5]) * eta;
+  g.J01 = (v[4] - v[0]) * (1 - xi) + (v[6] - v[2]) * xi;
+  g.J11 = (v[5] - v[1]) * (1 - xi) + (v[7] - v[3]) * xi;
+  g.det = g.J00 * g.J11 - g.J01 * g.J10;
+  return g;
+}
+
+__device__ __forceinline__ void face_nrm(const double* v, int f, double& nx, double& ny, double& len) {
+  int a, b;
+  if (f == 0) { a = 0; b = 2; }
+  else if (f == 1) { a = 1; b = 3; }
+  else if (f == 2) { a = 0; b = 1; }
+  else { a = 2; b = 3; }
+  const double tx = v[2 * b] - v[2 * a], ty = v[2 * b + 1] - v[2 * a + 1];
+  len = sqrt(tx * tx + ty * ty);
+  if (f == 1 || f == 2) { nx = ty / len; ny = -tx / len; }
+  else { nx = -ty / len; ny = tx / len; }
+}
+
+template <int K>
+struct Sz {
+  static constexpr int NB = (K + 1) * (K + 1);
+  static constexpr int NQ = (K + 2) * (K + 2);
+  static constexpr int NF = K + 2;
+  static constexpr int NRT = 2 * K * (K + 1);
+  static constexpr int NRTX = K * (K + 1);
+  static constexpr int NPHI = K * K;
+  static constexpr int NL = NRT + NPHI;
+};
+
+// local RT index of face node m of face f (RT_{K-1} nodal basis)
+template <int K>
+__device__ __forceinline__ int rt_face(int f, int m) {
+  if (f == 0) return (K + 1) * m;
+  if (f == 1) return K + (K + 1) * m;
+  if (f == 2) return Sz<K>::NRTX + m;
+  return Sz<K>::NRTX + m + K * K;
+}
+
+// ------------------------------------------------------------------------
+// Carrier right-hand sides (SolarCell.cpp:1073-1414 and 1451-1726)
+// ------------------------------------------------------------------------
+template <int K, bool SEMI>
+__global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, DevPoisson P, DevTabs T, DevPhys ph,
+                                                    const double* __restrict__ u1, const double* __restrict__ u2,
+                                                    const double* __restrict__ ou1, const double* __restrict__ ou2,
+                                                    const double* __restrict__ pvec, double* __restrict__ b1,
+                                                    double* __restrict__ b2) {
+  using S = Sz<K>;
+  constexpr int NB = S::NB, NQ = S::NQ, NF = S::NF, NRT = S::NRT;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R.ncell) return;
+  const int nc = R.ncell;
+  double v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) v[t] = R.V[8 * (size_t)c + t];
+  double d1[NB], d2[NB];
+  const size_t ub = 2 * (size_t)nc * NB + (size_t)c * NB;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    d1[a] = u1[ub + a];
+    d2[a] = u2[ub + a];
+  }
+  const int pc = SEMI ? c : P.ncell_semi + c;
+  double dp[NRT];
+#pragma unroll
+  for (int i = 0; i < NRT; ++i) dp[i] = pvec[P.idx[(size_t)pc * P.nloc + i]];
+  double a1p[2 * NB], a1v[NB], a2p[2 * NB], a2v[NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    a1p[a] = a1p[NB + a] = a1v[a] = 0.0;
+    a2p[a] = a2p[NB + a] = a2v[a] = 0.0;
+  }
+  const double inv_perm = 1.0 / (SEMI ? ph.eps_s : ph.eps_e);
+  const double idt = 1.0 / ph.dt;
+  for (int q = 0; q < NQ; ++q) {
+    const GeoD g = bilin(v, T.qx[q], T.qy[q]);
+    const double jxw = T.qw[q] * g.det;
+    double n1 = 0.0, n2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      const double ph_a = T.phi[a * NQ + q];
+      n1 += d1[a] * ph_a;
+      n2 += d2[a] * ph_a;
+    }
+    double hx = 0.0, hy = 0.0;
+#pragma unroll
+    for (int i = 0; i < NRT; ++i) {
+      hx += dp[i] * T.psix[i * NQ + q];
+      hy += dp[i] * T.psiy[i * NQ + q];
+    }
+    const double Ex = (g.J00 * hx + g.J01 * hy) / g.det;
+    const double Ey = (g.J10 * hx + g.J11 * hy) / g.det;
+    double src = 0.0;
+    if (SEMI) {
+      if (ph.gen_mode == 0) src = ph.alpha * ph.G0 * exp(ph.alpha * (g.y - ph.H));
+      else if (ph.gen_mode == 1) src = ph.G0 * exp(-ph.alpha * g.x);
+      if (ph.srh)
+        src += (ph.ni * ph.ni - n1 * n2) / (ph.tau_n * (n1 + ph.ni) + ph.tau_p * (n2 + ph.ni));
+    }
+    const double sv1 = jxw * (n1 * idt + src), sv2 = jxw * (n2 * idt + src);
+    const double cp1 = -1.0 * inv_perm * n1 * jxw, cp2 = 1.0 * inv_perm * n2 * jxw;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      const double ph_a = T.phi[a * NQ + q];
+      a1v[a] += sv1 * ph_a;
+      a2v[a] += sv2 * ph_a;
+      a1p[a] += cp1 * Ex * ph_a;
+      a1p[NB + a] += cp1 * Ey * ph_a;
+      a2p[a] += cp2 * Ex * ph_a;
+      a2p[NB + a] += cp2 * Ey * ph_a;
+    }
+  }
+  const double hc = R.h[c];
+  for (int f = 0; f < 4; ++f) {
+    const int bid = R.bid[4 * (size_t)c + f];
+    if (bid < 0) continue;
+    if (bid == 2) continue;  // Neumann: nothing (SolarCell.cpp:1403-1406)
+    if (!SEMI && bid == 3) continue;
+    double nx, ny, len;
+    face_nrm(v, f, nx, ny, len);
+    const double* fph = T.fphi + (size_t)f * NB * NF;
+    if (bid == 1) {  // Dirichlet
+      const double e1 = SEMI ? ph.ne : ph.rinf, e2 = SEMI ? ph.pe : ph.oinf;
+      const double pen = ph.penalty / hc;
+      for (int q = 0; q < NF; ++q) {
+        const double jw = T.fw[q] * len;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          const double fa = fph[a * NF + q];
+          a1p[a] -= fa * nx * jw * e1;
+          a1p[NB + a] -= fa * ny * jw * e1;
+          a1v[a] += pen * fa * jw * e1;
+          a2p[a] -= fa * nx * jw * e2;
+          a2p[NB + a] -= fa * ny * jw * e2;
+          a2v[a] += pen * fa * jw * e2;
+        }
+      }
+      continue;
+    }
+    // interface (0) and Schottky (3) use the own traces
+    double o1[NB], o2[NB];
+    const double* foph = T.fphi + (size_t)(f ^ 1) * NB * NF;
+    if (bid == 0) {
+      const int e = R.sigma_nb[c];
+      if (e < 0) continue;
+      const size_t ob = 2 * (size_t)O.ncell * NB + (size_t)e * NB;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        o1[a] = ou1[ob + a];
+        o2[a] = ou2[ob + a];
+      }
+    }
+    for (int q = 0; q < NF; ++q) {
+      const double jw = T.fw[q] * len;
+      double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        t1 += d1[a] * fph[a * NF + q];
+        t2 += d2[a] * fph[a * NF + q];
+      }
+      double w1, w2;
+      if (bid == 0) {
+        double r1 = 0.0, r2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          r1 += o1[a] * foph[a * NF + q];
+          r2 += o2[a] * foph[a * NF + q];
+        }
+        if (SEMI) {
+          w1 = -ph.k_et * (t1 - ph.ne) * r2 * jw;  // -v k_et (n - n^e) o
+          w2 = ph.k_ht * (t2 - ph.pe) * r1 * jw;   // +v k_ht (p - p^e) r
+        } else {
+          const double cur = (-ph.k_et * (r1 - ph.ne) * t2 + ph.k_ht * (r2 - ph.pe) * t1) * jw;
+          w1 = cur;
+          w2 = -cur;
+        }
+      } else {  // Schottky (SolarCell.cpp:1348-1402)
+        w1 = -ph.v_rec_n * (t1 - ph.ne) * jw;
+        w2 = ph.v_rec_p * (t2 - ph.pe) * jw;
+      }
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        const double fa = fph[a * NF + q];
+        a1v[a] += fa * w1;
+        a2v[a] += fa * w2;
+      }
+    }
+  }
+  const size_t jxo = (size_t)c * NB, jyo = (size_t)nc * NB + (size_t)c * NB, uo = 2 * (size_t)nc * NB + (size_t)c * NB;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    b1[jxo + a] = a1p[a];
+    b1[jyo + a] = a1p[NB + a];
+    b1[uo + a] = a1v[a];
+    b2[jxo + a] = a2p[a];
+    b2[jyo + a] = a2p[NB + a];
+    b2[uo + a] = a2v[a];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Schur complement transforms
+// ------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(64) void k_schur_prep(DevRegion R, const double* __restrict__ b, double mu,
+                                                   double* __restrict__ W) {
+  constexpr int NB = Sz<K>::NB;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R.ncell) return;
+  const size_t nc = R.ncell;
+  const double* Minv = R.table + (size_t)R.type[c] * R.stride;
+  double fx[NB], fy[NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    fx[a] = b[(size_t)c * NB + a];
+    fy[a] = b[nc * NB + (size_t)c * NB + a];
+  }
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      const double m = Minv[a * NB + t];
+      sx += m * fx[t];
+      sy += m * fy[t];
+    }
+    W[(size_t)c * 2 * NB + a] = mu * sx;
+    W[(size_t)c * 2 * NB + NB + a] = mu * sy;
+  }
+}
+
+template <int K, int NRHS>
+__global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __restrict__ b1,
+                                                  const double* __restrict__ b2, const double* __restrict__ W1,
+                                                  const double* __restrict__ W2, double* __restrict__ g1,
+                                                  double* __restrict__ g2) {
+  constexpr int NB = Sz<K>::NB;
+  const int X = blockIdx.x * blockDim.x + threadIdx.x;
+  if (X >= R.ncell) return;
+  const size_t nc = R.ncell;
+  double a1[NB], a2[NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    a1[a] = b1[2 * nc * NB + (size_t)X * NB + a];
+    a2[a] = (NRHS > 1) ? b2[2 * nc * NB + (size_t)X * NB + a] : 0.0;
+  }
+  {
+    const double* B = R.table + (size_t)R.type[X] * R.stride + NB * NB;
+    for (int i = 0; i < 2 * NB; ++i) {
+      const double w1 = W1[(size_t)X * 2 * NB + i];
+      const double w2 = (NRHS > 1) ? W2[(size_t)X * 2 * NB + i] : 0.0;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        a1[a] += B[i * NB + a] * w1;
+        if (NRHS > 1) a2[a] += B[i * NB + a] * w2;
+      }
+    }
+  }
+  for (int f = 0; f < 4; ++f) {
+    const int M = R.nbr[4 * (size_t)X + f];
+    if (M < 0) continue;
+    const double* B = R.table + (size_t)R.type[M] * R.stride + NB * NB + 2 * NB * NB + (f ^ 1) * 2 * NB * NB;
+    for (int i = 0; i < 2 * NB; ++i) {
+      const double w1 = W1[(size_t)M * 2 * NB + i];
+      const double w2 = (NRHS > 1) ? W2[(size_t)M * 2 * NB + i] : 0.0;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        a1[a] += B[i * NB + a] * w1;
+        if (NRHS > 1) a2[a] += B[i * NB + a] * w2;
+      }
+    }
+  }
+  const int pb = R.perm_base[X];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    g1[pb + a] = a1[a];
+    if (NRHS > 1) g2[pb + a] = a2[a];
+  }
+}
+
+template <int K, int NRHS>
+__global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double* __restrict__ W1,
+                                                      const double* __restrict__ W2, const double* __restrict__ x1,
+                                                      const double* __restrict__ x2, double mu1, double mu2,
+                                                      double* __restrict__ o1, double* __restrict__ o2) {
+  constexpr int NB = Sz<K>::NB;
+  const int M = blockIdx.x * blockDim.x + threadIdx.x;
+  if (M >= R.ncell) return;
+  const size_t nc = R.ncell;
+  const double* tb = R.table + (size_t)R.type[M] * R.stride;
+  double s1[2 * NB], s2[2 * NB], u1[NB], u2[NB];
+  const int pb = R.perm_base[M];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    u1[a] = x1[pb + a];
+    u2[a] = (NRHS > 1) ? x2[pb + a] : 0.0;
+  }
+  for (int i = 0; i < 2 * NB; ++i) {
+    double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      t1 += tb[NB * NB + i * NB + a] * u1[a];
+      if (NRHS > 1) t2 += tb[NB * NB + i * NB + a] * u2[a];
+    }
+    s1[i] = t1;
+    s2[i] = t2;
+  }
+  for (int f = 0; f < 4; ++f) {
+    const int X = R.nbr[4 * (size_t)M + f];
+    if (X < 0) continue;
+    const int px = R.perm_base[X];
+    double v1[NB], v2[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      v1[a] = x1[px + a];
+      v2[a] = (NRHS > 1) ? x2[px + a] : 0.0;
+    }
+    const double* B = tb + NB * NB + 2 * NB * NB + f * 2 * NB * NB;
+    for (int i = 0; i < 2 * NB; ++i) {
+      double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        t1 += B[i * NB + a] * v1[a];
+        if (NRHS > 1) t2 += B[i * NB + a] * v2[a];
+      }
+      s1[i] += t1;
+      s2[i] += t2;
+    }
+  }
+  const double* Minv = tb;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        t1 += Minv[a * NB + t] * s1[h * NB + t];
+        if (NRHS > 1) t2 += Minv[a * NB + t] * s2[h * NB + t];
+      }
+      const size_t off = (size_t)h * nc * NB + (size_t)M * NB + a;
+      o1[off] = W1[(size_t)M * 2 * NB + h * NB + a] - mu1 * t1;
+      if (NRHS > 1) o2[off] = W2[(size_t)M * 2 * NB + h * NB + a] - mu2 * t2;
+    }
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    o1[2 * nc * NB + (size_t)M * NB + a] = u1[a];
+    if (NRHS > 1) o2[2 * nc * NB + (size_t)M * NB + a] = u2[a];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Supernodal triangular solves, one wavefront per row
+// ------------------------------------------------------------------------
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int2* __restrict__ tasks, int ntask,
+                                                    const double* __restrict__ v0, const double* __restrict__ v1,
+                                                    double* __restrict__ t0, double* __restrict__ t1) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
+  if (w >= ntask) return;
+  const int2 tk = tasks[w];
+  const int p = F.p0[tk.x] + tk.y;
+  double a0 = 0.0, a1 = 0.0;
+  const int64_t c1 = F.cptr[p + 1];
+  for (int64_t c = F.cptr[p]; c < c1; ++c) {
+    const int d = F.cnode[c];
+    const int nsd = F.ns[d];
+    const double* row = F.L21 + F.l21_off[d] + (int64_t)F.crow[c] * nsd;
+    const int base = F.p0[d];
+    for (int j = lane; j < nsd; j += WAVE) {
+      const double L = row[j];
+      a0 += L * v0[base + j];
+      if (NRHS > 1) a1 += L * v1[base + j];
+    }
+  }
+  a0 = wave_sum(a0);
+  if (NRHS > 1) a1 = wave_sum(a1);
+  if (lane == 0) {
+    t0[p] = v0[p] - a0;
+    if (NRHS > 1) t1[p] = v1[p] - a1;
+  }
+}
+
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_fwd_tri(DevFactor F, const int2* __restrict__ tasks, int ntask,
+                                                 const double* __restrict__ t0, const double* __restrict__ t1,
+                                                 double* __restrict__ v0, double* __restrict__ v1) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
+  if (w >= ntask) return;
+  const int2 tk = tasks[w];
+  const int s = tk.x, r = tk.y, ns = F.ns[s], p0 = F.p0[s];
+  const double* row = F.Linv + F.linv_off[s] + (int64_t)r * ns;
+  double a0 = 0.0, a1 = 0.0;
+  for (int j = lane; j <= r; j += WAVE) {
+    const double L = row[j];
+    a0 += L * t0[p0 + j];
+    if (NRHS > 1) a1 += L * t1[p0 + j];
+  }
+  a0 = wave_sum(a0);
+  if (NRHS > 1) a1 = wave_sum(a1);
+  if (lane == 0) {
+    v0[p0 + r] = a0;
+    if (NRHS > 1) v1[p0 + r] = a1;
+  }
+}
+
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_bwd_gather(DevFactor F, const int2* __restrict__ tasks, int ntask,
+                                                    const double* __restrict__ v0, const double* __restrict__ v1,
+                                                    double* __restrict__ t0, double* __restrict__ t1) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
+  if (w >= ntask) return;
+  const int2 tk = tasks[w];
+  const int s = tk.x, j = tk.y, ms = F.ms[s], p = F.p0[s] + j;
+  const double* row = F.L21T + F.l21_off[s] + (int64_t)j * ms;
+  const int* rows = F.rows + F.rows_off[s];
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = lane; i < ms; i += WAVE) {
+    const double L = row[i];
+    const int ri = rows[i];
+    a0 += L * v0[ri];
+    if (NRHS > 1) a1 += L * v1[ri];
+  }
+  a0 = wave_sum(a0);
+  if (NRHS > 1) a1 = wave_sum(a1);
+  if (lane == 0) {
+    t0[p] = v0[p] - a0;
+    if (NRHS > 1) t1[p] = v1[p] - a1;
+  }
+}
+
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_bwd_tri(DevFactor F, const int2* __restrict__ tasks, int ntask,
+                                                 const double* __restrict__ t0, const double* __restrict__ t1,
+                                                 double* __restrict__ v0, double* __restrict__ v1) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
+  if (w >= ntask) return;
+  const int2 tk = tasks[w];
+  const int s = tk.x, j = tk.y, ns = F.ns[s], p0 = F.p0[s];
+  const double* row = F.LinvT + F.linv_off[s] + (int64_t)j * ns;
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = j + lane; i < ns; i += WAVE) {
+    const double L = row[i];
+    a0 += L * t0[p0 + i];
+    if (NRHS > 1) a1 += L * t1[p0 + i];
+  }
+  a0 = wave_sum(a0);
+  if (NRHS > 1) a1 = wave_sum(a1);
+  if (lane == 0) {
+    v0[p0 + j] = a0;
+    if (NRHS > 1) v1[p0 + j] = a1;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Hybridised mixed Poisson
+// ------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void poisson_fphi(const DevPoisson& P, const DevTabs& T, const DevPhys& ph, int Kc,
+                                             const double* un, const double* up, const double* ur,
+                                             const double* uo, int ncs, int nce, double* fphi) {
+  using S = Sz<K>;
+  constexpr int NB = S::NB, NQ = S::NQ, NPHI = S::NPHI;
+  const bool semi = Kc < ncs;
+  const int c = semi ? Kc : Kc - ncs;
+  const int nc = semi ? ncs : nce;
+  const double* u1 = semi ? un : ur;
+  const double* u2 = semi ? up : uo;
+  const double rho0 = semi ? (ph.ne - ph.pe) : 0.0;  // N_D - N_A (SolarCell.cpp:542-548)
+  double v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) v[t] = P.V[8 * (size_t)Kc + t];
+  double d1[NB], d2[NB];
+  const size_t ub = 2 * (size_t)nc * NB + (size_t)c * NB;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    d1[a] = u1[ub + a];
+    d2[a] = u2[ub + a];
+  }
+#pragma unroll
+  for (int l = 0; l < NPHI; ++l) fphi[l] = 0.0;
+  for (int q = 0; q < NQ; ++q) {
+    const GeoD g = bilin(v, T.qx[q], T.qy[q]);
+    const double jxw = T.qw[q] * g.det;
+    double n1 = 0.0, n2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      n1 += d1[a] * T.phi[a * NQ + q];
+      n2 += d2[a] * T.phi[a * NQ + q];
+    }
+    const double rho = rho0 + (-1.0 * n1 + 1.0 * n2);
+#pragma unroll
+    for (int l = 0; l < NPHI; ++l) fphi[l] -= T.phiP[l * NQ + q] * rho * jxw;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, DevPhys ph, const double* __restrict__ un,
+                                                     const double* __restrict__ up, const double* __restrict__ ur,
+                                                     const double* __restrict__ uo, int ncs, int nce,
+                                                     double* __restrict__ y) {
+  using S = Sz<K>;
+  constexpr int NRT = S::NRT, NL = S::NL, NPHI = S::NPHI;
+  const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Kc >= P.ncell) return;
+  double f[NL];
+  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+#pragma unroll
+  for (int i = 0; i < NRT; ++i) f[i] = tb[NL * NL + i];
+  poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, f + NRT);
+  for (int i = 0; i < NL; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) s += tb[i * NL + j] * f[j];
+    y[(size_t)Kc * NL + i] = s;
+  }
+  (void)NPHI;
+}
+
+// face DoF ownership for the continuous D layout: left/bottom faces belong to
+// the cell; right/top faces only on the right/top boundary.
+template <int K>
+__device__ __forceinline__ int face_of_local(int i, int& m) {
+  constexpr int NRTX = Sz<K>::NRTX;
+  if (i < NRTX) {
+    const int a = i % (K + 1), b = i / (K + 1);
+    m = b;
+    if (a == 0) return 0;
+    if (a == K) return 1;
+    return -1;
+  }
+  const int ii = i - NRTX, a = ii % K, b = ii / K;
+  m = a;
+  if (b == 0) return 2;
+  if (b == K) return 3;
+  return -1;
+}
+
+template <int K>
+__device__ __forceinline__ bool owns_face(const DevPoisson& P, int Kc, int f) {
+  if (f == 0 || f == 2) return true;
+  if (f == 1) return P.pI[Kc] == P.NX - 1;
+  return P.pJ[Kc] == P.NY - 1;
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_cell_from_b(DevPoisson P, const double* __restrict__ b,
+                                                            double* __restrict__ y) {
+  constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
+  const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Kc >= P.ncell) return;
+  const int* id = P.idx + (size_t)Kc * NL;
+  double f[NL];
+  for (int i = 0; i < NL; ++i) {
+    int m;
+    const int fc = (i < NRT) ? face_of_local<K>(i, m) : -1;
+    f[i] = (fc < 0 || owns_face<K>(P, Kc, fc)) ? b[id[i]] : 0.0;
+  }
+  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  for (int i = 0; i < NL; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) s += tb[i * NL + j] * f[j];
+    y[(size_t)Kc * NL + i] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_poisson_face(DevPoisson P, const double* __restrict__ y,
+                                                      double* __restrict__ r) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= P.nmult) return;
+  double s = 0.0;
+  for (int e = 0; e < 2; ++e) {
+    const int c = P.mcell[2 * m + e];
+    if (c >= 0) s += P.mcoef[2 * m + e] * y[(size_t)c * P.nloc + P.mloc[2 * m + e]];
+  }
+  r[P.mult_perm[m]] = s;
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_recover(DevPoisson P, const double* __restrict__ lam,
+                                                        const double* __restrict__ y, double* __restrict__ out) {
+  constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
+  const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Kc >= P.ncell) return;
+  double e[4 * K];
+  int ej[4 * K];
+  int ne = 0;
+  for (int f = 0; f < 4; ++f) {
+    const int base = P.mult_face[4 * (size_t)Kc + f];
+    if (base < 0) continue;
+    const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+    for (int m = 0; m < K; ++m) {
+      e[ne] = sf * P.gw[m] * lam[P.mult_perm[base + m]];
+      ej[ne] = rt_face<K>(f, m);
+      ++ne;
+    }
+  }
+  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  const int* id = P.idx + (size_t)Kc * NL;
+  for (int i = 0; i < NL; ++i) {
+    double s = y[(size_t)Kc * NL + i];
+    for (int t = 0; t < ne; ++t) s -= tb[i * NL + ej[t]] * e[t];
+    if (i < NRT) {
+      int m;
+      const int fc = face_of_local<K>(i, m);
+      if (fc >= 0) {
+        if (!owns_face<K>(P, Kc, fc)) continue;
+        if (P.pbid[4 * (size_t)Kc + fc] == 2) s = 0.0;  // Neumann D.n = 0 (Poisson.cpp:41-47)
+      }
+    }
+    out[id[i]] = s;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTabs T, DevPhys ph,
+                                                             const double* __restrict__ un,
+                                                             const double* __restrict__ up,
+                                                             const double* __restrict__ ur,
+                                                             const double* __restrict__ uo, int ncs, int nce,
+                                                             double* __restrict__ b) {
+  constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT, NPHI = Sz<K>::NPHI;
+  const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Kc >= P.ncell) return;
+  double fphi[NPHI];
+  poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, fphi);
+  const int* id = P.idx + (size_t)Kc * NL;
+  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  for (int i = 0; i < NRT; ++i) {
+    int m;
+    const int fc = face_of_local<K>(i, m);
+    if (fc >= 0 && !owns_face<K>(P, Kc, fc)) continue;
+    b[id[i]] = tb[NL * NL + i];
+  }
+#pragma unroll
+  for (int l = 0; l < NPHI; ++l) b[id[NRT + l]] = fphi[l];
+}
+
+__global__ __launch_bounds__(256) void k_diff_norms(const double* __restrict__ a, const double* __restrict__ b,
+                                                    int64_t n, double* acc) {
+  __shared__ double s0[256], s1[256];
+  double d = 0.0, o = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double t = a[i] - b[i];
+    d += t * t;
+    o += b[i] * b[i];
+  }
+  s0[threadIdx.x] = d;
+  s1[threadIdx.x] = o;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[0], s0[0]);
+    atomicAdd(&acc[1], s1[0]);
+  }
+}
+
+inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+#define PECS_DISPATCH_K(k, CALL) \
+  switch (k) {                   \
+    case 1: { constexpr int KK = 1; CALL; } break; \
+    case 2: { constexpr int KK = 2; CALL; } break; \
+    case 3: { constexpr int KK = 3; CALL; } break; \
+    default: return (int)hipErrorInvalidValue;      \
+  }
+
+int launch_rhs_carrier(int k, bool semi, const DevRegion& R, const DevRegion& O, const DevPoisson& P,
+                       const DevTabs& T, const DevPhys& ph, const double* u1, const double* u2, const double* ou1,
+                       const double* ou2, const double* pvec, double* b1, double* b2, hipStream_t s) {
+  if (R.ncell == 0) return 0;
+  if (semi) {
+    PECS_DISPATCH_K(k, (k_rhs_carrier<KK, true><<<nblk(R.ncell, 64), 64, 0, s>>>(R, O, P, T, ph, u1, u2, ou1, ou2,
+                                                                                  pvec, b1, b2)));
+  } else {
+    PECS_DISPATCH_K(k, (k_rhs_carrier<KK, false><<<nblk(R.ncell, 64), 64, 0, s>>>(R, O, P, T, ph, u1, u2, ou1, ou2,
+                                                                                   pvec, b1, b2)));
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_schur_prep(int k, const DevRegion& R, const double* b, double mu, double* W, hipStream_t s) {
+  PECS_DISPATCH_K(k, (k_schur_prep<KK><<<nblk(R.ncell, 64), 64, 0, s>>>(R, b, mu, W)));
+  return (int)hipGetLastError();
+}
+
+int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, const double* W1,
+                     const double* W2, double* g1, double* g2, int nrhs, hipStream_t s) {
+  if (nrhs == 2) {
+    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 2><<<nblk(R.ncell, 64), 64, 0, s>>>(R, b1, b2, W1, W2, g1, g2)));
+  } else {
+    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 1><<<nblk(R.ncell, 64), 64, 0, s>>>(R, b1, b1, W1, W1, g1, g1)));
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_schur_recover(int k, const DevRegion& R, const double* W1, const double* W2, const double* xs1,
+                         const double* xs2, double mu1, double mu2, double* out1, double* out2, const double*,
+                         const double*, int nrhs, hipStream_t s) {
+  if (nrhs == 2) {
+    PECS_DISPATCH_K(k, (k_schur_recover<KK, 2><<<nblk(R.ncell, 64), 64, 0, s>>>(R, W1, W2, xs1, xs2, mu1, mu2, out1,
+                                                                               out2)));
+  } else {
+    PECS_DISPATCH_K(k, (k_schur_recover<KK, 1><<<nblk(R.ncell, 64), 64, 0, s>>>(R, W1, W1, xs1, xs1, mu1, mu1, out1,
+                                                                               out1)));
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_chol_solve(const DevFactor& F, const int* off, int nlevels, double* v0, double* v1, double* t0,
+                      double* t1, int nrhs, hipStream_t s) {
+  for (int l = 0; l < nlevels; ++l) {
+    const int nt = off[l + 1] - off[l];
+    if (nt == 0) continue;
+    const int2* tk = F.tasks + off[l];
+    const int nb = nblk((int64_t)nt * WAVE, 256);
+    if (nrhs == 2) {
+      k_fwd_gather<2><<<nb, 256, 0, s>>>(F, tk, nt, v0, v1, t0, t1);
+      k_fwd_tri<2><<<nb, 256, 0, s>>>(F, tk, nt, t0, t1, v0, v1);
+    } else {
+      k_fwd_gather<1><<<nb, 256, 0, s>>>(F, tk, nt, v0, v0, t0, t0);
+      k_fwd_tri<1><<<nb, 256, 0, s>>>(F, tk, nt, t0, t0, v0, v0);
+    }
+  }
+  for (int l = nlevels - 1; l >= 0; --l) {
+    const int nt = off[l + 1] - off[l];
+    if (nt == 0) continue;
+    const int2* tk = F.tasks + off[l];
+    const int nb = nblk((int64_t)nt * WAVE, 256);
+    if (nrhs == 2) {
+      k_bwd_gather<2><<<nb, 256, 0, s>>>(F, tk, nt, v0, v1, t0, t1);
+      k_bwd_tri<2><<<nb, 256, 0, s>>>(F, tk, nt, t0, t1, v0, v1);
+    } else {
+      k_bwd_gather<1><<<nb, 256, 0, s>>>(F, tk, nt, v0, v0, t0, t0);
+      k_bwd_tri<1><<<nb, 256, 0, s>>>(F, tk, nt, t0, t0, v0, v0);
+    }
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
+                        const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
+                        hipStream_t s) {
+  PECS_DISPATCH_K(k, (k_poisson_cell<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, T, ph, un, up, ur, uo, ncs, nce, y)));
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_cell_from_b(const DevPoisson& P, const double* b, double* y, hipStream_t s) {
+  const int k = (P.nphi == 1) ? 1 : (P.nphi == 4 ? 2 : 3);
+  PECS_DISPATCH_K(k, (k_poisson_cell_from_b<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, b, y)));
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s) {
+  if (P.nmult == 0) return 0;
+  k_poisson_face<<<nblk(P.nmult, 256), 256, 0, s>>>(P, y, r);
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
+                           hipStream_t s) {
+  PECS_DISPATCH_K(k, (k_poisson_recover<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, lam, y, out)));
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
+                                const double* up, const double* ur, const double* uo, int ncs, int nce, double* b,
+                                hipStream_t s) {
+  PECS_DISPATCH_K(k, (k_poisson_rhs_assemble<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, T, ph, un, up, ur, uo, ncs, nce,
+                                                                                  b)));
+  return (int)hipGetLastError();
+}
+
+int launch_diff_norms(const double* a, const double* b, int64_t n, double* acc, hipStream_t s) {
+  int nb = nblk(n, 256);
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  k_diff_norms<<<nb, 256, 0, s>>>(a, b, n, acc);
+  return (int)hipGetLastError();
+}
+
+}  // namespace pecs

@@ -1,0 +1,483 @@
+// Constant local operators (host side, setup); see ops.hpp.
+#include "ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace pecs {
+
+static const int OPP[4] = {1, 0, 3, 2};
+static const double BETA = 0.70710678118654752440;  // (1,1)/sqrt(2), LDG.cpp:455-458
+
+bool dense_inverse(int n, const double* A, double* Ainv) {
+  std::vector<double> a(A, A + (size_t)n * n);
+  std::vector<int> piv(n);
+  for (int i = 0; i < n * n; ++i) Ainv[i] = 0.0;
+  for (int i = 0; i < n; ++i) Ainv[i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::fabs(a[r * n + c]) > std::fabs(a[p * n + c])) p = r;
+    if (a[p * n + c] == 0.0) return false;
+    if (p != c)
+      for (int t = 0; t < n; ++t) {
+        std::swap(a[p * n + t], a[c * n + t]);
+        std::swap(Ainv[p * n + t], Ainv[c * n + t]);
+      }
+    const double inv = 1.0 / a[c * n + c];
+    for (int t = 0; t < n; ++t) {
+      a[c * n + t] *= inv;
+      Ainv[c * n + t] *= inv;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[r * n + c];
+      if (f == 0.0) continue;
+      for (int t = 0; t < n; ++t) {
+        a[r * n + t] -= f * a[c * n + t];
+        Ainv[r * n + t] -= f * Ainv[c * n + t];
+      }
+    }
+  }
+  return true;
+}
+
+static uint64_t fnv1a(const double* p, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n * sizeof(double); ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// De-duplicate per-cell records of `stride` doubles (bitwise identical cells
+// share one table entry).
+static void dedup(const std::vector<double>& rec, int ncell, int stride, std::vector<int>& type,
+                  std::vector<double>& table, int& ntype) {
+  std::unordered_map<uint64_t, std::vector<int>> seen;
+  type.assign(ncell, -1);
+  table.clear();
+  ntype = 0;
+  for (int c = 0; c < ncell; ++c) {
+    const double* r = &rec[(size_t)c * stride];
+    uint64_t h = fnv1a(r, stride);
+    auto& cand = seen[h];
+    int found = -1;
+    for (int t : cand)
+      if (std::memcmp(&table[(size_t)t * stride], r, stride * sizeof(double)) == 0) {
+        found = t;
+        break;
+      }
+    if (found < 0) {
+      found = ntype++;
+      table.insert(table.end(), r, r + stride);
+      cand.push_back(found);
+    }
+    type[c] = found;
+  }
+}
+
+LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P) {
+  const int nb = T.nb, nq = T.nq, nfq = T.nfq;
+  LdgRegionOps O;
+  O.nb = nb;
+  O.ncell = R.ncell;
+  O.cell.resize(R.ncell);
+  std::vector<double> hdiam(R.ncell);
+  for (int c = 0; c < R.ncell; ++c) hdiam[c] = cell_diameter(&R.V[8 * c]);
+  for (int c = 0; c < R.ncell; ++c) {
+    const double* v = &R.V[8 * c];
+    const double h = hdiam[c];
+    LdgCellOps& L = O.cell[c];
+    L.Mk.assign(nb * nb, 0);
+    L.Minv.assign(nb * nb, 0);
+    L.Bself.assign(2 * nb * nb, 0);
+    L.Bf.assign(4 * 2 * nb * nb, 0);
+    L.Dself.assign(nb * nb, 0);
+    L.Df.assign(4 * nb * nb, 0);
+    for (int q = 0; q < nq; ++q) {
+      Geo g = bilinear(v, T.qx[q], T.qy[q]);
+      const double jxw = T.qw[q] * g.det;
+      for (int a = 0; a < nb; ++a) {
+        const double pa = T.phi[a * nq + q];
+        const double dx = (g.J11 * T.dphx[a * nq + q] - g.J10 * T.dphy[a * nq + q]) / g.det;
+        const double dy = (-g.J01 * T.dphx[a * nq + q] + g.J00 * T.dphy[a * nq + q]) / g.det;
+        for (int b = 0; b < nb; ++b) {
+          const double pb = T.phi[b * nq + q];
+          L.Mk[a * nb + b] += jxw * pa * pb;
+          L.Bself[a * nb + b] -= jxw * dx * pb;         // -(div p_i, u_j), p_i = (phi_a, 0)
+          L.Bself[(nb + a) * nb + b] -= jxw * dy * pb;  // p_i = (0, phi_a)
+        }
+      }
+    }
+    for (int f = 0; f < 4; ++f) {
+      double nx, ny, len;
+      face_normal(v, f, nx, ny, len);
+      const int e = R.nbr[4 * c + f];
+      const int bid = R.bid[4 * c + f];
+      const double* ps = &T.fphi[(size_t)f * nb * nfq];
+      const double* po = &T.fphi[(size_t)OPP[f] * nb * nfq];
+      const double sig = (e >= 0) ? P.penalty / std::min(h, hdiam[e]) : P.penalty / h;
+      for (int q = 0; q < nfq; ++q) {
+        const double jw = T.fw[q] * len;
+        for (int a = 0; a < nb; ++a) {
+          const double pa = ps[a * nfq + q];
+          for (int b = 0; b < nb; ++b) {
+            const double pbs = ps[b * nfq + q];
+            if (e >= 0) {
+              const double pbo = po[b * nfq + q];
+              L.Bself[a * nb + b] += jw * (0.5 * pa * nx + BETA * pa) * pbs;
+              L.Bself[(nb + a) * nb + b] += jw * (0.5 * pa * ny + BETA * pa) * pbs;
+              L.Bf[(f * 2 * nb + a) * nb + b] += jw * (0.5 * pa * nx - BETA * pa) * pbo;
+              L.Bf[(f * 2 * nb + nb + a) * nb + b] += jw * (0.5 * pa * ny - BETA * pa) * pbo;
+              L.Dself[a * nb + b] += sig * jw * pa * pbs;
+              L.Df[(f * nb + a) * nb + b] -= sig * jw * pa * pbo;
+            } else if (bid == DIRICHLET) {
+              L.Dself[a * nb + b] += sig * jw * pa * pbs;
+            } else {  // Interface, Neumann, Schottky: p.n u (LDG.cpp:236-272)
+              L.Bself[a * nb + b] += jw * pa * nx * pbs;
+              L.Bself[(nb + a) * nb + b] += jw * pa * ny * pbs;
+            }
+          }
+        }
+      }
+    }
+    for (int t = 0; t < nb * nb; ++t) L.Dself[t] += (P.theta / P.dt) * L.Mk[t];
+    if (!dense_inverse(nb, L.Mk.data(), L.Minv.data())) throw std::runtime_error("singular cell mass matrix");
+  }
+  O.type_stride = nb * nb + 5 * 2 * nb * nb;
+  std::vector<double> rec((size_t)R.ncell * O.type_stride);
+  for (int c = 0; c < R.ncell; ++c) {
+    double* r = &rec[(size_t)c * O.type_stride];
+    const LdgCellOps& L = O.cell[c];
+    std::copy(L.Minv.begin(), L.Minv.end(), r);
+    std::copy(L.Bself.begin(), L.Bself.end(), r + nb * nb);
+    std::copy(L.Bf.begin(), L.Bf.end(), r + nb * nb + 2 * nb * nb);
+  }
+  dedup(rec, R.ncell, O.type_stride, O.type, O.table, O.ntype);
+  return O;
+}
+
+static int offset_index(const BlockSPD& S, int di, int dj) {
+  for (int o = 0; o < S.noff(); ++o)
+    if (S.offi[o] == di && S.offj[o] == dj) return o;
+  return -1;
+}
+
+BlockSPD schur_matrix(const Region& R, const LdgRegionOps& ops, double mu) {
+  const int nb = ops.nb, N = R.N;
+  BlockSPD S;
+  S.NXc = N;
+  S.NYc = N;
+  S.ndof.assign(N * N, nb);
+  S.sep_width = 2;
+  for (int dj = -2; dj <= 2; ++dj)
+    for (int di = -2; di <= 2; ++di)
+      if (std::abs(di) + std::abs(dj) <= 2) {
+        S.offi.push_back(di);
+        S.offj.push_back(dj);
+      }
+  const int no = S.noff();
+  S.blk.assign((size_t)N * N * no, {});
+  auto block = [&](int X, int Y) -> double* {
+    int gx = R.ci[X] + N * R.cj[X];
+    int o = offset_index(S, R.ci[Y] - R.ci[X], R.cj[Y] - R.cj[X]);
+    auto& b = S.blk[(size_t)gx * no + o];
+    if (b.empty()) b.assign(nb * nb, 0.0);
+    return b.data();
+  };
+  std::vector<double> Pm(2 * nb * nb);
+  for (int M = 0; M < R.ncell; ++M) {
+    const LdgCellOps& L = ops.cell[M];
+    int cells[5];
+    const double* Bs[5];
+    int nn = 0;
+    cells[nn] = M;
+    Bs[nn++] = L.Bself.data();
+    for (int f = 0; f < 4; ++f)
+      if (R.nbr[4 * M + f] >= 0) {
+        cells[nn] = R.nbr[4 * M + f];
+        Bs[nn++] = &L.Bf[(size_t)f * 2 * nb * nb];
+      }
+    for (int y = 0; y < nn; ++y) {
+      // Pm = blockdiag(Minv, Minv) * B_{M,Y}
+      for (int half = 0; half < 2; ++half)
+        for (int a = 0; a < nb; ++a)
+          for (int b = 0; b < nb; ++b) {
+            double s = 0;
+            for (int t = 0; t < nb; ++t) s += L.Minv[a * nb + t] * Bs[y][(half * nb + t) * nb + b];
+            Pm[(half * nb + a) * nb + b] = s;
+          }
+      for (int x = 0; x < nn; ++x) {
+        double* blk = block(cells[x], cells[y]);
+        for (int a = 0; a < nb; ++a)
+          for (int b = 0; b < nb; ++b) {
+            double s = 0;
+            for (int i = 0; i < 2 * nb; ++i) s += Bs[x][i * nb + a] * Pm[i * nb + b];
+            blk[a * nb + b] += mu * s;
+          }
+      }
+    }
+    double* d = block(M, M);
+    for (int t = 0; t < nb * nb; ++t) d[t] += L.Dself[t];
+    for (int f = 0; f < 4; ++f) {
+      int e = R.nbr[4 * M + f];
+      if (e < 0) continue;
+      double* b = block(M, e);
+      for (int t = 0; t < nb * nb; ++t) b[t] += L.Df[(size_t)f * nb * nb + t];
+    }
+  }
+  return S;
+}
+
+struct CsrBuilder {
+  std::vector<std::map<int, double>> rows;
+  explicit CsrBuilder(int n) : rows(n) {}
+  void add(int r, int c, double v) { rows[r][c] += v; }
+  void finish(std::vector<int64_t>& rowptr, std::vector<int>& col, std::vector<double>& val) {
+    rowptr.assign(rows.size() + 1, 0);
+    col.clear();
+    val.clear();
+    for (size_t r = 0; r < rows.size(); ++r) {
+      for (auto& kv : rows[r]) {
+        col.push_back(kv.first);
+        val.push_back(kv.second);
+      }
+      rowptr[r + 1] = (int64_t)col.size();
+    }
+  }
+};
+
+void ldg_csr(const Region& R, const LdgRegionOps& ops, double mu, double dt, std::vector<int64_t>& rowptr,
+             std::vector<int>& col, std::vector<double>& val, bool mass_only) {
+  const int nb = ops.nb, nc = R.ncell;
+  CsrBuilder C(3 * nb * nc);
+  auto jx = [&](int c, int a) { return c * nb + a; };
+  auto jy = [&](int c, int a) { return nc * nb + c * nb + a; };
+  auto uu = [&](int c, int a) { return 2 * nc * nb + c * nb + a; };
+  for (int M = 0; M < nc; ++M) {
+    const LdgCellOps& L = ops.cell[M];
+    if (mass_only) {
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b) C.add(uu(M, a), uu(M, b), L.Mk[a * nb + b] / dt);
+      continue;
+    }
+    for (int a = 0; a < nb; ++a)
+      for (int b = 0; b < nb; ++b) {
+        C.add(jx(M, a), jx(M, b), L.Mk[a * nb + b] / mu);
+        C.add(jy(M, a), jy(M, b), L.Mk[a * nb + b] / mu);
+        C.add(uu(M, a), uu(M, b), L.Dself[a * nb + b]);
+      }
+    auto addB = [&](const double* B, int X) {
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b) {
+          C.add(jx(M, a), uu(X, b), B[a * nb + b]);
+          C.add(jy(M, a), uu(X, b), B[(nb + a) * nb + b]);
+          C.add(uu(X, b), jx(M, a), -B[a * nb + b]);  // C = -B^T
+          C.add(uu(X, b), jy(M, a), -B[(nb + a) * nb + b]);
+        }
+    };
+    addB(L.Bself.data(), M);
+    for (int f = 0; f < 4; ++f) {
+      int e = R.nbr[4 * M + f];
+      if (e < 0) continue;
+      addB(&L.Bf[(size_t)f * 2 * nb * nb], e);
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b) C.add(uu(M, a), uu(e, b), L.Df[(size_t)(f * nb + a) * nb + b]);
+    }
+  }
+  C.finish(rowptr, col, val);
+}
+
+PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, const RefTables& T, const Params& P) {
+  PoissonOps O;
+  const int nrt = T.nrt, nphi = T.nphi, nl = T.nlocP, nq = T.nq, nfq = T.nfq;
+  O.nloc = nl;
+  O.nrt = nrt;
+  O.nphi = nphi;
+  O.k = T.k;
+  const int nc = m.ncellP;
+  O.Ainv.assign((size_t)nc * nl * nl, 0);
+  O.A.assign((size_t)nc * nl * nl, 0);
+  O.fpsi.assign((size_t)nc * nrt, 0);
+  std::vector<double> psx(nrt), psy(nrt);
+  for (int K = 0; K < nc; ++K) {
+    const double* v = &m.pV[8 * K];
+    const double eps = (m.pmat[K] == SEMI_ID || m.pmat[K] == SEMI_BL_ID) ? P.eps_s : P.eps_e;
+    double* A = &O.A[(size_t)K * nl * nl];
+    for (int q = 0; q < nq; ++q) {
+      Geo g = bilinear(v, T.qx[q], T.qy[q]);
+      const double jxw = T.qw[q] * g.det;
+      for (int i = 0; i < nrt; ++i) {
+        const double hx = T.psix[i * nq + q], hy = T.psiy[i * nq + q];
+        psx[i] = (g.J00 * hx + g.J01 * hy) / g.det;
+        psy[i] = (g.J10 * hx + g.J11 * hy) / g.det;
+      }
+      for (int i = 0; i < nrt; ++i) {
+        const double di = T.divpsi[i * nq + q] / g.det;
+        for (int j = 0; j < nrt; ++j) A[i * nl + j] += jxw * (psx[i] * psx[j] + psy[i] * psy[j]) / eps;
+        for (int l = 0; l < nphi; ++l) {
+          const double pl = T.phiP[l * nq + q];
+          A[i * nl + nrt + l] -= jxw * di * pl;           // -div(psi_i) phi_j
+          A[(nrt + l) * nl + i] -= P.lam2 * jxw * pl * di;  // -lambda^2 phi_i div(psi_j)
+        }
+      }
+    }
+    if (!dense_inverse(nl, A, &O.Ainv[(size_t)K * nl * nl])) throw std::runtime_error("singular local mixed matrix");
+    // constant Dirichlet / Schottky data of semiconductor cells (SolarCell.cpp:593-681)
+    if (m.pmat[K] == SEMI_ID || m.pmat[K] == SEMI_BL_ID) {
+      for (int f = 0; f < 4; ++f) {
+        const int bid = m.pbid[4 * K + f];
+        if (bid != DIRICHLET && bid != SCHOTTKY) continue;
+        double nx, ny, len;
+        face_normal(v, f, nx, ny, len);
+        for (int q = 0; q < nfq; ++q) {
+          double xi, eta;
+          face_point(f, T.fs[q], xi, eta);
+          Geo g = bilinear(v, xi, eta);
+          const double jw = T.fw[q] * len;
+          const double bi = (g.x == 0.0) ? P.phi_bi : 0.0;  // BiasValues.cpp:11-27
+          double other;
+          if (bid == DIRICHLET) other = (g.x == 0.0) ? P.phi_app : 0.0;  // BiasValues.cpp:74-91
+          else other = (P.schottky_location_set && g.y == P.schottky_location) ? P.phi_sch : 0.0;
+          const double datum = bi - other;
+          if (datum == 0.0) continue;
+          for (int i = 0; i < nrt; ++i) {
+            const double hx = T.fpsix[(f * nrt + i) * nfq + q], hy = T.fpsiy[(f * nrt + i) * nfq + q];
+            const double px = (g.J00 * hx + g.J01 * hy) / g.det, py = (g.J10 * hx + g.J11 * hy) / g.det;
+            O.fpsi[(size_t)K * nrt + i] += -(px * nx + py * ny) * datum * jw;
+          }
+        }
+      }
+    }
+  }
+  // multipliers: interior and Neumann faces
+  const int kf = T.deg + 1;
+  auto is_mult = [&](int K, int f) {
+    int b = m.pbid[4 * K + f];
+    return b == INTERIOR || b == NEUMANN;
+  };
+  const int NG = m.NX * m.NY;
+  O.ndof_owner.assign(NG, 0);
+  O.owner_base.assign(NG, 0);
+  std::vector<int> own_face(4 * (size_t)nc, -1);  // owned face multiplier base
+  int cnt = 0;
+  for (int G = 0; G < NG; ++G) {
+    const int K = m.pcell_of[G];
+    const int I = m.pI[K], J = m.pJ[K];
+    O.owner_base[G] = cnt;
+    const int order[4] = {0, 2, 1, 3};
+    for (int t = 0; t < 4; ++t) {
+      const int f = order[t];
+      bool owned = false;
+      if (f == 0 || f == 2) owned = is_mult(K, f);
+      if (f == 1) owned = (I == m.NX - 1) && is_mult(K, f);
+      if (f == 3) owned = (J == m.NY - 1) && is_mult(K, f);
+      if (owned) {
+        own_face[4 * K + f] = cnt;
+        cnt += kf;
+      }
+    }
+    O.ndof_owner[G] = cnt - O.owner_base[G];
+  }
+  O.nmult = cnt;
+  O.mult_face.assign(4 * (size_t)nc, -1);
+  for (int K = 0; K < nc; ++K) {
+    const int I = m.pI[K], J = m.pJ[K];
+    for (int f = 0; f < 4; ++f) {
+      if (!is_mult(K, f)) continue;
+      int base = own_face[4 * K + f];
+      if (base < 0) {
+        if (f == 1) base = own_face[4 * m.pcell_of[(I + 1) + m.NX * J] + 0];
+        if (f == 3) base = own_face[4 * m.pcell_of[I + m.NX * (J + 1)] + 2];
+      }
+      if (base < 0) throw std::runtime_error("multiplier ownership");
+      O.mult_face[4 * K + f] = base;
+    }
+  }
+  O.constrained.assign(Lay.n, 0);
+  for (int K = 0; K < nc; ++K)
+    for (int f = 0; f < 4; ++f)
+      if (m.pbid[4 * K + f] == NEUMANN)
+        for (int mm = 0; mm < kf; ++mm) O.constrained[Lay.idx[(size_t)K * nl + T.rt_face_dof(f, mm)]] = 1;
+  O.type_stride = nl * nl + nrt;
+  std::vector<double> rec((size_t)nc * O.type_stride);
+  for (int K = 0; K < nc; ++K) {
+    std::copy(&O.Ainv[(size_t)K * nl * nl], &O.Ainv[(size_t)(K + 1) * nl * nl], &rec[(size_t)K * O.type_stride]);
+    std::copy(&O.fpsi[(size_t)K * nrt], &O.fpsi[(size_t)(K + 1) * nrt], &rec[(size_t)K * O.type_stride + nl * nl]);
+  }
+  dedup(rec, nc, O.type_stride, O.type, O.table, O.ntype);
+  return O;
+}
+
+BlockSPD multiplier_matrix(const SolarCellMesh& m, const PoissonOps& O, const RefTables& T) {
+  BlockSPD S;
+  S.NXc = m.NX;
+  S.NYc = m.NY;
+  S.ndof = O.ndof_owner;
+  S.sep_width = 1;
+  const int oi[7] = {0, 1, -1, 0, 0, 1, -1}, oj[7] = {0, 0, 0, 1, -1, -1, 1};
+  for (int o = 0; o < 7; ++o) {
+    S.offi.push_back(oi[o]);
+    S.offj.push_back(oj[o]);
+  }
+  const int no = 7, nl = O.nloc, kf = T.deg + 1;
+  S.blk.assign((size_t)m.NX * m.NY * no, {});
+  std::vector<double> gw, gx;
+  gauss01(kf, gx, gw);
+  // owner grid cell of a multiplier index
+  std::vector<int> owner_of(O.nmult);
+  for (int G = 0; G < m.NX * m.NY; ++G)
+    for (int t = 0; t < O.ndof_owner[G]; ++t) owner_of[O.owner_base[G] + t] = G;
+  for (int K = 0; K < m.ncellP; ++K) {
+    const double* Ai = &O.Ainv[(size_t)K * nl * nl];
+    for (int f = 0; f < 4; ++f) {
+      const int bf = O.mult_face[4 * K + f];
+      if (bf < 0) continue;
+      const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+      for (int g = 0; g < 4; ++g) {
+        const int bg = O.mult_face[4 * K + g];
+        if (bg < 0) continue;
+        const double sg = (g == 1 || g == 3) ? 1.0 : -1.0;
+        const int Gf = owner_of[bf], Gg = owner_of[bg];
+        const int di = (Gg % m.NX) - (Gf % m.NX), dj = (Gg / m.NX) - (Gf / m.NX);
+        int o = -1;
+        for (int t = 0; t < no; ++t)
+          if (S.offi[t] == di && S.offj[t] == dj) o = t;
+        if (o < 0) throw std::runtime_error("multiplier coupling outside stencil");
+        auto& blk = S.blk[(size_t)Gf * no + o];
+        const int rf = S.ndof[Gf], rg = S.ndof[Gg];
+        if (blk.empty()) blk.assign((size_t)rf * rg, 0.0);
+        for (int a = 0; a < kf; ++a)
+          for (int b = 0; b < kf; ++b) {
+            const int ra = bf - O.owner_base[Gf] + a, cb = bg - O.owner_base[Gg] + b;
+            blk[(size_t)ra * rg + cb] +=
+                sf * sg * gw[a] * gw[b] * Ai[T.rt_face_dof(f, a) * nl + T.rt_face_dof(g, b)];
+          }
+      }
+    }
+  }
+  return S;
+}
+
+void mixed_csr(const SolarCellMesh& m, const PoissonLayout& L, const PoissonOps& O, std::vector<int64_t>& rowptr,
+               std::vector<int>& col, std::vector<double>& val) {
+  CsrBuilder C(L.n);
+  const int nl = O.nloc;
+  for (int K = 0; K < m.ncellP; ++K) {
+    const int* id = &L.idx[(size_t)K * nl];
+    const double* A = &O.A[(size_t)K * nl * nl];
+    for (int i = 0; i < nl; ++i)
+      for (int j = 0; j < nl; ++j)
+        if (A[i * nl + j] != 0.0) C.add(id[i], id[j], A[i * nl + j]);
+  }
+  C.finish(rowptr, col, val);
+}
+
+}  // namespace pecs

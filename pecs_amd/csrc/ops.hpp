@@ -1,0 +1,116 @@
+// Constant local operators of the LDG and mixed systems (host side, setup).
+//
+// LDG (LDG.cpp:40-678).  Per carrier the reference assembles
+//     [ A   B ] [q]   [f_p]     A = mu^{-1} (p, q)          (block diagonal)
+//     [ C   D ] [u] = [f_v]     B = -(div p, u) + face terms
+//                               C = -(grad v, q) + face terms
+//                               D = (theta/dt)(v, u) + penalty terms
+// and solves it with UMFPACK (Carrier.cpp:26-40).  Written per cell with its
+// own outward normal n_M the interior-face terms of LDG.cpp:461-621 are
+//     p-rows: 1/2 p.n_M (u_s + u_o) + (beta.p)(u_s - u_o)
+//     v-rows: v [1/2 n_M.(q_s + q_o) - beta.(q_s - q_o) + sigma_F (u_s - u_o)]
+// (the lower-index rule of LDG.cpp:390-391 does not change the matrix), and
+// integrating the cell term of C by parts shows C = -B^T exactly.  With A SPD
+// and D symmetric positive definite the density Schur complement
+//     S = D + B^T A^{-1} B
+// is SPD; it is factored once by sparse Cholesky and each step solves
+//     S u = f_v + B^T A^{-1} f_p,   q = A^{-1}(f_p - B u),
+// which is the same solution as the reference's direct solve.
+//
+// Mixed Poisson (MixedFEM.cpp:29-164, Poisson.cpp:18-105) is hybridised: the
+// RT space is broken per cell and normal continuity (and D.n = 0 on Neumann
+// faces) is imposed by face multipliers lambda.  Eliminating each cell,
+//     H lambda = sum_K E_K A_K^{-1} f_K,   H = sum_K E_K A_K^{-1} E_K^T  (SPD)
+// and x_K = A_K^{-1}(f_K - E_K^T lambda) recovers the same (D, Phi) as the
+// reference's continuous mixed solve.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fe.hpp"
+#include "mesh.hpp"
+
+namespace pecs {
+
+// Scaled parameters of one problem instance (Parameters.hpp:25-80 roles).
+struct Params {
+  int k = 1;
+  double dt = 1e-3, theta = 1.0, penalty = 1.0;
+  double mu[4] = {1, 1, 1, 1};  // electrons, holes, reductants, oxidants
+  double eps_s = 1, eps_e = 1, lam2 = 1;
+  double k_et = 1e-2, k_ht = 1e-2, v_rec_n = 0, v_rec_p = 0;
+  double ne = 1, pe = 0, rinf = 1, oinf = 1;
+  double phi_bi = 0, phi_app = 0, phi_sch = 0, schottky_location = 0;
+  int schottky_location_set = 0;
+  int gen_mode = 0;  // 0: reference alpha*G0*exp(alpha*(y-H)); 1: G0*exp(-alpha*x); 2: off
+  double G0 = 1, alpha = 1;
+  int srh = 0;
+  double ni = 0.1, tau_n = 1, tau_p = 1;
+};
+
+// Dense per-cell LDG operators (row-major).
+struct LdgCellOps {
+  std::vector<double> Minv;   // nb x nb, inverse of the density mass matrix
+  std::vector<double> Bself;  // 2nb x nb
+  std::vector<double> Bf;     // 4 x (2nb x nb), coupling to the face neighbour
+  std::vector<double> Dself;  // nb x nb
+  std::vector<double> Df;     // 4 x (nb x nb)
+  std::vector<double> Mk;     // nb x nb mass
+};
+
+struct LdgRegionOps {
+  int nb = 0, ncell = 0;
+  std::vector<LdgCellOps> cell;  // per Morton cell
+  // de-duplicated device table: per type Minv | Bself | Bf[4]
+  std::vector<int> type;
+  std::vector<double> table;
+  int ntype = 0, type_stride = 0;
+};
+
+LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P);
+
+// Block-sparse SPD matrix on a structured cell grid (input to the Cholesky).
+struct BlockSPD {
+  int NXc = 0, NYc = 0;
+  std::vector<int> ndof;          // per grid cell gid = i + NXc*j
+  std::vector<int> offi, offj;    // coupling offsets
+  std::vector<std::vector<double>> blk;  // [gid*noff + o]: ndof[gid] x ndof[nbr]
+  int sep_width = 1;
+  int noff() const { return (int)offi.size(); }
+};
+
+BlockSPD schur_matrix(const Region& R, const LdgRegionOps& ops, double mu);
+
+// Full constant LDG matrix [[A,B],[C,D]] in CSR (reference layout) and the mass
+// matrix, rebuilt from the local operators (for parity checks and callers that
+// want the constant matrices, SolarCell.cpp:822-932).
+void ldg_csr(const Region& R, const LdgRegionOps& ops, double mu, double dt, std::vector<int64_t>& rowptr,
+             std::vector<int>& col, std::vector<double>& val, bool mass_only);
+
+struct PoissonOps {
+  int nloc = 0, nrt = 0, nphi = 0, k = 0;
+  std::vector<double> Ainv;   // per cell nloc x nloc
+  std::vector<double> fpsi;   // per cell nrt (constant Dirichlet / Schottky part)
+  std::vector<double> A;      // per cell nloc x nloc (kept for CSR export)
+  // multipliers
+  std::vector<int> mult_face;  // per cell, 4 faces: multiplier index base (global, unpermuted) or -1
+  int nmult = 0;
+  std::vector<int> ndof_owner;      // per grid cell: number of owned multiplier DoFs
+  std::vector<int> owner_base;      // per grid cell: first owned multiplier index
+  // de-duplicated device table: per type Ainv | fpsi
+  std::vector<int> type;
+  std::vector<double> table;
+  int ntype = 0, type_stride = 0;
+  std::vector<char> constrained;  // per global Poisson DoF: Neumann D.n constraint
+};
+
+PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& L, const RefTables& T, const Params& P);
+BlockSPD multiplier_matrix(const SolarCellMesh& m, const PoissonOps& ops, const RefTables& T);
+void mixed_csr(const SolarCellMesh& m, const PoissonLayout& L, const PoissonOps& ops, std::vector<int64_t>& rowptr,
+               std::vector<int>& col, std::vector<double>& val);
+
+// small dense helpers
+bool dense_inverse(int n, const double* A, double* Ainv);
+
+}  // namespace pecs
