@@ -129,6 +129,17 @@ int pecs_step(pecs_ctx* ctx, int nsteps);
  * SolarCell.cpp:2042-2046). */
 int pecs_last_residual(pecs_ctx* ctx, double* residual);
 
+/* Per-phase timing of pecs_step with hipEvents on the context stream
+ * (the TimerOutput sections of SolarCell.cpp:1905-2087): phases 0 rhs semi,
+ * 1 rhs elec, 2 Schur transform semi, 3 Cholesky semi, 4 recover semi,
+ * 5-7 the same for the electrolyte, 8 Poisson condensation, 9 Poisson
+ * multiplier Cholesky, 10 Poisson recovery.  ms[] accumulates since enable. */
+int pecs_enable_timing(pecs_ctx* ctx, int enable);
+int pecs_phase_times(pecs_ctx* ctx, double* ms, int n, int* steps);
+/* Algorithmic bytes of one forward + backward triangular sweep with a factor
+ * (0 semiconductor Schur, 1 electrolyte Schur, 2 Poisson multipliers). */
+int pecs_factor_bytes(pecs_ctx* ctx, int which, double* bytes);
+
 /* Device-pointer variants (no copies, no synchronisation). */
 int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,
                                const double* u_r, const double* u_o, double* rhs_n, double* rhs_p);

@@ -117,10 +117,16 @@ class OracleSolarCell:
         self.Ne = 3 * nb * m.elec.ncell
 
     def factor(self):
+        # The LDG matrices have a positive definite symmetric part (the
+        # off-diagonal blocks satisfy C = -B^T), so LU without pivoting on a
+        # minimum-degree ordering of A + A^T is stable; the mixed matrix is a
+        # saddle point and keeps SuperLU's threshold pivoting.
         for name in ("n", "p", "r", "o"):
-            self.lu[name] = spla.splu(getattr(self, "A_" + name).tocsc())
+            self.lu[name] = spla.splu(getattr(self, "A_" + name).tocsc(), permc_spec="MMD_AT_PLUS_A",
+                                      diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
         Af = self.A_P[self.free][:, self.free]
-        self.lu["P"] = spla.splu(Af.tocsc())
+        self.lu["P"] = spla.splu(Af.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.1,
+                                 options=dict(SymmetricMode=True))
 
     def solve(self, which, b):
         if which == "P":

@@ -96,6 +96,38 @@ def lib():
 GEN_MODES = {"ref": 0, "config": 1, "off": 2}
 
 
+def default_params():
+    """Scaled parameters of the measured configurations (unit square split at x = 0.5).
+
+    Keys: k (LDG degree), r (global refinement), r1, r2, H, L (Grid.cpp
+    geometry), insulated, schottky (boundary passes), dt, theta, penalty,
+    mu_{n,p,r,o}, eps_s, eps_e, lam2 (Debye length^2), k_et, k_ht, v_rec_{n,p},
+    ne, pe, rinf, oinf (equilibria / doping), phi_bi, phi_app, phi_sch,
+    gen_mode ('ref' | 'config' | 'off'), G0, alpha, srh, ni, tau_n, tau_p.
+    """
+    return dict(
+        k=1, r=4, r1=0.5, r2=0.5, H=1.0, L=1.0, insulated=False, schottky=False,
+        dt=1e-3, penalty=1.0, theta=1.0,
+        mu_n=1.0, mu_p=1.0, mu_r=1.0, mu_o=1.0,
+        eps_s=1.0, eps_e=1.0, lam2=1.0,
+        k_et=1e-2, k_ht=1e-2, v_rec_n=0.0, v_rec_p=0.0,
+        ne=1.0, pe=0.0, rinf=1.0, oinf=1.0,
+        phi_bi=0.0, phi_app=0.0, phi_sch=0.0,
+        gen_mode="ref", G0=1.0, alpha=1.0,
+        srh=False, ni=0.1, tau_n=1.0, tau_p=1.0,
+    )
+
+
+def workload_params(refine, degree, dt):
+    """The measured workload: constant donor doping 1, Beer-Lambert generation
+    G = exp(-alpha x) with alpha = 1, SRH recombination (tau = 1, n_i = 0.1),
+    interface transfer rates 1e-2, no bias, all other coefficients 1."""
+    p = default_params()
+    p.update(k=degree, r=refine, dt=dt, gen_mode="config", G0=1.0, alpha=1.0, srh=True, ni=0.1,
+             tau_n=1.0, tau_p=1.0, ne=1.0, pe=0.0, rinf=1.0, oinf=1.0, k_et=1e-2, k_ht=1e-2)
+    return p
+
+
 def to_c(prm):
     """Build (CMesh, CParams) from a parameter dict (keys as oracle.solarcell.default_params)."""
     m = CMesh(prm["r1"], prm["r2"], prm["H"], prm["L"], prm["r"], int(prm["insulated"]), int(prm["schottky"]))

@@ -28,6 +28,7 @@ struct FactorDev {
   std::vector<int> off;  // per level task offsets (host)
   int nlevels = 0, n = 0;
   double nnz = 0.0;
+  double tri_bytes = 0.0;      // algorithmic bytes of a forward + backward sweep
   std::vector<int> cell_base;  // host copy: grid cell -> permuted base
 };
 
@@ -63,8 +64,15 @@ struct pecs_ctx {
   double last_res = -1.0;
   double setup_seconds = 0.0, factor_seconds = 0.0;
   std::vector<void*> allocs;
+  // optional per-phase timing of pecs_step (hipEvents recorded between phases)
+  int timing = 0;
+  std::vector<hipEvent_t> events;  // NPHASE+1 per recorded step
+  int ev_used = 0;
+  std::vector<double> phase_ms;
+  int timed_steps = 0;
 
   ~pecs_ctx() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
   }
 };
@@ -181,6 +189,8 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf) {
   }
   D.off[F.nlevels] = (int)tasks.size();
   D.nnz = (double)F.Linv.size() + (double)F.L21.size();
+  for (int s2 = 0; s2 < nn; ++s2)
+    D.tri_bytes += 2.0 * 8.0 * (0.5 * ns[s2] * (ns[s2] + 1.0) + (double)ms[s2] * ns[s2]);
   DevFactor& d = D.d;
   d.n = F.n;
   d.nnodes = nn;
@@ -282,15 +292,89 @@ void poisson_from_densities(pecs_ctx* c, const double* un, const double* up, con
   poisson_solve_from_y(c, out);
 }
 
+// Phases of one step, in stream order (timing mode records an event after each):
+// 0 rhs semiconductor, 1 rhs electrolyte, 2 Schur transform semi, 3 Cholesky
+// semi, 4 recover semi, 5 Schur transform elec, 6 Cholesky elec, 7 recover
+// elec, 8 Poisson condensation, 9 Poisson multiplier Cholesky, 10 Poisson
+// recovery.
+constexpr int NPHASE = 11;
+
+struct PhaseMarks {
+  pecs_ctx* c;
+  hipEvent_t* ev;
+  int i = 0;
+  void mark() {
+    if (ev) HIPCHECK(hipEventRecord(ev[i++], c->stream));
+  }
+};
+
+void solve_region_marked(pecs_ctx* c, int reg, const double* b0, const double* b1, double* x0, double* x1,
+                         PhaseMarks& pm) {
+  const DevRegion& R = reg == 0 ? c->Rs : c->Re;
+  FactorDev* F = reg == 0 ? c->Fs : c->Fe;
+  const bool share = reg == 0 ? c->share_s : c->share_e;
+  const double mu0 = c->P.mu[2 * reg], mu1 = c->P.mu[2 * reg + 1];
+  double* W0 = c->W[2 * reg];
+  double* W1 = c->W[2 * reg + 1];
+  hipStream_t s = c->stream;
+  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, mu0, W0, s));
+  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, mu1, W1, s));
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg][0], c->g[reg][1], 2, s));
+  pm.mark();
+  if (share) {
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], c->g[reg][1],
+                                           c->t[reg][0], c->t[reg][1], 2, s));
+  } else {
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], nullptr,
+                                           c->t[reg][0], nullptr, 1, s));
+    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].off.data(), F[1].nlevels, c->g[reg][1], nullptr,
+                                           c->t[reg][1], nullptr, 1, s));
+  }
+  pm.mark();
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W0, W1, c->g[reg][0], c->g[reg][1], mu0, mu1, x0, x1, b0, b1,
+                                            2, s));
+  pm.mark();
+}
+
 void step_once(pecs_ctx* c) {
   hipStream_t s = c->stream;
+  PhaseMarks pm{c, nullptr};
+  if (c->timing && c->ev_used + NPHASE + 1 <= (int)c->events.size()) {
+    pm.ev = &c->events[c->ev_used];
+    c->ev_used += NPHASE + 1;
+    pm.mark();
+  }
   HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->ph, c->u[0], c->u[1],
                                           c->u[2], c->u[3], c->pv, c->b[0], c->b[1], s));
+  pm.mark();
   HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->ph, c->u[2], c->u[3],
                                           c->u[0], c->u[1], c->pv, c->b[2], c->b[3], s));
-  solve_region(c, 0, c->b[0], c->b[1], c->u[0], c->u[1], 2);
-  solve_region(c, 1, c->b[2], c->b[3], c->u[2], c->u[3], 2);
-  poisson_from_densities(c, c->u[0], c->u[1], c->u[2], c->u[3], c->pv);
+  pm.mark();
+  solve_region_marked(c, 0, c->b[0], c->b[1], c->u[0], c->u[1], pm);
+  solve_region_marked(c, 1, c->b[2], c->b[3], c->u[2], c->u[3], pm);
+  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->ph, c->u[0], c->u[1], c->u[2], c->u[3],
+                                           c->mesh.semi.ncell, c->mesh.elec.ncell, c->yP, s));
+  pm.mark();
+  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.off.data(), c->FH.nlevels, c->rH, nullptr, c->tH, nullptr, 1,
+                                         s));
+  pm.mark();
+  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s));
+  pm.mark();
+}
+
+void collect_timing(pecs_ctx* c) {
+  if (!c->timing || c->ev_used == 0) return;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b + NPHASE + 1 <= c->ev_used; b += NPHASE + 1) {
+    for (int p = 0; p < NPHASE; ++p) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, c->events[b + p], c->events[b + p + 1]));
+      c->phase_ms[p] += ms;
+    }
+    c->timed_steps++;
+  }
+  c->ev_used = 0;
 }
 
 int fail(pecs_ctx* c, const std::exception& e) {
@@ -733,7 +817,9 @@ int pecs_step(pecs_ctx* ctx, int nsteps) {
         HIPCHECK(hipMemcpyAsync(ctx->old[4], ctx->pv, nsv[4] * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
       }
       step_once(ctx);
+      if (ctx->timing && ctx->ev_used + NPHASE + 1 > (int)ctx->events.size()) collect_timing(ctx);
     }
+    collect_timing(ctx);
     if (nsteps > 0) {
       const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
       HIPCHECK(hipMemsetAsync(ctx->acc, 0, 2 * sizeof(double), ctx->stream));
@@ -746,6 +832,37 @@ int pecs_step(pecs_ctx* ctx, int nsteps) {
       ctx->last_res = (h[1] > 0) ? std::sqrt(h[0] / h[1]) / ctx->P.dt : 0.0;
     }
   })
+}
+
+int pecs_enable_timing(pecs_ctx* ctx, int enable) {
+  PECS_TRY(ctx, {
+    ctx->timing = enable ? 1 : 0;
+    if (ctx->timing && ctx->events.empty()) {
+      ctx->events.resize(64 * (NPHASE + 1));
+      for (auto& e : ctx->events) HIPCHECK(hipEventCreate(&e));
+    }
+    ctx->phase_ms.assign(NPHASE, 0.0);
+    ctx->timed_steps = 0;
+    ctx->ev_used = 0;
+  })
+}
+
+int pecs_phase_times(pecs_ctx* ctx, double* ms, int n, int* steps) {
+  if (!ctx || !ms) return -1;
+  for (int i = 0; i < n && i < NPHASE; ++i) ms[i] = i < (int)ctx->phase_ms.size() ? ctx->phase_ms[i] : 0.0;
+  if (steps) *steps = ctx->timed_steps;
+  return 0;
+}
+
+int pecs_factor_bytes(pecs_ctx* ctx, int which, double* bytes) {
+  // Algorithmic bytes of one triangular solve pair (forward + backward) with the
+  // factor of system `which` (0: semiconductor Schur, 1: electrolyte Schur,
+  // 2: Poisson multipliers): the lower triangles of the inverse diagonal
+  // blocks and the off-diagonal panels, read once per sweep, 8 B per value.
+  if (!ctx || !bytes) return -1;
+  const FactorDev* F = which == 0 ? &ctx->Fs[0] : (which == 1 ? &ctx->Fe[0] : &ctx->FH);
+  *bytes = F->tri_bytes;
+  return 0;
 }
 
 int pecs_last_residual(pecs_ctx* ctx, double* r) {

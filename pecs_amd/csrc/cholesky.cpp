@@ -4,6 +4,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -61,6 +62,56 @@ struct NDBuilder {
   }
 };
 
+// F[i][j] -= sum_t P[i][t] P[j][t] for k1 <= j <= i < nf, P = F[:, k0:k0+pw].
+// Register-blocked 4 x 8 micro-kernel over a packed copy of the panel and of
+// its transpose.
+void trailing_update(double* F, int nf, int k0, int k1, int pw) {
+  const int nt = nf - k1;
+  const int ntp = (nt + 7) & ~7;
+  std::vector<double> P((size_t)(nt + 4) * pw, 0.0), PT((size_t)pw * (ntp + 8), 0.0);
+  const int ld = ntp + 8;
+#pragma omp parallel for schedule(static) if (nt > 256)
+  for (int i = 0; i < nt; ++i)
+    for (int t = 0; t < pw; ++t) {
+      const double v = F[(size_t)(k1 + i) * nf + k0 + t];
+      P[(size_t)i * pw + t] = v;
+      PT[(size_t)t * ld + i] = v;
+    }
+  const int nib = (nt + 3) / 4;
+#pragma omp parallel for schedule(dynamic, 4) if (nt > 64)
+  for (int ib = nib - 1; ib >= 0; --ib) {
+    const int i0 = 4 * ib;
+    const double* a0 = &P[(size_t)(i0 + 0) * pw];
+    const double* a1 = &P[(size_t)(i0 + 1) * pw];
+    const double* a2 = &P[(size_t)(i0 + 2) * pw];
+    const double* a3 = &P[(size_t)(i0 + 3) * pw];
+    for (int j0 = 0; j0 <= i0 + 3 && j0 < nt; j0 += 8) {
+      double acc[4][8];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
+      const double* bt = &PT[j0];
+      for (int t = 0; t < pw; ++t) {
+        const double* b = bt + (size_t)t * ld;
+        const double x0 = a0[t], x1 = a1[t], x2 = a2[t], x3 = a3[t];
+#pragma omp simd
+        for (int c = 0; c < 8; ++c) {
+          acc[0][c] += x0 * b[c];
+          acc[1][c] += x1 * b[c];
+          acc[2][c] += x2 * b[c];
+          acc[3][c] += x3 * b[c];
+        }
+      }
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r;
+        if (i >= nt) break;
+        double* Fi = &F[(size_t)(k1 + i) * nf + k1];
+        const int jmax = std::min(j0 + 8, i + 1);
+        for (int j = j0; j < jmax; ++j) Fi[j] -= acc[r][j - j0];
+      }
+    }
+  }
+}
+
 // Partial Cholesky of the leading ns columns of the lower triangle of the
 // nf x nf row-major front F (upper triangle ignored).  On return F holds L11,
 // L21 and the Schur update in its lower triangle.
@@ -90,32 +141,38 @@ void partial_cholesky(double* F, int nf, int ns, double& flops) {
     if (nt <= 0) continue;
     const int pw = k1 - k0;
     flops += (double)nt * nt * pw;
-#pragma omp parallel for schedule(dynamic, 16) if (nt > 128)
-    for (int i = k1; i < nf; ++i) {
-      const double* Li = &F[(size_t)i * nf + k0];
-      double* Fi = &F[(size_t)i * nf];
-      for (int j = k1; j <= i; ++j) {
-        const double* Lj = &F[(size_t)j * nf + k0];
-        double s = 0.0;
-#pragma omp simd reduction(+ : s)
-        for (int t = 0; t < pw; ++t) s += Li[t] * Lj[t];
-        Fi[j] -= s;
-      }
-    }
+    trailing_update(F, nf, k0, k1, pw);
   }
 }
 
 // Inverse of a lower-triangular n x n block stored with leading dimension ld.
+// Blocked: each column block J of X = L^{-1} is independent; row block I of it
+// is L_II^{-1} (delta_IJ - sum_{J<=K<I} L_IK X_KJ), computed with contiguous
+// row segments.
 void lower_inverse(const double* L, int n, int ld, double* X) {
+  const int B = 32;
+  const int nbk = (n + B - 1) / B;
+#pragma omp parallel for schedule(static) if (n > 256)
   for (size_t t = 0; t < (size_t)n * n; ++t) X[t] = 0.0;
-#pragma omp parallel for schedule(dynamic, 8) if (n > 256)
-  for (int j = 0; j < n; ++j) {
-    X[(size_t)j * n + j] = 1.0 / L[(size_t)j * ld + j];
-    for (int i = j + 1; i < n; ++i) {
-      double s = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) if (n > 128)
+  for (int jb = 0; jb < nbk; ++jb) {
+    const int j0 = jb * B, j1 = std::min(n, j0 + B), w = j1 - j0;
+    double T[32];
+    for (int i = j0; i < n; ++i) {
+      // T = delta_i - sum_{t in [j0, i)} L[i][t] X[t][j0:j1]
+      for (int c = 0; c < w; ++c) T[c] = (i == j0 + c) ? 1.0 : 0.0;
       const double* Li = &L[(size_t)i * ld];
-      for (int t = j; t < i; ++t) s += Li[t] * X[(size_t)t * n + j];
-      X[(size_t)i * n + j] = -s / Li[i];
+      for (int t = j0; t < i; ++t) {
+        const double l = Li[t];
+        if (l == 0.0) continue;
+        const double* Xt = &X[(size_t)t * n + j0];
+#pragma omp simd
+        for (int c = 0; c < 32; ++c)
+          if (c < w) T[c] -= l * Xt[c];
+      }
+      const double inv = 1.0 / Li[i];
+      double* Xi = &X[(size_t)i * n + j0];
+      for (int c = 0; c < w; ++c) Xi[c] = (j0 + c <= i) ? T[c] * inv : 0.0;
     }
   }
 }
@@ -206,16 +263,21 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
     for (int c : structc[s])
       for (int a = 0; a < S.ndof[c]; ++a) F.rows[F.nodes[s].rows_off + t++] = F.cell_base[c] + a;
   }
-  // numeric multifrontal factorisation
+  // numeric multifrontal factorisation, level by level on the elimination tree:
+  // nodes of one level are independent (their children are in lower levels).
   std::vector<std::vector<double>> upd(nn);
-  std::vector<int> front_off(NG, -1);
-  for (int s = 0; s < nn; ++s) {
+  const int nthreads = omp_get_max_threads();
+  std::vector<std::vector<int>> front_off_t(nthreads, std::vector<int>(NG, -1));
+  std::vector<double> flops_t(nthreads, 0.0), ta_t(nthreads, 0.0), tf_t(nthreads, 0.0), ti_t(nthreads, 0.0);
+  auto process = [&](int s, int tid) {
+    std::vector<int>& front_off = front_off_t[tid];
     const SNode& N = F.nodes[s];
     const int ns = N.ns, ms = N.ms, nf = ns + ms;
-    if (ns == 0) {
-      // a supernode without DoFs only forwards its children's updates
-    }
-    std::vector<double> Fr((size_t)nf * nf, 0.0);
+    const bool big = nf > 1024;
+    auto tA = std::chrono::steady_clock::now();
+    std::vector<double> Fr((size_t)nf * nf);
+#pragma omp parallel for schedule(static) if (big)
+    for (int r = 0; r < nf; ++r) std::fill(&Fr[(size_t)r * nf], &Fr[(size_t)r * nf] + r + 1, 0.0);
     int off = 0;
     for (int c : nd.nodes[s].cells) {
       front_off[c] = off;
@@ -258,6 +320,7 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
       for (int Y : structc[c])
         for (int a = 0; a < S.ndof[Y]; ++a) map[t++] = front_off[Y] + a;
       const std::vector<double>& U = upd[c];
+#pragma omp parallel for schedule(dynamic, 64) if (big)
       for (int r = 0; r < mc; ++r) {
         double* row = &Fr[(size_t)map[r] * nf];
         const double* ur = &U[(size_t)r * mc];
@@ -267,17 +330,40 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
     }
     for (int c : nd.nodes[s].cells) front_off[c] = -1;
     for (int c : structc[s]) front_off[c] = -1;
-    if (ns > 0) partial_cholesky(Fr.data(), nf, ns, F.flops);
-    F.flops += (double)ns * ns * ns / 3.0;
+    auto tB = std::chrono::steady_clock::now();
+    if (ns > 0) partial_cholesky(Fr.data(), nf, ns, flops_t[tid]);
+    flops_t[tid] += (double)ns * ns * ns / 3.0;
+    auto tC = std::chrono::steady_clock::now();
     // store Linv, L21 and the update matrix
     if (ns > 0) lower_inverse(Fr.data(), ns, nf, &F.Linv[N.linv_off]);
+    auto tD = std::chrono::steady_clock::now();
+    ta_t[tid] += std::chrono::duration<double>(tB - tA).count();
+    tf_t[tid] += std::chrono::duration<double>(tC - tB).count();
+    ti_t[tid] += std::chrono::duration<double>(tD - tC).count();
+#pragma omp parallel for schedule(static) if (big)
     for (int r = 0; r < ms; ++r)
       for (int c2 = 0; c2 < ns; ++c2) F.L21[N.l21_off + (int64_t)r * ns + c2] = Fr[(size_t)(ns + r) * nf + c2];
     if (ms > 0) {
-      upd[s].assign((size_t)ms * ms, 0.0);
+      upd[s].resize((size_t)ms * ms);
+#pragma omp parallel for schedule(static) if (big)
       for (int r = 0; r < ms; ++r)
         for (int q = 0; q <= r; ++q) upd[s][(size_t)r * ms + q] = Fr[(size_t)(ns + r) * nf + ns + q];
     }
+  };
+  for (int l = 0; l < F.nlevels; ++l) {
+    const std::vector<int>& lv = F.level_nodes[l];
+    if ((int)lv.size() >= 2 * nthreads) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int i = 0; i < (int)lv.size(); ++i) process(lv[i], omp_get_thread_num());
+    } else {
+      for (int s : lv) process(s, 0);
+    }
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    F.flops += flops_t[t];
+    F.t_assemble += ta_t[t];
+    F.t_factor += tf_t[t];
+    F.t_inverse += ti_t[t];
   }
   // forward contribution lists
   std::vector<int64_t> cnt(F.n + 1, 0);

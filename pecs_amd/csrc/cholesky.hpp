@@ -36,6 +36,7 @@ struct SupernodalFactor {
   std::vector<int64_t> contrib_ptr;
   std::vector<int> contrib_node, contrib_row;
   double flops = 0.0;
+  double t_assemble = 0.0, t_factor = 0.0, t_inverse = 0.0;
   int64_t nnz_L() const { return (int64_t)Linv.size() + (int64_t)L21.size(); }
 };
 

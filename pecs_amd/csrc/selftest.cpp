@@ -89,6 +89,7 @@ static double check(const char* name, const BlockSPD& S, int leaf) {
   std::printf("%s: n=%d nodes=%zu levels=%d nnzL=%lld (%.1f MB) flops=%.3g factor %.2fs  resid=%.3e err=%.3e\n", name,
               F.n, F.nodes.size(), F.nlevels, (long long)F.nnz_L(), F.nnz_L() * 8e-6, F.flops, ts, rel,
               std::sqrt(ne / nx));
+  std::printf("   time: assemble %.2fs factor %.2fs inverse %.2fs\n", F.t_assemble, F.t_factor, F.t_inverse);
   return rel;
 }
 
