@@ -25,11 +25,14 @@ thread_local std::string g_error;
 
 struct FactorDev {
   DevFactor d{};
-  std::vector<int> off;  // per level task offsets (host)
+  std::vector<int> foff, boff, goff, xoff, fw, bw;  // per level (host)
   int nlevels = 0, n = 0;
   double nnz = 0.0;
   double tri_bytes = 0.0;      // algorithmic bytes of a forward + backward sweep
   std::vector<int> cell_base;  // host copy: grid cell -> permuted base
+  FactorLevels lv() const {
+    return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(), bw.data(), nlevels};
+  }
 };
 
 }  // namespace
@@ -156,59 +159,62 @@ void validate(const pecs_mesh* m, const pecs_params* q) {
 }
 
 FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf) {
-  SupernodalFactor F = cholesky_factor(S, leaf);
+  PackedFactor P;
+  {
+    SupernodalFactor F = cholesky_factor(S, leaf);
+    P = pack_factor(F);
+  }
   FactorDev D;
-  D.n = F.n;
-  D.nlevels = F.nlevels;
-  D.cell_base = F.cell_base;
-  const int nn = (int)F.nodes.size();
-  std::vector<int> p0(nn), ns(nn), ms(nn);
-  std::vector<int64_t> lo(nn), l2(nn), ro(nn);
-  for (int s = 0; s < nn; ++s) {
-    p0[s] = F.nodes[s].p0;
-    ns[s] = F.nodes[s].ns;
-    ms[s] = F.nodes[s].ms;
-    lo[s] = F.nodes[s].linv_off;
-    l2[s] = F.nodes[s].l21_off;
-    ro[s] = F.nodes[s].rows_off;
-  }
-  std::vector<double> LinvT(F.Linv.size()), L21T(F.L21.size());
-  for (int s = 0; s < nn; ++s) {
-    const int n_ = ns[s], m_ = ms[s];
-    for (int i = 0; i < n_; ++i)
-      for (int j = 0; j < n_; ++j) LinvT[lo[s] + (int64_t)j * n_ + i] = F.Linv[lo[s] + (int64_t)i * n_ + j];
-    for (int r = 0; r < m_; ++r)
-      for (int j = 0; j < n_; ++j) L21T[l2[s] + (int64_t)j * m_ + r] = F.L21[l2[s] + (int64_t)r * n_ + j];
-  }
-  std::vector<int2> tasks;
-  D.off.assign(F.nlevels + 1, 0);
-  for (int l = 0; l < F.nlevels; ++l) {
-    D.off[l] = (int)tasks.size();
-    for (int s : F.level_nodes[l])
-      for (int r = 0; r < ns[s]; ++r) tasks.push_back(make_int2(s, r));
-  }
-  D.off[F.nlevels] = (int)tasks.size();
-  D.nnz = (double)F.Linv.size() + (double)F.L21.size();
-  for (int s2 = 0; s2 < nn; ++s2)
-    D.tri_bytes += 2.0 * 8.0 * (0.5 * ns[s2] * (ns[s2] + 1.0) + (double)ms[s2] * ns[s2]);
+  D.n = P.n;
+  D.nlevels = P.nlevels;
+  D.cell_base = P.cell_base;
+  D.foff = P.foff;
+  D.boff = P.boff;
+  D.goff = P.goff;
+  D.xoff = P.xoff;
+  D.fw = P.fw;
+  D.bw = P.bw;
+  D.nnz = (double)P.fwd.size();
+  D.tri_bytes = P.tri_bytes;
+  const int nn = (int)P.p0.size();
+  const auto& p0 = P.p0;
+  const auto& ns = P.ns;
+  const auto& ms = P.ms;
+  const auto& fo = P.fwd_off;
+  const auto& bo = P.bwd_off;
+  const auto& ro = P.rows_off;
+  const auto& zo = P.z_off;
+  const auto& fwd = P.fwd;
+  const auto& bwd = P.bwd;
+  const auto& czidx = P.czidx;
+  const int64_t zl = P.zlen;
   DevFactor& d = D.d;
-  d.n = F.n;
+  d.n = P.n;
   d.nnodes = nn;
+  d.zlen = zl;
   d.p0 = upload(c, p0);
   d.ns = upload(c, ns);
   d.ms = upload(c, ms);
-  d.linv_off = upload(c, lo);
-  d.l21_off = upload(c, l2);
+  d.fwd_off = upload(c, fo);
+  d.bwd_off = upload(c, bo);
   d.rows_off = upload(c, ro);
-  d.Linv = upload(c, F.Linv);
-  d.LinvT = upload(c, LinvT);
-  d.L21 = upload(c, F.L21);
-  d.L21T = upload(c, L21T);
-  d.rows = upload(c, F.rows);
-  d.cptr = upload(c, F.contrib_ptr);
-  d.cnode = upload(c, F.contrib_node);
-  d.crow = upload(c, F.contrib_row);
-  d.tasks = upload(c, tasks);
+  d.z_off = upload(c, zo);
+  d.fwd = upload(c, fwd);
+  d.bwd = upload(c, bwd);
+  d.rows = upload(c, P.rows);
+  d.cptr = upload(c, P.cptr);
+  d.czidx = upload(c, czidx);
+  std::vector<int4> ft(P.ftiles.size() / 4), bt(P.btiles.size() / 4);
+  for (size_t t = 0; t < ft.size(); ++t)
+    ft[t] = make_int4(P.ftiles[4 * t], P.ftiles[4 * t + 1], P.ftiles[4 * t + 2], 0);
+  for (size_t t = 0; t < bt.size(); ++t)
+    bt[t] = make_int4(P.btiles[4 * t], P.btiles[4 * t + 1], P.btiles[4 * t + 2], 0);
+  d.ftiles = upload(c, ft);
+  d.lrows = upload(c, P.lrows);
+  d.xidx = upload(c, P.xidx);
+  d.btiles = upload(c, bt);
+  d.Z = dalloc<double>(c, 2 * (size_t)std::max<int64_t>(zl, 1));
+  d.X = dalloc<double>(c, 2 * (size_t)P.n);
   return D;
 }
 
@@ -245,14 +251,14 @@ void solve_region(pecs_ctx* c, int reg, const double* b0, const double* b1, doub
   if (nrhs == 2) HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, mu1, W1, s));
   HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg][0], c->g[reg][1], nrhs, s));
   if (nrhs == 2 && share) {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], c->g[reg][1],
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], c->g[reg][1],
                                            c->t[reg][0], c->t[reg][1], 2, s));
   } else {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], nullptr,
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], nullptr,
                                            c->t[reg][0], nullptr, 1, s));
     if (nrhs == 2) {
       const FactorDev& F1 = share ? F[0] : F[1];
-      HIPCHECK((hipError_t)launch_chol_solve(F1.d, F1.off.data(), F1.nlevels, c->g[reg][1], nullptr, c->t[reg][1],
+      HIPCHECK((hipError_t)launch_chol_solve(F1.d, F1.lv(), c->g[reg][1], nullptr, c->t[reg][1],
                                              nullptr, 1, s));
     }
   }
@@ -272,7 +278,7 @@ void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
   double* W = c->W[which];
   HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b, mu, W, s));
   HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b, b, W, W, c->g[reg][0], c->g[reg][0], 1, s));
-  HIPCHECK((hipError_t)launch_chol_solve(Fc.d, Fc.off.data(), Fc.nlevels, c->g[reg][0], nullptr, c->t[reg][0],
+  HIPCHECK((hipError_t)launch_chol_solve(Fc.d, Fc.lv(), c->g[reg][0], nullptr, c->t[reg][0],
                                          nullptr, 1, s));
   HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W, W, c->g[reg][0], c->g[reg][0], mu, mu, x, x, b, b, 1, s));
 }
@@ -280,7 +286,7 @@ void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
 void poisson_solve_from_y(pecs_ctx* c, double* out) {
   hipStream_t s = c->stream;
   HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.off.data(), c->FH.nlevels, c->rH, nullptr, c->tH, nullptr, 1,
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, nullptr, c->tH, nullptr, 1,
                                          s));
   HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, out, s));
 }
@@ -322,12 +328,12 @@ void solve_region_marked(pecs_ctx* c, int reg, const double* b0, const double* b
   HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg][0], c->g[reg][1], 2, s));
   pm.mark();
   if (share) {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], c->g[reg][1],
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], c->g[reg][1],
                                            c->t[reg][0], c->t[reg][1], 2, s));
   } else {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].off.data(), F[0].nlevels, c->g[reg][0], nullptr,
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], nullptr,
                                            c->t[reg][0], nullptr, 1, s));
-    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].off.data(), F[1].nlevels, c->g[reg][1], nullptr,
+    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].lv(), c->g[reg][1], nullptr,
                                            c->t[reg][1], nullptr, 1, s));
   }
   pm.mark();
@@ -356,7 +362,7 @@ void step_once(pecs_ctx* c) {
                                            c->mesh.semi.ncell, c->mesh.elec.ncell, c->yP, s));
   pm.mark();
   HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.off.data(), c->FH.nlevels, c->rH, nullptr, c->tH, nullptr, 1,
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, nullptr, c->tH, nullptr, 1,
                                          s));
   pm.mark();
   HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s));

@@ -384,4 +384,120 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
   return F;
 }
 
+PackedFactor pack_factor(const SupernodalFactor& F) {
+  PackedFactor P;
+  const int nn = (int)F.nodes.size();
+  P.n = F.n;
+  P.nlevels = F.nlevels;
+  P.cell_base = F.cell_base;
+  P.p0.resize(nn);
+  P.ns.resize(nn);
+  P.ms.resize(nn);
+  P.fwd_off.resize(nn);
+  P.bwd_off.resize(nn);
+  P.rows_off.resize(nn);
+  P.z_off.resize(nn);
+  int64_t nf = 0, zl = 0;
+  for (int s = 0; s < nn; ++s) {
+    P.p0[s] = F.nodes[s].p0;
+    P.ns[s] = F.nodes[s].ns;
+    P.ms[s] = F.nodes[s].ms;
+    P.rows_off[s] = F.nodes[s].rows_off;
+    const int64_t n_ = P.ns[s], m_ = P.ms[s];
+    P.fwd_off[s] = nf;
+    P.bwd_off[s] = nf;
+    P.z_off[s] = zl;
+    nf += n_ * (n_ + 1) / 2 + m_ * n_;
+    zl += m_;
+    P.tri_bytes += 2.0 * 8.0 * (0.5 * n_ * (n_ + 1.0) + (double)m_ * n_);
+  }
+  P.zlen = zl;
+  P.fwd.assign(nf, 0.0);
+  P.bwd.assign(nf, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int s = 0; s < nn; ++s) {
+    const int n_ = P.ns[s], m_ = P.ms[s];
+    const double* Li = &F.Linv[F.nodes[s].linv_off];
+    const double* L21 = &F.L21[F.nodes[s].l21_off];
+    double* f = &P.fwd[P.fwd_off[s]];
+    for (int r = 0; r < n_; ++r)
+      for (int j = 0; j <= r; ++j) f[(int64_t)r * (r + 1) / 2 + j] = Li[(int64_t)r * n_ + j];
+    // W21 = L21 Linv, 8 panel rows at a time so each Linv row is reused
+    double* W = f + (int64_t)n_ * (n_ + 1) / 2;
+    for (int i0 = 0; i0 < m_; i0 += 8) {
+      const int i1 = std::min(m_, i0 + 8);
+      for (int k = 0; k < n_; ++k) {
+        const double* Lk = Li + (int64_t)k * n_;
+        for (int i = i0; i < i1; ++i) {
+          const double a = L21[(int64_t)i * n_ + k];
+          if (a == 0.0) continue;
+          double* Wi = W + (int64_t)i * n_;
+#pragma omp simd
+          for (int j = 0; j <= k; ++j) Wi[j] += a * Lk[j];
+        }
+      }
+    }
+    double* bw = &P.bwd[P.bwd_off[s]];
+    for (int j = 0; j < n_; ++j) {
+      double* row = bw + (int64_t)j * (n_ + m_) - (int64_t)j * (j - 1) / 2;
+      for (int i = j; i < n_; ++i) row[i - j] = Li[(int64_t)i * n_ + j];
+      for (int i = 0; i < m_; ++i) row[n_ - j + i] = -W[(int64_t)i * n_ + j];
+    }
+  }
+  // lanes per row: the smallest power of two >= an eighth of the mean row length
+  auto lanes = [](double mean) {
+    int w = 8;
+    while (w < 64 && w * 8 < mean) w *= 2;
+    return w;
+  };
+  P.foff.assign(F.nlevels + 1, 0);
+  P.boff.assign(F.nlevels + 1, 0);
+  P.goff.assign(F.nlevels + 1, 0);
+  P.xoff.assign(F.nlevels + 1, 0);
+  P.fw.assign(F.nlevels, 64);
+  P.bw.assign(F.nlevels, 64);
+  for (int l = 0; l < F.nlevels; ++l) {
+    double fl = 0, fr = 0, bl = 0, br = 0;
+    for (int s : F.level_nodes[l]) {
+      const double n_ = P.ns[s], m_ = P.ms[s];
+      fl += 0.5 * n_ * (n_ + 1) + m_ * n_;
+      fr += n_ + m_;
+      bl += 0.5 * n_ * (n_ + 1) + m_ * n_;
+      br += n_;
+    }
+    P.fw[l] = lanes(fr > 0 ? fl / fr : 1);
+    P.bw[l] = lanes(br > 0 ? bl / br : 1);
+    P.foff[l] = (int)(P.ftiles.size() / 4);
+    P.boff[l] = (int)(P.btiles.size() / 4);
+    P.goff[l] = (int)P.lrows.size();
+    P.xoff[l] = (int)P.xidx.size();
+    // tile height: at most sweep_tile_rows(W), smaller on levels with few rows
+    // so that each level still spreads over >= ~4096 workgroups
+    auto height = [](int w, double nrows) {
+      int h = sweep_tile_rows(w);
+      const int pass = 256 / w;
+      while (h > pass && nrows / h < 8192) h /= 2;
+      return h;
+    };
+    const int rf = height(P.fw[l], fr), rb = height(P.bw[l], br);
+    for (int s : F.level_nodes[l]) {
+      for (int r = 0; r < P.ns[s]; ++r) P.lrows.push_back(P.p0[s] + r);
+      for (int i = 0; i < P.ms[s]; ++i) P.xidx.push_back(P.rows_off[s] + i);
+      const int nf = P.ns[s] + P.ms[s];
+      for (int r0 = 0; r0 < nf; r0 += rf) P.ftiles.insert(P.ftiles.end(), {s, r0, std::min(rf, nf - r0), 0});
+      for (int r0 = 0; r0 < P.ns[s]; r0 += rb)
+        P.btiles.insert(P.btiles.end(), {s, r0, std::min(rb, P.ns[s] - r0), 0});
+    }
+  }
+  P.foff[F.nlevels] = (int)(P.ftiles.size() / 4);
+  P.boff[F.nlevels] = (int)(P.btiles.size() / 4);
+  P.goff[F.nlevels] = (int)P.lrows.size();
+  P.xoff[F.nlevels] = (int)P.xidx.size();
+  P.rows = F.rows;
+  P.cptr = F.contrib_ptr;
+  P.czidx.resize(F.contrib_node.size());
+  for (size_t t = 0; t < P.czidx.size(); ++t) P.czidx[t] = P.z_off[F.contrib_node[t]] + F.contrib_row[t];
+  return P;
+}
+
 }  // namespace pecs

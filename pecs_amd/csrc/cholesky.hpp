@@ -42,4 +42,32 @@ struct SupernodalFactor {
 
 SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells);
 
+// The factor rearranged for level-scheduled dense matrix-vector sweeps (the
+// layout the GPU solve kernels read).  Per supernode s:
+//   fwd: packed lower triangle of Linv (row r: r+1 values at r(r+1)/2), then
+//        W21 = L21 Linv (ms x ns row-major);
+//   bwd: per own row j, [Linv^T(j, j:ns) | -W21^T(j, :)] (ns - j + ms values)
+//        at offset j(ns+ms) - j(j-1)/2.
+// Forward at a level: t_p = b_p - sum z[czidx[cptr[p]:cptr[p+1]]], then
+// [y_own; z_s] = [Linv; W21] t_own.  Backward: x_own = bwd rows . [y; x_rows].
+// The sweeps run over row tiles (node, first row, rows) of sweep_tile_rows(W)
+// rows, W lanes per row chosen per level from the mean row length.
+inline int sweep_tile_rows(int w) { return w <= 16 ? 64 : (w == 32 ? 32 : 16); }
+
+struct PackedFactor {
+  int n = 0, nlevels = 0;
+  int64_t zlen = 0;
+  std::vector<int> cell_base, p0, ns, ms, rows;
+  std::vector<int64_t> fwd_off, bwd_off, rows_off, z_off, cptr, czidx;
+  std::vector<double> fwd, bwd;
+  std::vector<int> ftiles, btiles;    // flattened (node, row0, nrows, 0) grouped by level
+  std::vector<int> foff, boff;        // per level offsets (tiles)
+  std::vector<int> lrows, goff;       // own rows of each level's nodes + offsets
+  std::vector<int64_t> xidx;          // panel-row positions of each level's nodes
+  std::vector<int> xoff;              // per level offsets into xidx
+  std::vector<int> fw, bw;            // lanes per row of the sweeps per level
+  double tri_bytes = 0.0;             // bytes read by one forward + backward sweep
+};
+PackedFactor pack_factor(const SupernodalFactor& F);
+
 }  // namespace pecs

@@ -392,109 +392,160 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
 }
 
 // ------------------------------------------------------------------------
-// Supernodal triangular solves, one wavefront per row
+// Supernodal triangular solves (S = L L^T), level by level
 // ------------------------------------------------------------------------
+// One workgroup per tile of rows of one supernode.  The vector the rows are
+// dotted with is staged in LDS in chunks of CW entries per right-hand side;
+// W lanes share a row (256/W rows per pass), partial sums stay in registers
+// across chunks.
+//   forward  (FWD): vector t_own (= b_own - updates z owed by descendants,
+//                   from k_fwd_gather), rows of [Linv; W21]; outputs
+//                   y_own -> Y, z -> Z.
+//   backward:       vector [y_own; x at the panel rows], rows
+//                   [Linv^T(j, j:) | -W21^T(j, :)]; outputs x_own -> X.
+constexpr int CW = 1024;
+
+template <int W>
+__device__ __forceinline__ double group_sum(double x) {
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, W);
+  return x;
+}
+
+template <int W>
+struct SweepShape {
+  static constexpr int R = 256 / W;              // rows per pass
+  static constexpr int RB = (W <= 16) ? 64 : (W == 32 ? 32 : 16);  // rows per tile (sweep_tile_rows)
+  static constexpr int MAXR = RB / R;              // passes per tile
+};
+
+// forward, part 1: t_p = b_p - sum of the updates z owed to row p, into X
 template <int NRHS>
-__global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int2* __restrict__ tasks, int ntask,
-                                                    const double* __restrict__ v0, const double* __restrict__ v1,
-                                                    double* __restrict__ t0, double* __restrict__ t1) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
-  if (w >= ntask) return;
-  const int2 tk = tasks[w];
-  const int p = F.p0[tk.x] + tk.y;
-  double a0 = 0.0, a1 = 0.0;
+__global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int* __restrict__ lrows, int nrow,
+                                                    const double* __restrict__ b0, const double* __restrict__ b1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrow) return;
+  const int p = lrows[i];
+  double a0 = b0[p], a1 = (NRHS > 1) ? b1[p] : 0.0;
   const int64_t c1 = F.cptr[p + 1];
   for (int64_t c = F.cptr[p]; c < c1; ++c) {
-    const int d = F.cnode[c];
-    const int nsd = F.ns[d];
-    const double* row = F.L21 + F.l21_off[d] + (int64_t)F.crow[c] * nsd;
-    const int base = F.p0[d];
-    for (int j = lane; j < nsd; j += WAVE) {
-      const double L = row[j];
-      a0 += L * v0[base + j];
-      if (NRHS > 1) a1 += L * v1[base + j];
+    const int64_t z = F.czidx[c];
+    a0 -= F.Z[z];
+    if (NRHS > 1) a1 -= F.Z[F.zlen + z];
+  }
+  F.X[p] = a0;
+  if (NRHS > 1) F.X[F.n + p] = a1;
+}
+
+// backward, part 1: Z[q] = x[rows[q]] for the panel rows of the level's nodes
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_bwd_gather(DevFactor F, const int64_t* __restrict__ idx, int nidx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nidx) return;
+  const int64_t q = idx[i];
+  const int p = F.rows[q];
+  F.Z[q] = F.X[p];
+  if (NRHS > 1) F.Z[F.zlen + q] = F.X[F.n + p];
+}
+
+template <int NRHS, int W, bool FWD>
+__global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restrict__ tiles,
+                                               const double* __restrict__ b0, const double* __restrict__ b1,
+                                               double* __restrict__ Y0, double* __restrict__ Y1) {
+  using SS = SweepShape<W>;
+  constexpr int R = SS::R, MAXR = SS::MAXR;
+  __shared__ double vec[NRHS][CW];
+  const int4 tl = tiles[blockIdx.x];
+  const int s = tl.x, row0 = tl.y, nr = tl.z;
+  const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
+  const int VL = FWD ? ns : ns + ms;
+  const int g = threadIdx.x / W, sl = threadIdx.x % W;
+  double acc[MAXR][NRHS];
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q)
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) acc[q][h] = 0.0;
+  const int64_t foff = FWD ? F.fwd_off[s] : F.bwd_off[s];
+  for (int c0 = 0; c0 < VL; c0 += CW) {
+    const int cl = min(CW, VL - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cl; i += 256) {
+      const int pos = c0 + i;
+      if (FWD) {
+        vec[0][i] = F.X[p0 + pos];
+        if (NRHS > 1) vec[NRHS - 1][i] = F.X[F.n + p0 + pos];
+      } else if (pos < ns) {
+        vec[0][i] = Y0[p0 + pos];
+        if (NRHS > 1) vec[NRHS - 1][i] = Y1[p0 + pos];
+      } else {  // x at the panel rows, pre-gathered into Z by k_bwd_gather
+        const int64_t q = F.rows_off[s] + (pos - ns);
+        vec[0][i] = F.Z[q];
+        if (NRHS > 1) vec[NRHS - 1][i] = F.Z[F.zlen + q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXR; ++q) {
+      const int lr = row0 + g + q * R;
+      if (lr >= row0 + nr) break;
+      const double* ptr;
+      int start, end;
+      if (FWD) {
+        start = 0;
+        if (lr < ns) {
+          ptr = F.fwd + foff + (int64_t)lr * (lr + 1) / 2;
+          end = lr + 1;
+        } else {
+          ptr = F.fwd + foff + (int64_t)ns * (ns + 1) / 2 + (int64_t)(lr - ns) * ns;
+          end = ns;
+        }
+      } else {
+        // element at vector position pos (>= lr) is stored at index pos - lr
+        ptr = F.bwd + foff + (int64_t)lr * (ns + ms) - (int64_t)lr * (lr - 1) / 2 - lr;
+        start = lr;
+        end = ns + ms;
+      }
+      const int a = max(start, c0), e = min(end, c0 + cl);
+      double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
+      int pos = a + sl;
+      for (; pos + W < e; pos += 2 * W) {
+        const double L0 = __builtin_nontemporal_load(ptr + pos);
+        const double L1 = __builtin_nontemporal_load(ptr + pos + W);
+        s0 += L0 * vec[0][pos - c0];
+        u0 += L1 * vec[0][pos + W - c0];
+        if (NRHS > 1) {
+          s1 += L0 * vec[NRHS - 1][pos - c0];
+          u1 += L1 * vec[NRHS - 1][pos + W - c0];
+        }
+      }
+      if (pos < e) {
+        const double L0 = __builtin_nontemporal_load(ptr + pos);
+        s0 += L0 * vec[0][pos - c0];
+        if (NRHS > 1) s1 += L0 * vec[NRHS - 1][pos - c0];
+      }
+      acc[q][0] += s0 + u0;
+      if (NRHS > 1) acc[q][NRHS - 1] += s1 + u1;
     }
   }
-  a0 = wave_sum(a0);
-  if (NRHS > 1) a1 = wave_sum(a1);
-  if (lane == 0) {
-    t0[p] = v0[p] - a0;
-    if (NRHS > 1) t1[p] = v1[p] - a1;
-  }
-}
-
-template <int NRHS>
-__global__ __launch_bounds__(256) void k_fwd_tri(DevFactor F, const int2* __restrict__ tasks, int ntask,
-                                                 const double* __restrict__ t0, const double* __restrict__ t1,
-                                                 double* __restrict__ v0, double* __restrict__ v1) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
-  if (w >= ntask) return;
-  const int2 tk = tasks[w];
-  const int s = tk.x, r = tk.y, ns = F.ns[s], p0 = F.p0[s];
-  const double* row = F.Linv + F.linv_off[s] + (int64_t)r * ns;
-  double a0 = 0.0, a1 = 0.0;
-  for (int j = lane; j <= r; j += WAVE) {
-    const double L = row[j];
-    a0 += L * t0[p0 + j];
-    if (NRHS > 1) a1 += L * t1[p0 + j];
-  }
-  a0 = wave_sum(a0);
-  if (NRHS > 1) a1 = wave_sum(a1);
-  if (lane == 0) {
-    v0[p0 + r] = a0;
-    if (NRHS > 1) v1[p0 + r] = a1;
-  }
-}
-
-template <int NRHS>
-__global__ __launch_bounds__(256) void k_bwd_gather(DevFactor F, const int2* __restrict__ tasks, int ntask,
-                                                    const double* __restrict__ v0, const double* __restrict__ v1,
-                                                    double* __restrict__ t0, double* __restrict__ t1) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
-  if (w >= ntask) return;
-  const int2 tk = tasks[w];
-  const int s = tk.x, j = tk.y, ms = F.ms[s], p = F.p0[s] + j;
-  const double* row = F.L21T + F.l21_off[s] + (int64_t)j * ms;
-  const int* rows = F.rows + F.rows_off[s];
-  double a0 = 0.0, a1 = 0.0;
-  for (int i = lane; i < ms; i += WAVE) {
-    const double L = row[i];
-    const int ri = rows[i];
-    a0 += L * v0[ri];
-    if (NRHS > 1) a1 += L * v1[ri];
-  }
-  a0 = wave_sum(a0);
-  if (NRHS > 1) a1 = wave_sum(a1);
-  if (lane == 0) {
-    t0[p] = v0[p] - a0;
-    if (NRHS > 1) t1[p] = v1[p] - a1;
-  }
-}
-
-template <int NRHS>
-__global__ __launch_bounds__(256) void k_bwd_tri(DevFactor F, const int2* __restrict__ tasks, int ntask,
-                                                 const double* __restrict__ t0, const double* __restrict__ t1,
-                                                 double* __restrict__ v0, double* __restrict__ v1) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / WAVE);
-  if (w >= ntask) return;
-  const int2 tk = tasks[w];
-  const int s = tk.x, j = tk.y, ns = F.ns[s], p0 = F.p0[s];
-  const double* row = F.LinvT + F.linv_off[s] + (int64_t)j * ns;
-  double a0 = 0.0, a1 = 0.0;
-  for (int i = j + lane; i < ns; i += WAVE) {
-    const double L = row[i];
-    a0 += L * t0[p0 + i];
-    if (NRHS > 1) a1 += L * t1[p0 + i];
-  }
-  a0 = wave_sum(a0);
-  if (NRHS > 1) a1 = wave_sum(a1);
-  if (lane == 0) {
-    v0[p0 + j] = a0;
-    if (NRHS > 1) v1[p0 + j] = a1;
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q) {
+    const int lr = row0 + g + q * R;
+    double r0 = group_sum<W>(acc[q][0]);
+    double r1 = (NRHS > 1) ? group_sum<W>(acc[q][NRHS - 1]) : 0.0;
+    if (lr >= row0 + nr || sl != 0) continue;
+    if (FWD) {
+      if (lr < ns) {
+        Y0[p0 + lr] = r0;
+        if (NRHS > 1) Y1[p0 + lr] = r1;
+      } else {
+        const int64_t z = F.z_off[s] + (lr - ns);
+        F.Z[z] = r0;
+        if (NRHS > 1) F.Z[F.zlen + z] = r1;
+      }
+    } else {
+      F.X[p0 + lr] = r0;
+      if (NRHS > 1) F.X[F.n + p0 + lr] = r1;
+    }
   }
 }
 
@@ -762,34 +813,54 @@ int launch_schur_recover(int k, const DevRegion& R, const double* W1, const doub
   return (int)hipGetLastError();
 }
 
-int launch_chol_solve(const DevFactor& F, const int* off, int nlevels, double* v0, double* v1, double* t0,
-                      double* t1, int nrhs, hipStream_t s) {
-  for (int l = 0; l < nlevels; ++l) {
-    const int nt = off[l + 1] - off[l];
+#define PECS_DISPATCH_W(w, CALL)                     \
+  switch (w) {                                       \
+    case 8: { constexpr int WW = 8; CALL; } break;   \
+    case 16: { constexpr int WW = 16; CALL; } break; \
+    case 32: { constexpr int WW = 32; CALL; } break; \
+    default: { constexpr int WW = 64; CALL; } break; \
+  }
+
+int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, double* v1, double* t0, double* t1,
+                      int nrhs, hipStream_t s) {
+  if (nrhs != 2) {
+    v1 = v0;
+    t1 = t0;
+  }
+  for (int l = 0; l < L.nlevels; ++l) {
+    const int ng = L.goff[l + 1] - L.goff[l];
+    if (ng > 0) {
+      if (nrhs == 2) k_fwd_gather<2><<<nblk(ng, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v0, v1);
+      else k_fwd_gather<1><<<nblk(ng, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v0, v1);
+    }
+    const int nt = L.foff[l + 1] - L.foff[l];
     if (nt == 0) continue;
-    const int2* tk = F.tasks + off[l];
-    const int nb = nblk((int64_t)nt * WAVE, 256);
+    const int4* tl = F.ftiles + L.foff[l];
     if (nrhs == 2) {
-      k_fwd_gather<2><<<nb, 256, 0, s>>>(F, tk, nt, v0, v1, t0, t1);
-      k_fwd_tri<2><<<nb, 256, 0, s>>>(F, tk, nt, t0, t1, v0, v1);
+      PECS_DISPATCH_W(L.fw[l], (k_sweep<2, WW, true><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
     } else {
-      k_fwd_gather<1><<<nb, 256, 0, s>>>(F, tk, nt, v0, v0, t0, t0);
-      k_fwd_tri<1><<<nb, 256, 0, s>>>(F, tk, nt, t0, t0, v0, v0);
+      PECS_DISPATCH_W(L.fw[l], (k_sweep<1, WW, true><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
     }
   }
-  for (int l = nlevels - 1; l >= 0; --l) {
-    const int nt = off[l + 1] - off[l];
+  for (int l = L.nlevels - 1; l >= 0; --l) {
+    const int nx = L.xoff[l + 1] - L.xoff[l];
+    if (nx > 0) {
+      if (nrhs == 2) k_bwd_gather<2><<<nblk(nx, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
+      else k_bwd_gather<1><<<nblk(nx, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
+    }
+    const int nt = L.boff[l + 1] - L.boff[l];
     if (nt == 0) continue;
-    const int2* tk = F.tasks + off[l];
-    const int nb = nblk((int64_t)nt * WAVE, 256);
+    const int4* tl = F.btiles + L.boff[l];
     if (nrhs == 2) {
-      k_bwd_gather<2><<<nb, 256, 0, s>>>(F, tk, nt, v0, v1, t0, t1);
-      k_bwd_tri<2><<<nb, 256, 0, s>>>(F, tk, nt, t0, t1, v0, v1);
+      PECS_DISPATCH_W(L.bw[l], (k_sweep<2, WW, false><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
     } else {
-      k_bwd_gather<1><<<nb, 256, 0, s>>>(F, tk, nt, v0, v0, t0, t0);
-      k_bwd_tri<1><<<nb, 256, 0, s>>>(F, tk, nt, t0, t0, v0, v0);
+      PECS_DISPATCH_W(L.bw[l], (k_sweep<1, WW, false><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
     }
   }
+  hipError_t e = hipMemcpyAsync(v0, F.X, (size_t)F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess && nrhs == 2)
+    e = hipMemcpyAsync(v1, F.X + F.n, (size_t)F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return (int)e;
   return (int)hipGetLastError();
 }
 

@@ -49,23 +49,45 @@ struct DevPhys {
   int gen_mode, srh;
 };
 
+// Supernodal Cholesky factor S = L L^T stored for level-scheduled dense
+// matrix-vector sweeps.  Per supernode s (own DoFs [p0, p0+ns), panel rows
+// `rows` of length ms):
+//   fwd: packed lower triangle of Linv = L11^{-1} (row r has r+1 entries),
+//        then W21 = L21 Linv (ms x ns), so one GEMV [Linv; W21] t_own gives
+//        y_own and the updates z = L21 y_own owed to the ancestors;
+//   bwd: per own row j the concatenation [Linv^T(j, j:ns) | -W21^T(j, :)],
+//        so x_own = Linv^T y_own - W21^T x_rows is one GEMV per row.
 struct DevFactor {
   int n, nnodes;
+  int64_t zlen;             // sum of ms (length of one right-hand side's z)
   const int* p0;
   const int* ns;
   const int* ms;
-  const int64_t* linv_off;
-  const int64_t* l21_off;
+  const int64_t* fwd_off;
+  const int64_t* bwd_off;
   const int64_t* rows_off;
-  const double* Linv;   // row-major lower
-  const double* LinvT;  // row-major of the transpose (upper)
-  const double* L21;    // ms x ns
-  const double* L21T;   // ns x ms
+  const int64_t* z_off;
+  const double* fwd;
+  const double* bwd;
   const int* rows;
-  const int64_t* cptr;
-  const int* cnode;
-  const int* crow;
-  const int2* tasks;    // (node, local row), grouped by level
+  const int64_t* cptr;      // per permuted row p: range in czidx
+  const int64_t* czidx;     // z indices subtracted from row p
+  const int* lrows;         // own rows of each level's nodes, grouped by level
+  const int4* ftiles;       // forward row tiles (node, first row, rows, 0), grouped by level
+  const int4* btiles;       // backward row tiles, grouped by level
+  const int64_t* xidx;      // panel-row positions (into rows) of each level's nodes, grouped by level
+  double* Z;                // 2 * zlen scratch: forward updates, then backward gathered x
+  double* X;                // 2 * n scratch: forward t, then backward output
+};
+
+struct FactorLevels {
+  const int* foff;  // host: per level offsets into ftiles (nlevels+1)
+  const int* boff;  // host: per level offsets into btiles
+  const int* goff;  // host: per level offsets into lrows
+  const int* xoff;  // host: per level offsets into xidx
+  const int* fw;    // host: lanes per row of the forward sweep per level
+  const int* bw;    // host: lanes per row of the backward sweep per level
+  int nlevels;
 };
 
 // launchers (return hipError_t as int)
@@ -78,8 +100,8 @@ int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* 
 int launch_schur_recover(int k, const DevRegion& R, const double* W1, const double* W2, const double* xs1,
                          const double* xs2, double mu1, double mu2, double* out1, double* out2, const double* b1,
                          const double* b2, int nrhs, hipStream_t s);
-int launch_chol_solve(const DevFactor& F, const int* level_task_off_host, int nlevels, double* v0, double* v1,
-                      double* tmp0, double* tmp1, int nrhs, hipStream_t s);
+int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, double* v1, double* tmp0,
+                      double* tmp1, int nrhs, hipStream_t s);
 int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
                         const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
                         hipStream_t s);

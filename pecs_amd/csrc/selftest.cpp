@@ -67,6 +67,53 @@ static void factor_solve_host(const SupernodalFactor& F, const double* b, double
   }
 }
 
+// Host emulation of the GPU sweep algorithm on the packed factor: the same row
+// tiles, the same staged vectors, the same storage offsets.
+static void packed_solve_host(const PackedFactor& P, const double* b, double* x) {
+  std::vector<double> Y(P.n), z(P.zlen), X(P.n), vec;
+  for (int l = 0; l < P.nlevels; ++l)
+    for (int q = P.foff[l]; q < P.foff[l + 1]; ++q) {
+      const int s = P.ftiles[4 * q], r0 = P.ftiles[4 * q + 1], nr = P.ftiles[4 * q + 2];
+      const int ns = P.ns[s], p0 = P.p0[s];
+      vec.assign(ns, 0.0);
+      for (int pos = 0; pos < ns; ++pos) {
+        double t = b[p0 + pos];
+        for (int64_t c = P.cptr[p0 + pos]; c < P.cptr[p0 + pos + 1]; ++c) t -= z[P.czidx[c]];
+        vec[pos] = t;
+      }
+      for (int lr = r0; lr < r0 + nr; ++lr) {
+        const double* ptr;
+        int end;
+        if (lr < ns) {
+          ptr = &P.fwd[P.fwd_off[s] + (int64_t)lr * (lr + 1) / 2];
+          end = lr + 1;
+        } else {
+          ptr = &P.fwd[P.fwd_off[s] + (int64_t)ns * (ns + 1) / 2 + (int64_t)(lr - ns) * ns];
+          end = ns;
+        }
+        double a = 0;
+        for (int pos = 0; pos < end; ++pos) a += ptr[pos] * vec[pos];
+        if (lr < ns) Y[p0 + lr] = a;
+        else z[P.z_off[s] + lr - ns] = a;
+      }
+    }
+  for (int l = P.nlevels - 1; l >= 0; --l)
+    for (int q = P.boff[l]; q < P.boff[l + 1]; ++q) {
+      const int s = P.btiles[4 * q], r0 = P.btiles[4 * q + 1], nr = P.btiles[4 * q + 2];
+      const int ns = P.ns[s], ms = P.ms[s], p0 = P.p0[s];
+      vec.assign(ns + ms, 0.0);
+      for (int pos = 0; pos < ns; ++pos) vec[pos] = Y[p0 + pos];
+      for (int i = 0; i < ms; ++i) vec[ns + i] = X[P.rows[P.rows_off[s] + i]];
+      for (int lr = r0; lr < r0 + nr; ++lr) {
+        const double* ptr = &P.bwd[P.bwd_off[s] + (int64_t)lr * (ns + ms) - (int64_t)lr * (lr - 1) / 2 - lr];
+        double a = 0;
+        for (int pos = lr; pos < ns + ms; ++pos) a += ptr[pos] * vec[pos];
+        X[p0 + lr] = a;
+      }
+    }
+  for (int i = 0; i < P.n; ++i) x[i] = X[i];
+}
+
 static double check(const char* name, const BlockSPD& S, int leaf) {
   auto t0 = std::chrono::steady_clock::now();
   SupernodalFactor F = cholesky_factor(S, leaf);
@@ -86,6 +133,35 @@ static double check(const char* name, const BlockSPD& S, int leaf) {
     nx += x[i] * x[i];
   }
   double rel = std::sqrt(nr / nb);
+  {
+    PackedFactor P = pack_factor(F);
+    std::vector<double> xp(F.n), rp(F.n);
+    packed_solve_host(P, b.data(), xp.data());
+    block_matvec(S, F.cell_base, xp.data(), rp.data());
+    double nr2 = 0;
+    for (int i = 0; i < F.n; ++i) nr2 += (rp[i] - b[i]) * (rp[i] - b[i]);
+    const double rel2 = std::sqrt(nr2 / nb);
+    std::printf("   packed sweeps: resid=%.3e\n", rel2);
+    if (std::getenv("PECS_LEVELS"))
+      for (int l = 0; l < P.nlevels; ++l) {
+        double fb = 0, bb = 0;
+        int nodes = 0, rowsf = 0, rowsb = 0, maxns = 0, maxms = 0;
+        for (int q = P.foff[l]; q < P.foff[l + 1]; ++q) {
+          const int s = P.ftiles[4 * q], r0 = P.ftiles[4 * q + 1], nr = P.ftiles[4 * q + 2];
+          for (int r = r0; r < r0 + nr; ++r) fb += 8.0 * (r < P.ns[s] ? r + 1 : P.ns[s]);
+          if (r0 == 0) { nodes++; maxns = std::max(maxns, P.ns[s]); maxms = std::max(maxms, P.ms[s]); }
+          rowsf += nr;
+        }
+        for (int q = P.boff[l]; q < P.boff[l + 1]; ++q) {
+          const int s = P.btiles[4 * q], r0 = P.btiles[4 * q + 1], nr = P.btiles[4 * q + 2];
+          for (int j = r0; j < r0 + nr; ++j) bb += 8.0 * (P.ns[s] - j + P.ms[s]);
+          rowsb += nr;
+        }
+        std::printf("   level %2d nodes %5d maxns %5d maxms %5d fwd rows %8d %8.1f MB W%d  bwd rows %8d %8.1f MB W%d\n",
+                    l, nodes, maxns, maxms, rowsf, fb / 1e6, P.fw[l], rowsb, bb / 1e6, P.bw[l]);
+      }
+    rel = std::max(rel, rel2);
+  }
   std::printf("%s: n=%d nodes=%zu levels=%d nnzL=%lld (%.1f MB) flops=%.3g factor %.2fs  resid=%.3e err=%.3e\n", name,
               F.n, F.nodes.size(), F.nlevels, (long long)F.nnz_L(), F.nnz_L() * 8e-6, F.flops, ts, rel,
               std::sqrt(ne / nx));
