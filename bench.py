@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
+                    help="only run a plain device copy of this many GiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -69,6 +71,15 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
+    if args.copy_calibration_gib > 0:
+        n = int(args.copy_calibration_gib * (1 << 30)) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n}), flush=True)
+        return
     p = pecs_amd.workload_params(args.refine, args.degree, args.dt)
     t0 = time.perf_counter()
     g = pecs_amd.SolarCellGPU(p)

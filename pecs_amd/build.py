@@ -2,7 +2,8 @@
 
 Host-only setup code (meshes, local operators, nested dissection, multifrontal
 Cholesky) is compiled with g++ -fopenmp; device code and the C-ABI with hipcc
-for gfx950.  Outputs go to pecs_amd/ (the .so) and pecs_amd/_build/.
+for gfx950.  The C++ host classes and the pecs_run driver (pecs_amd/host) link
+against the library.  Outputs go to pecs_amd/ (the .so) and pecs_amd/_build/.
 """
 import os
 import shutil
@@ -14,6 +15,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpecs_amd.so")
 SELFTEST = os.path.join(BUILD, "pecs_selftest")
+HOST = os.path.join(HERE, "host")
+RUNNER = os.path.join(BUILD, "pecs_run")
+HOST_TEST = os.path.join(BUILD, "pecs_host_test")
 
 HOST_SRC = ["mesh.cpp", "ops.cpp", "cholesky.cpp"]
 HIP_SRC = ["kernels.hip", "abi.hip"]
@@ -48,6 +52,10 @@ def build(verbose=False):
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-o", tmp] + hobjs + objs + ["-lgomp"], verbose)
     os.replace(tmp, LIB)
     _run(["g++"] + CXXFLAGS + [os.path.join(CSRC, "selftest.cpp")] + objs + ["-o", SELFTEST], verbose)
+    # host classes in the reference roles + the command-line driver, linked to the library
+    link = ["-L" + HERE, "-lpecs_amd", "-Wl,-rpath,$ORIGIN/.."]
+    _run(["g++", "-O2", "-std=c++17", os.path.join(HOST, "main.cpp"), "-o", RUNNER] + link, verbose)
+    _run(["g++", "-O2", "-std=c++17", os.path.join(HOST, "host_test.cpp"), "-o", HOST_TEST] + link, verbose)
     return LIB
 
 

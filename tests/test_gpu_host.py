@@ -1,0 +1,151 @@
+"""GPU tests of the C++ host layer: pecs_run (the main.cpp role) reads an
+input_file.prm, runs the IMEX loop on the GPU and writes <Name>.dofs restart
+files (CarrierPair.cpp:89-121); the result must match the oracle to the
+trajectory bar (1e-10 relative), the per-call host-vector loop must match the
+device-resident loop, and a restart from the written files must continue the
+same trajectory (SolarCell.cpp:1978-1995).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.solarcell import OracleSolarCell, default_params
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUNNER = os.path.join(ROOT, "pecs_amd", "_build", "pecs_run")
+HOST_TEST = os.path.join(ROOT, "pecs_amd", "_build", "pecs_host_test")
+
+# Characteristic scales chosen so the scaled coefficients are O(1): Debye
+# length^2 = V_T eps_0 / (q rho L^2) ~ 1 for rho = 1.4e4, L = 1.
+PRM = """subsection computational
+  set global refinements = 3
+  set time step size     = 0.002
+  set end time           = 0.01
+  set end time 2         = 0.016
+  set time stamps        = 2
+  set restart status     = {restart}
+end
+subsection mesh
+  set radius one  = 0.4
+  set radius two  = 0.55
+  set mesh height = 1
+  set mesh length = 1.2
+end
+subsection physical
+  set characteristic length      = 1.0
+  set characteristic time        = 1.0
+  set characteristic density     = 1.4e4
+  set intrinsic density          = 1.4e3
+  set photon flux                = 1.4e4
+  set absorption coefficient     = 1.0
+  set applied bias               = 0.003
+  set built in bias              = 0.01
+  set schottky bias              = 0.002
+  set illumination status        = true
+  set insulated                  = true
+  set schottky status            = true
+  set semiconductor permittivity = 2.0
+  set electrolyte permittivity   = 3.0
+end
+subsection electrons
+  set mobility = 40
+  set transfer rate = 1e-6
+  set recombination velocity = 0.3
+end
+subsection holes
+  set mobility = 30
+  set transfer rate = 2e-6
+  set recombination velocity = 0.2
+end
+subsection reductants
+  set mobility = 20
+end
+subsection oxidants
+  set mobility = 50
+end
+"""
+NAMES = {"n": "Electrons", "p": "Holes", "r": "Reductants", "o": "Oxidants"}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _read_dofs(path):
+    raw = open(path, "rb").read()
+    head, body = raw.split(b"\n[", 1)
+    n = int(head)
+    assert body[-1:] == b"]" and len(body) == 8 * n + 1
+    return np.frombuffer(body[:-1], dtype="<f8").copy()
+
+
+def _oracle_params(prm_file, degree):
+    out = subprocess.run([HOST_TEST, "prm", str(prm_file), str(degree)], capture_output=True, text=True,
+                         check=True).stdout
+    kv = {k: float(v) for k, v in (line.split("=") for line in out.strip().splitlines())}
+    p = default_params()
+    p.update(k=degree, r=int(kv["n_refine"]), r1=kv["r1"], r2=kv["r2"], H=kv["H"], L=kv["L"],
+             insulated=bool(kv["insulated"]), schottky=bool(kv["schottky"]), dt=kv["dt"],
+             mu_n=kv["mu0"], mu_p=kv["mu1"], mu_r=kv["mu2"], mu_o=kv["mu3"], eps_s=kv["eps_s"],
+             eps_e=kv["eps_e"], lam2=kv["lambda2"], k_et=kv["k_et"], k_ht=kv["k_ht"], v_rec_n=kv["v_rec_n"],
+             v_rec_p=kv["v_rec_p"], ne=kv["n_e"], pe=kv["p_e"], rinf=kv["r_inf"], oinf=kv["o_inf"],
+             phi_bi=kv["phi_bi"], phi_app=kv["phi_app"], phi_sch=kv["phi_sch"], gen_mode="ref", G0=kv["G0"],
+             alpha=kv["alpha"], srh=False)
+    assert kv["gen_mode"] == 0
+    return p
+
+
+def _run(args, timeout=300):
+    r = subprocess.run([RUNNER] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_cli_run_restart_and_host_loop(tmp_path):
+    degree = 1
+    prm = tmp_path / "input_file.prm"
+    prm.write_text(PRM.format(restart="false"))
+    dev, host = tmp_path / "dev", tmp_path / "host"
+    dev.mkdir()
+    host.mkdir()
+    _say("[gpu] pecs_run device loop, reference step count")
+    out = _run([prm, "--degree", degree, "--out", dev])
+    assert "output 2: step" in out
+    steps = int(out.split("output 2: step")[1].split(",")[0])
+    _say(f"[gpu] pecs_run host loop, {steps} steps")
+    _run([prm, "--degree", degree, "--out", host, "--host-loop"])
+
+    p = _oracle_params(prm, degree)
+    o = OracleSolarCell(p)
+    st = o.run(steps)
+    for c, name in NAMES.items():
+        d = _read_dofs(dev / (name + ".dofs"))
+        h = _read_dofs(host / (name + ".dofs"))
+        assert _rel(d, st[c]) < 1e-10, (name, _rel(d, st[c]))
+        assert _rel(h, d) < 1e-12, (name, _rel(h, d))
+
+    _say("[gpu] pecs_run restart from the written .dofs")
+    prm.write_text(PRM.format(restart="true"))
+    out = _run([prm, "--degree", degree, "--out", dev])
+    steps2 = int(out.split("output 2: step")[1].split(",")[0])
+    assert steps2 >= 2
+    st2 = o.run(steps2, st)
+    for c, name in NAMES.items():
+        d = _read_dofs(dev / (name + ".dofs"))
+        assert _rel(d, st2[c]) < 1e-10, (name, _rel(d, st2[c]))
+
+
+def test_cli_steady_stop(tmp_path):
+    prm = tmp_path / "input_file.prm"
+    prm.write_text(PRM.format(restart="false"))
+    _say("[gpu] pecs_run steady-state stop")
+    out = _run([prm, "--degree", 1, "--steps", 40, "--steady-tol", 1e30, "--check-every", 3, "--out", tmp_path])
+    assert "steady state reached" in out and "3 steps in" in out
