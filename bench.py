@@ -13,6 +13,12 @@ triangular solves of the semiconductor Schur factor, two right-hand sides),
 timed live with hipEvents on the context stream over the timed steps.
 cpu_baseline = the numpy/scipy oracle (a port of the reference algorithm,
 single core) on a bounded sample, reported beside it.
+
+time_to_steady_state: wall seconds from the equilibrium initial state until
+the relative steady residual ||x^{n+1}-x^n|| / (||x^n|| dt) over the five
+state vectors falls to 1e-8 (the reference has no criterion of its own),
+on the measured workload and, beside the oracle on one core, on the small
+workload (refinement 4, k = 1, dt = 1e-3).
 """
 import argparse
 import json
@@ -48,6 +54,43 @@ def cpu_baseline(seconds_budget=20.0):
                       f"{n} steps, U={U} DOFs/step, setup excluded"}
 
 
+def cpu_time_to_steady(tol, max_steps=20000):
+    """Oracle (one core) on the small workload: refinement 4, k = 1, dt = 1e-3."""
+    import numpy as np
+    from oracle.solarcell import OracleSolarCell
+    from pecs_amd import workload_params
+    o = OracleSolarCell(workload_params(4, 1, 1e-3))
+    st = o.initial_state()
+    t0 = time.perf_counter()
+    vec = lambda s: np.concatenate([s[k] for k in "nproP"])
+    n, res = 0, float("inf")
+    while n < max_steps and res > tol:
+        new = o.step(st)
+        a, b = vec(new), vec(st)
+        res = float(np.linalg.norm(a - b) / (np.linalg.norm(b) * 1e-3))
+        st = new
+        n += 1
+    return {"s": time.perf_counter() - t0, "steps": n, "residual": res, "cores": 1, "kind": "port",
+            "reached": res <= tol}
+
+
+def gpu_time_to_steady(g, tol, check_every, max_steps):
+    """Steps from the equilibrium initial state until the relative steady
+    residual ||x^{n+1}-x^n|| / (||x^n|| dt) <= tol, checked every check_every
+    steps (the residual is that of the last step of each block)."""
+    import torch
+    g.initial_state()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n, res = 0, float("inf")
+    while n < max_steps and res > tol:
+        k = min(check_every, max_steps - n)
+        g.step(k)
+        n += k
+        res = g.residual()
+    return {"s": time.perf_counter() - t0, "steps": n, "residual": res, "reached": res <= tol}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +101,10 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--steady-tol", type=float, default=1e-8)
+    ap.add_argument("--steady-max-steps", type=int, default=8000)
+    ap.add_argument("--steady-check-every", type=int, default=25)
+    ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady-state legs")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -102,26 +149,31 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     res = g.residual()
-    ph = (ctypes.c_double * 11)()
+    ph = (ctypes.c_double * 8)()
     nst = ctypes.c_int()
-    L.pecs_phase_times(g.ctx, ph, 11, ctypes.byref(nst))
-    phase = [ph[i] / max(nst.value, 1) for i in range(11)]
-    names = ["rhs_semi", "rhs_elec", "schur_rhs_semi", "chol_semi", "recover_semi", "schur_rhs_elec",
-             "chol_elec", "recover_elec", "poisson_cond", "chol_poisson", "poisson_recover"]
+    L.pecs_phase_times(g.ctx, ph, 8, ctypes.byref(nst))
+    phase = [ph[i] / max(nst.value, 1) for i in range(8)]
+    names = ["rhs_semi", "rhs_elec", "schur_transforms", "carrier_cholesky", "recoveries", "poisson_cond",
+             "chol_poisson", "poisson_recover"]
     print("[bench] per-step phase ms: " + ", ".join(f"{n}={v:.3f}" for n, v in zip(names, phase)),
           file=sys.stderr, flush=True)
+    # the dominant phase: forward + backward sweeps with both regions' Schur
+    # factors (run concurrently), two right-hand sides per factor read;
+    # vectors per region: 2 rhs x (read b, write y, read y, write x)
     fb = ctypes.c_double()
-    L.pecs_factor_bytes(g.ctx, 0, ctypes.byref(fb))
-    # two right-hand sides share each factor read; vectors: 2 rhs x (read b, write y, read y, write x)
-    n_schur = st["n_schur_semi"]
-    chol_bytes = fb.value + 2 * 4 * 8.0 * n_schur
+    chol_bytes = 0.0
+    # (the workload's carriers of a region have equal mobilities, so each
+    # region has one Schur factor serving both of its carriers)
+    for which, n_schur in ((0, g.n_semi // 3), (1, g.n_elec // 3)):
+        L.pecs_factor_bytes(g.ctx, which, ctypes.byref(fb))
+        chol_bytes += fb.value + 2 * 4 * 8.0 * n_schur
     chol_ms = phase[3]
     achieved = chol_bytes / (chol_ms * 1e-3) / 1e9 if chol_ms > 0 else 0.0
     U = 2 * g.n_semi + 2 * g.n_elec + g.n_poisson
     ms_per_step = elapsed / args.steps * 1e3
     value = U * args.steps / elapsed
     out = {
-        "metric": "DOF-updates/s (IMEX step: 4 carrier + Poisson unknowns advanced per step)",
+        "metric": "time-to-steady-state per bias point (s) + DOF-updates/s, GPU vs plain path",
         "value": value,
         "unit": "DOF-updates/s",
         "n_gpus": 1,
@@ -139,13 +191,32 @@ def main():
                    "dofs_per_step": U, "n_carrier_semi": g.n_semi, "n_carrier_elec": g.n_elec,
                    "n_poisson": g.n_poisson, "setup_s": round(setup_s, 2),
                    "steady_residual_last_step": res,
+                   "factor_stats": {k: v for k, v in st.items() if k not in ("setup_s",)},
                    "phase_ms_per_step": {n: round(v, 4) for n, v in zip(names, phase)}},
-        "roofline": {"kernel": "supernodal triangular solves (fwd+bwd) of the semiconductor Schur factor, 2 rhs",
+        "roofline": {"kernel": "supernodal triangular sweeps (fwd+bwd) with the semiconductor and electrolyte "
+                               "Schur factors, 2 rhs each, one step",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": chol_bytes, "avg_ms": chol_ms},
     }
+    if not args.no_steady:
+        tts = gpu_time_to_steady(g, args.steady_tol, args.steady_check_every, args.steady_max_steps)
+        tts["workload"] = out["config"]["workload"]
+        print(f"[bench] time to steady state: {tts}", file=sys.stderr, flush=True)
+        out["time_to_steady_state"] = {"tol": args.steady_tol, "gpu": tts}
     g.close()
+    if not args.no_steady:
+        small = pecs_amd.SolarCellGPU(pecs_amd.workload_params(4, 1, 1e-3))
+        t_small = gpu_time_to_steady(small, args.steady_tol, args.steady_check_every, 20000)
+        small.close()
+        t_small["workload"] = "refinement 4 (16x16 quads/region), LDG k=1, RT0xDGQ0 Poisson, dt=0.001"
+        out["time_to_steady_state"]["small_gpu"] = t_small
+        if not args.no_cpu_baseline:
+            c = cpu_time_to_steady(args.steady_tol)
+            c["workload"] = t_small["workload"]
+            out["time_to_steady_state"]["small_cpu"] = c
+        print(f"[bench] small workload time to steady state: {out['time_to_steady_state']}", file=sys.stderr,
+              flush=True)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -131,14 +131,22 @@ int pecs_last_residual(pecs_ctx* ctx, double* residual);
 
 /* Per-phase timing of pecs_step with hipEvents on the context stream
  * (the TimerOutput sections of SolarCell.cpp:1905-2087): phases 0 rhs semi,
- * 1 rhs elec, 2 Schur transform semi, 3 Cholesky semi, 4 recover semi,
- * 5-7 the same for the electrolyte, 8 Poisson condensation, 9 Poisson
- * multiplier Cholesky, 10 Poisson recovery.  ms[] accumulates since enable. */
+ * 1 rhs elec, 2 Schur transforms of both regions, 3 carrier Cholesky solves of
+ * both regions (concurrent on two streams), 4 recoveries of both regions,
+ * 5 Poisson condensation, 6 Poisson multiplier Cholesky, 7 Poisson recovery.
+ * ms[] accumulates since enable. */
 int pecs_enable_timing(pecs_ctx* ctx, int enable);
 int pecs_phase_times(pecs_ctx* ctx, double* ms, int n, int* steps);
 /* Algorithmic bytes of one forward + backward triangular sweep with a factor
  * (0 semiconductor Schur, 1 electrolyte Schur, 2 Poisson multipliers). */
 int pecs_factor_bytes(pecs_ctx* ctx, int which, double* bytes);
+
+/* Tile shapes chosen at setup for the triangular sweeps of a factor (0
+ * semiconductor Schur, 1 electrolyte Schur, 2 Poisson multipliers): per level
+ * l, out[6l..6l+5] = forward lanes per row, row passes, rows per tile, then the
+ * same for the backward sweep.  No counterpart in the reference (UMFPACK's
+ * solve has no tunable shape). */
+int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels);
 
 /* Device-pointer variants (no copies, no synchronisation). */
 int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,

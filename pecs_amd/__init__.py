@@ -23,6 +23,7 @@ ABI_FUNCTIONS = [
     "pecs_rhs_electrolyte", "pecs_rhs_poisson", "pecs_solve", "pecs_set_state", "pecs_get_state",
     "pecs_initial_state", "pecs_step", "pecs_last_residual", "pecs_rhs_semiconductor_dev",
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
+    "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config",
 ]
 
 
@@ -229,6 +230,17 @@ class SolarCellGPU:
         keys = ["factor_semi_doubles", "factor_elec_doubles", "factor_poisson_doubles", "setup_s", "factor_s",
                 "levels_semi", "levels_elec", "levels_poisson", "n_schur_semi", "n_multipliers"]
         return dict(zip(keys, out.tolist()))
+
+    def sweep_config(self, which):
+        """Tile shapes of the triangular sweeps per level: (fwd lanes, fwd passes,
+        fwd rows/tile, bwd lanes, bwd passes, bwd rows/tile)."""
+        out = (ctypes.c_int * (6 * 64))()
+        nl = ctypes.c_int()
+        L = lib()
+        L.pecs_sweep_config.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int)]
+        self._c(L.pecs_sweep_config(self.ctx, which, out, 6 * 64, ctypes.byref(nl)))
+        return [tuple(out[6 * l:6 * l + 6]) for l in range(nl.value)]
 
     @staticmethod
     def _f(a):

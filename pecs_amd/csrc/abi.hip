@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -25,13 +26,14 @@ thread_local std::string g_error;
 
 struct FactorDev {
   DevFactor d{};
-  std::vector<int> foff, boff, goff, xoff, fw, bw;  // per level (host)
+  std::vector<int> foff, boff, goff, xoff, fw, bw, fm, bm, frows, brows;  // per level (host)
   int nlevels = 0, n = 0;
   double nnz = 0.0;
   double tri_bytes = 0.0;      // algorithmic bytes of a forward + backward sweep
   std::vector<int> cell_base;  // host copy: grid cell -> permuted base
   FactorLevels lv() const {
-    return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(), bw.data(), nlevels};
+    return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(),
+                        bw.data(),   fm.data(),   bm.data(),   nlevels};
   }
 };
 
@@ -40,6 +42,8 @@ struct FactorDev {
 struct pecs_ctx {
   std::string err;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;            // second stream for the concurrent region solves
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   Params P;
   int k = 1;
   SolarCellMesh mesh;
@@ -75,6 +79,12 @@ struct pecs_ctx {
   int timed_steps = 0;
 
   ~pecs_ctx() {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
   }
@@ -158,7 +168,105 @@ void validate(const pecs_mesh* m, const pecs_params* q) {
   if (!(q->eps_s > 0 && q->eps_e > 0 && q->lambda2 > 0)) throw std::runtime_error("permittivities/lambda2 must be positive");
 }
 
-FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf) {
+// Pick, per level and sweep direction, the tile shape (lanes per row W, row
+// passes MAXR, rows per tile) with the shortest measured sweep on this GPU.
+// Each candidate is timed on the real factor level with the number of
+// right-hand sides it will be used with; a 512 MiB fill between timed runs
+// keeps the level's factor out of the caches, as it is in a solve.  Inputs
+// are zero vectors; outputs go to the factor's own scratch (X, Z) and to a
+// temporary Y.  PECS_NO_TUNE=1 keeps the default shapes of pack_factor.
+void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
+  const char* env = std::getenv("PECS_NO_TUNE");
+  if (env && env[0] == '1') return;
+  const size_t scrub_bytes = (size_t)512 << 20;
+  void* scrub = nullptr;
+  double* Y = nullptr;
+  int4* tbuf = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  size_t tcap = 0;
+  for (int l = 0; l < P.nlevels; ++l) {
+    size_t rows = 0;
+    for (int q = P.lnoff[l]; q < P.lnoff[l + 1]; ++q) rows += P.ns[P.lnodes[q]] + P.ms[P.lnodes[q]];
+    tcap = std::max(tcap, rows);  // at most one tile per row
+  }
+  auto release = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (scrub) (void)hipFree(scrub);
+    if (Y) (void)hipFree(Y);
+    if (tbuf) (void)hipFree(tbuf);
+  };
+  try {
+    HIPCHECK(hipMalloc(&scrub, scrub_bytes));
+    HIPCHECK(hipMalloc((void**)&Y, 2 * (size_t)P.n * sizeof(double)));
+    HIPCHECK(hipMalloc((void**)&tbuf, std::max<size_t>(tcap, 1) * sizeof(int4)));
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    hipStream_t s = c->stream;
+    HIPCHECK(hipMemsetAsync(Y, 0, 2 * (size_t)P.n * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(d.X, 0, 2 * (size_t)P.n * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(d.Z, 0, 2 * (size_t)std::max<int64_t>(P.zlen, 1) * sizeof(double), s));
+    std::vector<int> tl;
+    std::vector<int4> t4;
+    for (int l = 0; l < P.nlevels; ++l) {
+      for (int dir = 0; dir < 2; ++dir) {
+        const bool fwd = dir == 0;
+        float best = 1e30f;
+        int bw = fwd ? P.fw[l] : P.bw[l], bm = fwd ? P.fm[l] : P.bm[l], br = fwd ? P.frows[l] : P.brows[l];
+        for (int w : {8, 16, 32, 64}) {
+          for (int m : {1, 2, 4}) {
+            for (int half = 0; half < 2; ++half) {
+              const int R = 256 / w;
+              const int rows = (R * m) >> half;
+              if (rows < R) continue;
+              tl.clear();
+              level_tiles(P, l, fwd, rows, tl);
+              const int nt = (int)(tl.size() / 4);
+              if (nt == 0) continue;
+              t4.resize(nt);
+              for (int t = 0; t < nt; ++t) t4[t] = make_int4(tl[4 * t], tl[4 * t + 1], tl[4 * t + 2], 0);
+              HIPCHECK(hipMemcpyAsync(tbuf, t4.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
+              float tmin = 1e30f;
+              for (int rep = 0; rep < 3; ++rep) {
+                HIPCHECK(hipMemsetAsync(scrub, rep, scrub_bytes, s));
+                HIPCHECK(hipEventRecord(e0, s));
+                HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, Y + P.n, Y, Y + P.n, s));
+                HIPCHECK(hipEventRecord(e1, s));
+                HIPCHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                tmin = std::min(tmin, ms);
+              }
+              if (tmin < best) {
+                best = tmin;
+                bw = w;
+                bm = m;
+                br = rows;
+              }
+            }
+          }
+        }
+        if (fwd) {
+          P.fw[l] = bw;
+          P.fm[l] = bm;
+          P.frows[l] = br;
+        } else {
+          P.bw[l] = bw;
+          P.bm[l] = bm;
+          P.brows[l] = br;
+        }
+      }
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
+  rebuild_tiles(P);
+}
+
+FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   PackedFactor P;
   {
     SupernodalFactor F = cholesky_factor(S, leaf);
@@ -168,53 +276,47 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf) {
   D.n = P.n;
   D.nlevels = P.nlevels;
   D.cell_base = P.cell_base;
-  D.foff = P.foff;
-  D.boff = P.boff;
   D.goff = P.goff;
   D.xoff = P.xoff;
-  D.fw = P.fw;
-  D.bw = P.bw;
   D.nnz = (double)P.fwd.size();
   D.tri_bytes = P.tri_bytes;
   const int nn = (int)P.p0.size();
-  const auto& p0 = P.p0;
-  const auto& ns = P.ns;
-  const auto& ms = P.ms;
-  const auto& fo = P.fwd_off;
-  const auto& bo = P.bwd_off;
-  const auto& ro = P.rows_off;
-  const auto& zo = P.z_off;
-  const auto& fwd = P.fwd;
-  const auto& bwd = P.bwd;
-  const auto& czidx = P.czidx;
-  const int64_t zl = P.zlen;
   DevFactor& d = D.d;
   d.n = P.n;
   d.nnodes = nn;
-  d.zlen = zl;
-  d.p0 = upload(c, p0);
-  d.ns = upload(c, ns);
-  d.ms = upload(c, ms);
-  d.fwd_off = upload(c, fo);
-  d.bwd_off = upload(c, bo);
-  d.rows_off = upload(c, ro);
-  d.z_off = upload(c, zo);
-  d.fwd = upload(c, fwd);
-  d.bwd = upload(c, bwd);
+  d.zlen = P.zlen;
+  d.p0 = upload(c, P.p0);
+  d.ns = upload(c, P.ns);
+  d.ms = upload(c, P.ms);
+  d.fwd_off = upload(c, P.fwd_off);
+  d.bwd_off = upload(c, P.bwd_off);
+  d.rows_off = upload(c, P.rows_off);
+  d.z_off = upload(c, P.z_off);
+  d.fwd = upload(c, P.fwd);
+  d.bwd = upload(c, P.bwd);
   d.rows = upload(c, P.rows);
   d.cptr = upload(c, P.cptr);
-  d.czidx = upload(c, czidx);
+  d.czidx = upload(c, P.czidx);
+  d.lrows = upload(c, P.lrows);
+  d.xidx = upload(c, P.xidx);
+  d.Z = dalloc<double>(c, 2 * (size_t)std::max<int64_t>(P.zlen, 1));
+  d.X = dalloc<double>(c, 2 * (size_t)P.n);
+  tune_factor(c, P, d, nrhs);
   std::vector<int4> ft(P.ftiles.size() / 4), bt(P.btiles.size() / 4);
   for (size_t t = 0; t < ft.size(); ++t)
     ft[t] = make_int4(P.ftiles[4 * t], P.ftiles[4 * t + 1], P.ftiles[4 * t + 2], 0);
   for (size_t t = 0; t < bt.size(); ++t)
     bt[t] = make_int4(P.btiles[4 * t], P.btiles[4 * t + 1], P.btiles[4 * t + 2], 0);
   d.ftiles = upload(c, ft);
-  d.lrows = upload(c, P.lrows);
-  d.xidx = upload(c, P.xidx);
   d.btiles = upload(c, bt);
-  d.Z = dalloc<double>(c, 2 * (size_t)std::max<int64_t>(zl, 1));
-  d.X = dalloc<double>(c, 2 * (size_t)P.n);
+  D.foff = P.foff;
+  D.boff = P.boff;
+  D.fw = P.fw;
+  D.bw = P.bw;
+  D.fm = P.fm;
+  D.bm = P.bm;
+  D.frows = P.frows;
+  D.brows = P.brows;
   return D;
 }
 
@@ -237,33 +339,6 @@ DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, c
   for (int i = 0; i < R.ncell; ++i) pb[i] = F.cell_base[R.ci[i] + R.N * R.cj[i]];
   d.perm_base = upload(c, pb);
   return d;
-}
-
-void solve_region(pecs_ctx* c, int reg, const double* b0, const double* b1, double* x0, double* x1, int nrhs) {
-  const DevRegion& R = reg == 0 ? c->Rs : c->Re;
-  FactorDev* F = reg == 0 ? c->Fs : c->Fe;
-  const bool share = reg == 0 ? c->share_s : c->share_e;
-  const double mu0 = c->P.mu[2 * reg], mu1 = c->P.mu[2 * reg + 1];
-  double* W0 = c->W[2 * reg];
-  double* W1 = c->W[2 * reg + 1];
-  hipStream_t s = c->stream;
-  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, mu0, W0, s));
-  if (nrhs == 2) HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, mu1, W1, s));
-  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg][0], c->g[reg][1], nrhs, s));
-  if (nrhs == 2 && share) {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], c->g[reg][1],
-                                           c->t[reg][0], c->t[reg][1], 2, s));
-  } else {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], nullptr,
-                                           c->t[reg][0], nullptr, 1, s));
-    if (nrhs == 2) {
-      const FactorDev& F1 = share ? F[0] : F[1];
-      HIPCHECK((hipError_t)launch_chol_solve(F1.d, F1.lv(), c->g[reg][1], nullptr, c->t[reg][1],
-                                             nullptr, 1, s));
-    }
-  }
-  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W0, W1, c->g[reg][0], c->g[reg][1], mu0, mu1, x0, x1, b0, b1,
-                                            nrhs, s));
 }
 
 // single-carrier solve with its own factor
@@ -298,12 +373,13 @@ void poisson_from_densities(pecs_ctx* c, const double* un, const double* up, con
   poisson_solve_from_y(c, out);
 }
 
-// Phases of one step, in stream order (timing mode records an event after each):
-// 0 rhs semiconductor, 1 rhs electrolyte, 2 Schur transform semi, 3 Cholesky
-// semi, 4 recover semi, 5 Schur transform elec, 6 Cholesky elec, 7 recover
-// elec, 8 Poisson condensation, 9 Poisson multiplier Cholesky, 10 Poisson
-// recovery.
-constexpr int NPHASE = 11;
+// Phases of one step (timing mode records an event on the context stream
+// after each): 0 rhs semiconductor, 1 rhs electrolyte, 2 Schur transforms of
+// both regions, 3 the carrier Cholesky solves of both regions (run
+// concurrently on the context stream and a second stream, joined before the
+// mark), 4 recoveries of both regions, 5 Poisson condensation, 6 Poisson
+// multiplier Cholesky, 7 Poisson recovery.
+constexpr int NPHASE = 8;
 
 struct PhaseMarks {
   pecs_ctx* c;
@@ -314,32 +390,31 @@ struct PhaseMarks {
   }
 };
 
-void solve_region_marked(pecs_ctx* c, int reg, const double* b0, const double* b1, double* x0, double* x1,
-                         PhaseMarks& pm) {
+void region_schur(pecs_ctx* c, int reg, const double* b0, const double* b1, hipStream_t s) {
   const DevRegion& R = reg == 0 ? c->Rs : c->Re;
+  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, c->P.mu[2 * reg], c->W[2 * reg], s));
+  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, c->P.mu[2 * reg + 1], c->W[2 * reg + 1], s));
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, c->W[2 * reg], c->W[2 * reg + 1], c->g[reg][0],
+                                        c->g[reg][1], 2, s));
+}
+
+void region_chol(pecs_ctx* c, int reg, hipStream_t s) {
   FactorDev* F = reg == 0 ? c->Fs : c->Fe;
   const bool share = reg == 0 ? c->share_s : c->share_e;
-  const double mu0 = c->P.mu[2 * reg], mu1 = c->P.mu[2 * reg + 1];
-  double* W0 = c->W[2 * reg];
-  double* W1 = c->W[2 * reg + 1];
-  hipStream_t s = c->stream;
-  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, mu0, W0, s));
-  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, mu1, W1, s));
-  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg][0], c->g[reg][1], 2, s));
-  pm.mark();
   if (share) {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], c->g[reg][1],
-                                           c->t[reg][0], c->t[reg][1], 2, s));
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], c->g[reg][1], c->t[reg][0],
+                                           c->t[reg][1], 2, s));
   } else {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], nullptr,
-                                           c->t[reg][0], nullptr, 1, s));
-    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].lv(), c->g[reg][1], nullptr,
-                                           c->t[reg][1], nullptr, 1, s));
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], nullptr, c->t[reg][0], nullptr, 1, s));
+    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].lv(), c->g[reg][1], nullptr, c->t[reg][1], nullptr, 1, s));
   }
-  pm.mark();
-  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W0, W1, c->g[reg][0], c->g[reg][1], mu0, mu1, x0, x1, b0, b1,
-                                            2, s));
-  pm.mark();
+}
+
+void region_recover(pecs_ctx* c, int reg, const double* b0, const double* b1, double* x0, double* x1,
+                    hipStream_t s) {
+  const DevRegion& R = reg == 0 ? c->Rs : c->Re;
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, c->W[2 * reg], c->W[2 * reg + 1], c->g[reg][0], c->g[reg][1],
+                                            c->P.mu[2 * reg], c->P.mu[2 * reg + 1], x0, x1, b0, b1, 2, s));
 }
 
 void step_once(pecs_ctx* c) {
@@ -356,14 +431,26 @@ void step_once(pecs_ctx* c) {
   HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->ph, c->u[2], c->u[3],
                                           c->u[0], c->u[1], c->pv, c->b[2], c->b[3], s));
   pm.mark();
-  solve_region_marked(c, 0, c->b[0], c->b[1], c->u[0], c->u[1], pm);
-  solve_region_marked(c, 1, c->b[2], c->b[3], c->u[2], c->u[3], pm);
+  region_schur(c, 0, c->b[0], c->b[1], s);
+  region_schur(c, 1, c->b[2], c->b[3], s);
+  pm.mark();
+  // the two regions' solves share no data: the electrolyte runs on the side
+  // stream so its launches fill the GPU where the semiconductor's leave gaps
+  HIPCHECK(hipEventRecord(c->fork_ev, s));
+  HIPCHECK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+  region_chol(c, 0, s);
+  region_chol(c, 1, c->side);
+  HIPCHECK(hipEventRecord(c->join_ev, c->side));
+  HIPCHECK(hipStreamWaitEvent(s, c->join_ev, 0));
+  pm.mark();
+  region_recover(c, 0, c->b[0], c->b[1], c->u[0], c->u[1], s);
+  region_recover(c, 1, c->b[2], c->b[3], c->u[2], c->u[3], s);
+  pm.mark();
   HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->ph, c->u[0], c->u[1], c->u[2], c->u[3],
                                            c->mesh.semi.ncell, c->mesh.elec.ncell, c->yP, s));
   pm.mark();
   HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, nullptr, c->tH, nullptr, 1,
-                                         s));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, nullptr, c->tH, nullptr, 1, s));
   pm.mark();
   HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s));
   pm.mark();
@@ -434,6 +521,9 @@ int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
       throw std::runtime_error("no HIP device: the PECS path runs on the GPU only");
+    HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     c->P = to_params(params);
     c->k = params->degree;
     const int leaf = params->leaf_cells > 0 ? params->leaf_cells : 16;
@@ -451,11 +541,11 @@ int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out
     auto tf = std::chrono::steady_clock::now();
     c->share_s = c->P.mu[0] == c->P.mu[1];
     c->share_e = c->P.mu[2] == c->P.mu[3];
-    c->Fs[0] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[0]), leaf);
-    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[1]), leaf);
-    c->Fe[0] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[2]), leaf);
-    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[3]), leaf);
-    c->FH = make_factor(c.get(), multiplier_matrix(c->mesh, op, T), leaf);
+    c->Fs[0] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[0]), leaf, c->share_s ? 2 : 1);
+    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[1]), leaf, 1);
+    c->Fe[0] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[2]), leaf, c->share_e ? 2 : 1);
+    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[3]), leaf, 1);
+    c->FH = make_factor(c.get(), multiplier_matrix(c->mesh, op, T), leaf, 1);
     c->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
     // reference tables
     DevTabs& tb = c->tabs;
@@ -836,6 +926,18 @@ int pecs_step(pecs_ctx* ctx, int nsteps) {
       HIPCHECK(hipMemcpyAsync(h, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
       HIPCHECK(hipStreamSynchronize(ctx->stream));
       ctx->last_res = (h[1] > 0) ? std::sqrt(h[0] / h[1]) / ctx->P.dt : 0.0;
+    }
+  })
+}
+
+int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels) {
+  PECS_TRY(ctx, {
+    const FactorDev* F = which == 0 ? &ctx->Fs[0] : which == 1 ? &ctx->Fe[0] : which == 2 ? &ctx->FH : nullptr;
+    if (!F) throw std::runtime_error("which must be 0 (semiconductor), 1 (electrolyte) or 2 (Poisson multipliers)");
+    if (nlevels) *nlevels = F->nlevels;
+    for (int l = 0; l < F->nlevels && 6 * l + 5 < n; ++l) {
+      const int v[6] = {F->fw[l], F->fm[l], F->frows[l], F->bw[l], F->bm[l], F->brows[l]};
+      for (int j = 0; j < 6; ++j) out[6 * l + j] = v[j];
     }
   })
 }

@@ -450,12 +450,15 @@ PackedFactor pack_factor(const SupernodalFactor& F) {
     while (w < 64 && w * 8 < mean) w *= 2;
     return w;
   };
-  P.foff.assign(F.nlevels + 1, 0);
-  P.boff.assign(F.nlevels + 1, 0);
   P.goff.assign(F.nlevels + 1, 0);
   P.xoff.assign(F.nlevels + 1, 0);
   P.fw.assign(F.nlevels, 64);
   P.bw.assign(F.nlevels, 64);
+  P.fm.assign(F.nlevels, 1);
+  P.bm.assign(F.nlevels, 1);
+  P.frows.assign(F.nlevels, 4);
+  P.brows.assign(F.nlevels, 4);
+  P.lnoff.assign(F.nlevels + 1, 0);
   for (int l = 0; l < F.nlevels; ++l) {
     double fl = 0, fr = 0, bl = 0, br = 0;
     for (int s : F.level_nodes[l]) {
@@ -467,37 +470,62 @@ PackedFactor pack_factor(const SupernodalFactor& F) {
     }
     P.fw[l] = lanes(fr > 0 ? fl / fr : 1);
     P.bw[l] = lanes(br > 0 ? bl / br : 1);
-    P.foff[l] = (int)(P.ftiles.size() / 4);
-    P.boff[l] = (int)(P.btiles.size() / 4);
-    P.goff[l] = (int)P.lrows.size();
-    P.xoff[l] = (int)P.xidx.size();
     // tile height: at most sweep_tile_rows(W), smaller on levels with few rows
-    // so that each level still spreads over >= ~4096 workgroups
+    // so that each level still spreads over many workgroups
     auto height = [](int w, double nrows) {
       int h = sweep_tile_rows(w);
       const int pass = 256 / w;
       while (h > pass && nrows / h < 8192) h /= 2;
       return h;
     };
-    const int rf = height(P.fw[l], fr), rb = height(P.bw[l], br);
+    P.frows[l] = height(P.fw[l], fr);
+    P.brows[l] = height(P.bw[l], br);
+    P.fm[l] = std::max(1, P.frows[l] / (256 / P.fw[l]));
+    P.bm[l] = std::max(1, P.brows[l] / (256 / P.bw[l]));
+    P.goff[l] = (int)P.lrows.size();
+    P.xoff[l] = (int)P.xidx.size();
+    P.lnoff[l] = (int)P.lnodes.size();
     for (int s : F.level_nodes[l]) {
+      P.lnodes.push_back(s);
       for (int r = 0; r < P.ns[s]; ++r) P.lrows.push_back(P.p0[s] + r);
       for (int i = 0; i < P.ms[s]; ++i) P.xidx.push_back(P.rows_off[s] + i);
-      const int nf = P.ns[s] + P.ms[s];
-      for (int r0 = 0; r0 < nf; r0 += rf) P.ftiles.insert(P.ftiles.end(), {s, r0, std::min(rf, nf - r0), 0});
-      for (int r0 = 0; r0 < P.ns[s]; r0 += rb)
-        P.btiles.insert(P.btiles.end(), {s, r0, std::min(rb, P.ns[s] - r0), 0});
     }
   }
-  P.foff[F.nlevels] = (int)(P.ftiles.size() / 4);
-  P.boff[F.nlevels] = (int)(P.btiles.size() / 4);
   P.goff[F.nlevels] = (int)P.lrows.size();
   P.xoff[F.nlevels] = (int)P.xidx.size();
+  P.lnoff[F.nlevels] = (int)P.lnodes.size();
+  rebuild_tiles(P);
   P.rows = F.rows;
   P.cptr = F.contrib_ptr;
   P.czidx.resize(F.contrib_node.size());
   for (size_t t = 0; t < P.czidx.size(); ++t) P.czidx[t] = P.z_off[F.contrib_node[t]] + F.contrib_row[t];
   return P;
+}
+
+void level_tiles(const PackedFactor& P, int l, bool fwd, int rows, std::vector<int>& out) {
+  if (rows < 1) throw std::runtime_error("level_tiles: rows must be >= 1");
+  for (int q = P.lnoff[l]; q < P.lnoff[l + 1]; ++q) {
+    const int s = P.lnodes[q];
+    const int nr = fwd ? P.ns[s] + P.ms[s] : P.ns[s];
+    for (int r0 = 0; r0 < nr; r0 += rows) out.insert(out.end(), {s, r0, std::min(rows, nr - r0), 0});
+  }
+}
+
+void rebuild_tiles(PackedFactor& P) {
+  P.ftiles.clear();
+  P.btiles.clear();
+  P.foff.assign(P.nlevels + 1, 0);
+  P.boff.assign(P.nlevels + 1, 0);
+  for (int l = 0; l < P.nlevels; ++l) {
+    if (P.frows[l] > (256 / P.fw[l]) * P.fm[l] || P.brows[l] > (256 / P.bw[l]) * P.bm[l])
+      throw std::runtime_error("rebuild_tiles: tile height exceeds the rows one workgroup covers");
+    P.foff[l] = (int)(P.ftiles.size() / 4);
+    P.boff[l] = (int)(P.btiles.size() / 4);
+    level_tiles(P, l, true, P.frows[l], P.ftiles);
+    level_tiles(P, l, false, P.brows[l], P.btiles);
+  }
+  P.foff[P.nlevels] = (int)(P.ftiles.size() / 4);
+  P.boff[P.nlevels] = (int)(P.btiles.size() / 4);
 }
 
 }  // namespace pecs

@@ -50,8 +50,9 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells);
 //        at offset j(ns+ms) - j(j-1)/2.
 // Forward at a level: t_p = b_p - sum z[czidx[cptr[p]:cptr[p+1]]], then
 // [y_own; z_s] = [Linv; W21] t_own.  Backward: x_own = bwd rows . [y; x_rows].
-// The sweeps run over row tiles (node, first row, rows) of sweep_tile_rows(W)
-// rows, W lanes per row chosen per level from the mean row length.
+// The sweeps run over row tiles (node, first row, rows): W lanes per row,
+// 256/W rows per pass, at most MAXR passes per tile, chosen per level and
+// direction (a default rule here; the device library times the candidates).
 inline int sweep_tile_rows(int w) { return w <= 16 ? 64 : (w == 32 ? 32 : 16); }
 
 struct PackedFactor {
@@ -66,8 +67,17 @@ struct PackedFactor {
   std::vector<int64_t> xidx;          // panel-row positions of each level's nodes
   std::vector<int> xoff;              // per level offsets into xidx
   std::vector<int> fw, bw;            // lanes per row of the sweeps per level
+  std::vector<int> fm, bm;            // row passes per tile per level
+  std::vector<int> frows, brows;      // tile height per level (<= (256 / W) * MAXR)
+  std::vector<int> lnodes, lnoff;     // nodes of each level + offsets
   double tri_bytes = 0.0;             // bytes read by one forward + backward sweep
 };
 PackedFactor pack_factor(const SupernodalFactor& F);
+
+// Row tiles of level l (forward: all ns + ms rows of each node; backward: the
+// ns own rows) with `rows` rows per tile, appended as (node, row0, nrows, 0).
+void level_tiles(const PackedFactor& P, int l, bool fwd, int rows, std::vector<int>& out);
+// Rebuild ftiles/btiles and their level offsets from fw/fm/frows, bw/bm/brows.
+void rebuild_tiles(PackedFactor& P);
 
 }  // namespace pecs

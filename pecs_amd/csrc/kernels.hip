@@ -412,11 +412,13 @@ __device__ __forceinline__ double group_sum(double x) {
   return x;
 }
 
-template <int W>
+// W lanes per row, 256/W rows per pass, MAXR passes: a tile holds at most
+// (256/W) * MAXR rows of one supernode.
+template <int W, int MAXR_>
 struct SweepShape {
-  static constexpr int R = 256 / W;              // rows per pass
-  static constexpr int RB = (W <= 16) ? 64 : (W == 32 ? 32 : 16);  // rows per tile (sweep_tile_rows)
-  static constexpr int MAXR = RB / R;              // passes per tile
+  static constexpr int R = 256 / W;
+  static constexpr int MAXR = MAXR_;
+  static constexpr int RB = R * MAXR;
 };
 
 // forward, part 1: t_p = b_p - sum of the updates z owed to row p, into X
@@ -448,11 +450,11 @@ __global__ __launch_bounds__(256) void k_bwd_gather(DevFactor F, const int64_t* 
   if (NRHS > 1) F.Z[F.zlen + q] = F.X[F.n + p];
 }
 
-template <int NRHS, int W, bool FWD>
+template <int NRHS, int W, int MAXR_, bool FWD>
 __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restrict__ tiles,
                                                const double* __restrict__ b0, const double* __restrict__ b1,
                                                double* __restrict__ Y0, double* __restrict__ Y1) {
-  using SS = SweepShape<W>;
+  using SS = SweepShape<W, MAXR_>;
   constexpr int R = SS::R, MAXR = SS::MAXR;
   __shared__ double vec[NRHS][CW];
   const int4 tl = tiles[blockIdx.x];
@@ -460,13 +462,50 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
   const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
   const int VL = FWD ? ns : ns + ms;
   const int g = threadIdx.x / W, sl = threadIdx.x % W;
+  const double* __restrict__ base = FWD ? F.fwd + F.fwd_off[s] : F.bwd + F.bwd_off[s];
+  // The group's MAXR rows are walked together: every column step issues one
+  // load per row, so each lane keeps 2*MAXR independent loads in flight and
+  // reads each staged vector entry from LDS once for all its rows.  Row q
+  // holds vector positions [rs[q], re[q]), its element at position p at
+  // base[off[q] + p].
+  int64_t off[MAXR];
+  int rs[MAXR], re[MAXR];
+  int lo = VL, hi = 0;
+#pragma unroll
+  for (int q = 0; q < MAXR; ++q) {
+    const int lr = row0 + g + q * R;
+    if (lr < row0 + nr) {
+      if (FWD) {
+        rs[q] = 0;
+        if (lr < ns) {
+          off[q] = (int64_t)lr * (lr + 1) / 2;
+          re[q] = lr + 1;
+        } else {
+          off[q] = (int64_t)ns * (ns + 1) / 2 + (int64_t)(lr - ns) * ns;
+          re[q] = ns;
+        }
+      } else {
+        off[q] = (int64_t)lr * (ns + ms) - (int64_t)lr * (lr - 1) / 2 - lr;
+        rs[q] = lr;
+        re[q] = ns + ms;
+      }
+      lo = min(lo, rs[q]);
+      hi = max(hi, re[q]);
+    } else {
+      off[q] = 0;
+      rs[q] = 0;
+      re[q] = 0;
+    }
+  }
   double acc[MAXR][NRHS];
 #pragma unroll
   for (int q = 0; q < MAXR; ++q)
 #pragma unroll
     for (int h = 0; h < NRHS; ++h) acc[q][h] = 0.0;
-  const int64_t foff = FWD ? F.fwd_off[s] : F.bwd_off[s];
-  for (int c0 = 0; c0 < VL; c0 += CW) {
+  // backward rows start at their own index, so no row of the tile uses
+  // positions below row0
+  const int tlo = FWD ? 0 : row0;
+  for (int c0 = tlo; c0 < VL; c0 += CW) {
     const int cl = min(CW, VL - c0);
     __syncthreads();
     for (int i = threadIdx.x; i < cl; i += 256) {
@@ -484,47 +523,26 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
       }
     }
     __syncthreads();
+    const int e = min(hi, c0 + cl);
+    for (int pa = max(lo, c0) + sl; pa < e; pa += 2 * W) {
+      const int pb = pa + W;
+      const bool okb = pb < e;
+      double La[MAXR], Lb[MAXR];
 #pragma unroll
-    for (int q = 0; q < MAXR; ++q) {
-      const int lr = row0 + g + q * R;
-      if (lr >= row0 + nr) break;
-      const double* ptr;
-      int start, end;
-      if (FWD) {
-        start = 0;
-        if (lr < ns) {
-          ptr = F.fwd + foff + (int64_t)lr * (lr + 1) / 2;
-          end = lr + 1;
-        } else {
-          ptr = F.fwd + foff + (int64_t)ns * (ns + 1) / 2 + (int64_t)(lr - ns) * ns;
-          end = ns;
-        }
-      } else {
-        // element at vector position pos (>= lr) is stored at index pos - lr
-        ptr = F.bwd + foff + (int64_t)lr * (ns + ms) - (int64_t)lr * (lr - 1) / 2 - lr;
-        start = lr;
-        end = ns + ms;
+      for (int q = 0; q < MAXR; ++q) {
+        La[q] = (pa >= rs[q] && pa < re[q]) ? __builtin_nontemporal_load(base + off[q] + pa) : 0.0;
+        Lb[q] = (okb && pb >= rs[q] && pb < re[q]) ? __builtin_nontemporal_load(base + off[q] + pb) : 0.0;
       }
-      const int a = max(start, c0), e = min(end, c0 + cl);
-      double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
-      int pos = a + sl;
-      for (; pos + W < e; pos += 2 * W) {
-        const double L0 = __builtin_nontemporal_load(ptr + pos);
-        const double L1 = __builtin_nontemporal_load(ptr + pos + W);
-        s0 += L0 * vec[0][pos - c0];
-        u0 += L1 * vec[0][pos + W - c0];
-        if (NRHS > 1) {
-          s1 += L0 * vec[NRHS - 1][pos - c0];
-          u1 += L1 * vec[NRHS - 1][pos + W - c0];
-        }
+      const double va0 = vec[0][pa - c0];
+      const double vb0 = okb ? vec[0][pb - c0] : 0.0;
+#pragma unroll
+      for (int q = 0; q < MAXR; ++q) acc[q][0] += La[q] * va0 + Lb[q] * vb0;
+      if (NRHS > 1) {
+        const double va1 = vec[NRHS - 1][pa - c0];
+        const double vb1 = okb ? vec[NRHS - 1][pb - c0] : 0.0;
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) acc[q][NRHS - 1] += La[q] * va1 + Lb[q] * vb1;
       }
-      if (pos < e) {
-        const double L0 = __builtin_nontemporal_load(ptr + pos);
-        s0 += L0 * vec[0][pos - c0];
-        if (NRHS > 1) s1 += L0 * vec[NRHS - 1][pos - c0];
-      }
-      acc[q][0] += s0 + u0;
-      if (NRHS > 1) acc[q][NRHS - 1] += s1 + u1;
     }
   }
 #pragma unroll
@@ -813,13 +831,36 @@ int launch_schur_recover(int k, const DevRegion& R, const double* W1, const doub
   return (int)hipGetLastError();
 }
 
-#define PECS_DISPATCH_W(w, CALL)                     \
-  switch (w) {                                       \
-    case 8: { constexpr int WW = 8; CALL; } break;   \
-    case 16: { constexpr int WW = 16; CALL; } break; \
-    case 32: { constexpr int WW = 32; CALL; } break; \
-    default: { constexpr int WW = 64; CALL; } break; \
+template <int NRHS, bool FWD>
+static int sweep_dispatch(const DevFactor& F, const int4* tl, int nt, int w, int maxr, const double* v0,
+                          const double* v1, double* t0, double* t1, hipStream_t s) {
+#define PECS_SWEEP_CASE(WW, MM)                                                          \
+  if (w == WW && maxr == MM) {                                                           \
+    k_sweep<NRHS, WW, MM, FWD><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1);                \
+    return 0;                                                                            \
   }
+  PECS_SWEEP_CASE(8, 1) PECS_SWEEP_CASE(8, 2) PECS_SWEEP_CASE(8, 4)
+  PECS_SWEEP_CASE(16, 1) PECS_SWEEP_CASE(16, 2) PECS_SWEEP_CASE(16, 4)
+  PECS_SWEEP_CASE(32, 1) PECS_SWEEP_CASE(32, 2) PECS_SWEEP_CASE(32, 4)
+  PECS_SWEEP_CASE(64, 1) PECS_SWEEP_CASE(64, 2) PECS_SWEEP_CASE(64, 4)
+#undef PECS_SWEEP_CASE
+  return (int)hipErrorInvalidValue;
+}
+
+int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
+                 const double* v0, const double* v1, double* t0, double* t1, hipStream_t s) {
+  if (nt <= 0) return 0;
+  if (nrhs != 2) {
+    v1 = v0;
+    t1 = t0;
+  }
+  if (fwd) {
+    return nrhs == 2 ? sweep_dispatch<2, true>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s)
+                     : sweep_dispatch<1, true>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s);
+  }
+  return nrhs == 2 ? sweep_dispatch<2, false>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s)
+                   : sweep_dispatch<1, false>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s);
+}
 
 int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, double* v1, double* t0, double* t1,
                       int nrhs, hipStream_t s) {
@@ -834,13 +875,8 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, dou
       else k_fwd_gather<1><<<nblk(ng, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v0, v1);
     }
     const int nt = L.foff[l + 1] - L.foff[l];
-    if (nt == 0) continue;
-    const int4* tl = F.ftiles + L.foff[l];
-    if (nrhs == 2) {
-      PECS_DISPATCH_W(L.fw[l], (k_sweep<2, WW, true><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
-    } else {
-      PECS_DISPATCH_W(L.fw[l], (k_sweep<1, WW, true><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
-    }
+    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, v0, v1, t0, t1, s);
+    if (e) return e;
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
     const int nx = L.xoff[l + 1] - L.xoff[l];
@@ -849,13 +885,8 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, dou
       else k_bwd_gather<1><<<nblk(nx, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     }
     const int nt = L.boff[l + 1] - L.boff[l];
-    if (nt == 0) continue;
-    const int4* tl = F.btiles + L.boff[l];
-    if (nrhs == 2) {
-      PECS_DISPATCH_W(L.bw[l], (k_sweep<2, WW, false><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
-    } else {
-      PECS_DISPATCH_W(L.bw[l], (k_sweep<1, WW, false><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1)));
-    }
+    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, v0, v1, t0, t1, s);
+    if (e) return e;
   }
   hipError_t e = hipMemcpyAsync(v0, F.X, (size_t)F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess && nrhs == 2)

@@ -87,6 +87,8 @@ struct FactorLevels {
   const int* xoff;  // host: per level offsets into xidx
   const int* fw;    // host: lanes per row of the forward sweep per level
   const int* bw;    // host: lanes per row of the backward sweep per level
+  const int* fm;    // host: row passes per tile (tile rows <= (256 / fw) * fm) per level
+  const int* bm;    // host: the same for the backward sweep
   int nlevels;
 };
 
@@ -100,6 +102,11 @@ int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* 
 int launch_schur_recover(int k, const DevRegion& R, const double* W1, const double* W2, const double* xs1,
                          const double* xs2, double mu1, double mu2, double* out1, double* out2, const double* b1,
                          const double* b2, int nrhs, hipStream_t s);
+// One level's sweep over `nt` row tiles with W lanes per row and `maxr` row
+// passes per tile (W in {8,16,32,64}, maxr in {1,2,4}; every tile has at most
+// (256 / W) * maxr rows).
+int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
+                 const double* v0, const double* v1, double* t0, double* t1, hipStream_t s);
 int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, double* v1, double* tmp0,
                       double* tmp1, int nrhs, hipStream_t s);
 int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
