@@ -54,6 +54,22 @@ def cpu_baseline(seconds_budget=20.0):
                       f"{n} steps, U={U} DOFs/step, setup excluded"}
 
 
+def profiled_traffic(workload):
+    """HBM bytes per launch of the roofline kernel from the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy
+    calibration), when it was measured on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "carrier_solves_r8k2_hbm_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    vals = dict(line.split(": ", 1) for line in text.splitlines()
+                if line.startswith(("traffic_bytes_per_launch: ", "workload: ")))
+    if vals.get("workload", "").strip() != workload:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
+
+
 def cpu_time_to_steady(tol, max_steps=20000):
     """Oracle (one core) on the small workload: refinement 4, k = 1, dt = 1e-3."""
     import numpy as np
@@ -169,6 +185,8 @@ def main():
         chol_bytes += fb.value + 2 * 4 * 8.0 * n_schur
     chol_ms = phase[3]
     achieved = chol_bytes / (chol_ms * 1e-3) / 1e9 if chol_ms > 0 else 0.0
+    workload = (f"PECS half cell, refinement {args.refine} ({1 << args.refine}x{1 << args.refine} quads/region), "
+                f"LDG k={args.degree}, RT{args.degree - 1}xDGQ{args.degree - 1} Poisson, dt={args.dt}")
     U = 2 * g.n_semi + 2 * g.n_elec + g.n_poisson
     ms_per_step = elapsed / args.steps * 1e3
     value = U * args.steps / elapsed
@@ -185,9 +203,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: equilibrium initial state, configuration physics",
-        "config": {"workload": f"PECS half cell, refinement {args.refine} "
-                               f"({1 << args.refine}x{1 << args.refine} quads/region), LDG k={args.degree}, "
-                               f"RT{args.degree - 1}xDGQ{args.degree - 1} Poisson, dt={args.dt}",
+        "config": {"workload": workload,
                    "dofs_per_step": U, "n_carrier_semi": g.n_semi, "n_carrier_elec": g.n_elec,
                    "n_poisson": g.n_poisson, "setup_s": round(setup_s, 2),
                    "steady_residual_last_step": res,
@@ -196,9 +212,11 @@ def main():
         "roofline": {"kernel": "supernodal triangular sweeps (fwd+bwd) with the semiconductor and electrolyte "
                                "Schur factors, 2 rhs each, one step",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(workload),
+                     "traffic_source": "profiles/carrier_solves_r8k2_hbm_traffic.md (rocprofv3 --pmc)",
                      "algorithmic_bytes_per_launch": chol_bytes, "avg_ms": chol_ms},
     }
+    out["config"]["sweep_tiles_semi"] = [list(t) for t in g.sweep_config(0)]
     if not args.no_steady:
         tts = gpu_time_to_steady(g, args.steady_tol, args.steady_check_every, args.steady_max_steps)
         tts["workload"] = out["config"]["workload"]

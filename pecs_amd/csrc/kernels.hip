@@ -7,9 +7,10 @@
 //                   the other region evaluated at QGauss(k+2) points.
 //   k_schur_prep    W = A^{-1} f_p (cell-local inverse mass, times mobility).
 //   k_schur_rhs     g = f_v + B^T W (cell + 4 face neighbours).
-//   k_fwd_* / k_bwd_*  supernodal triangular solves with S = L L^T, one
-//                   wavefront per factor row, level by level on the
-//                   elimination tree (deterministic, no atomics).
+//   k_fwd_gather / k_sweep / k_bwd_gather  supernodal triangular solves
+//                   with S = L L^T, level by level on the elimination tree:
+//                   row tiles of dense [Linv; L21 Linv] panels, vector chunks
+//                   staged in LDS, W lanes per row (deterministic, no atomics).
 //   k_schur_recover q = W - A^{-1} B u; writes [Jx | Jy | u].
 //   k_poisson_*     hybridised mixed Poisson: cell condensation, face
 //                   assembly, multiplier solve, cell recovery.

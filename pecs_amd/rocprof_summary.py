@@ -6,16 +6,18 @@
 A triangular solve is a sequence of launches (per supernode level: a gather and
 a sweep, forward then backward, then one copy).  Its dispatches are found in
 the trace as the window between the marker kernels that bracket it in every
-step: k_schur_rhs ... k_schur_recover for the two carrier regions (the first
-window of a step is the semiconductor, the second the electrolyte) and
-k_poisson_face ... k_poisson_recover for the Poisson multipliers.
+step: k_schur_rhs ... k_schur_recover for the carrier solves of both regions
+(which run concurrently on two streams) and k_poisson_face ...
+k_poisson_recover for the Poisson multipliers.
 
-`trace` prints per-kernel statistics and the per-window sum of kernel
-durations (what a hipEvent pair around the solve measures, less launch gaps).
+`trace` prints per-kernel statistics and, per window, the sum of kernel
+durations and the span from the first dispatch's start to the last one's end
+(what a hipEvent pair around the concurrent solves measures).
 `pmc` sums FETCH_SIZE / WRITE_SIZE (KiB per dispatch) over each window; with
 --copy-bytes it also reports the counter reading of the largest plain copy
-dispatch (__amd_rocclr_copyBuffer or an elementwise copy kernel) against its
-known size, the scale factor that converts the counters into HBM bytes.
+dispatch (__amd_rocclr_copyBuffer) against its
+known size, the scale factor that converts the counters into HBM bytes
+(torch's device-to-device copy_ of 1 GiB, run by bench.py --copy-calibration-gib 1).
 """
 import argparse
 import csv
@@ -42,22 +44,15 @@ def _marker(name, table):
 
 def windows(dispatches):
     """dispatches: list of (dispatch_id, name, payload) in dispatch order.
-    Returns {label: [list of payload lists]} with labels semi / elec / poisson."""
+    Returns {label: [list of payload lists]}: 'carrier' (both regions' Cholesky
+    solves of one step, between the last k_schur_rhs and the first
+    k_schur_recover) and 'poisson'."""
     out = defaultdict(list)
-    cur, kind, carrier_count = None, None, 0
-    step_carrier = 0
+    cur, kind = None, None
     for _, name, payload in dispatches:
-        s = _short(name)
-        if s.startswith("k_rhs_carrier<") and ", true>" in s:
-            step_carrier = 0
         close = _marker(name, CLOSE)
         if cur is not None and close == kind:
-            if kind == "carrier":
-                label = "semi" if step_carrier == 0 else "elec"
-                step_carrier += 1
-            else:
-                label = "poisson"
-            out[label].append(cur)
+            out[kind].append(cur)
             cur, kind = None, None
             continue
         opened = _marker(name, OPEN)
@@ -74,17 +69,20 @@ def _read(path):
         return list(csv.DictReader(f))
 
 
-def trace(path, top=25):
+def trace(path, top=25, start="k_rhs_carrier"):
     rows = _read(path)
     rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    first = next((i for i, r in enumerate(rows) if _short(r["Kernel_Name"]).startswith(start)), 0)
+    print(f"per-kernel statistics from the first {start} dispatch on (setup dispatches before it, "
+          f"including the tile-shape timing, are left out): {len(rows) - first} of {len(rows)} dispatches")
     stats = OrderedDict()
-    for r in rows:
+    for r in rows[first:]:
         d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
         s = stats.setdefault(_short(r["Kernel_Name"]), [0, 0])
         s[0] += 1
         s[1] += d
     total = sum(v[1] for v in stats.values())
-    print(f"kernel dispatches: {len(rows)}, total kernel time {total / 1e6:.3f} ms")
+    print(f"total kernel time {total / 1e6:.3f} ms")
     print(f"{'kernel':60s} {'calls':>6s} {'total ms':>10s} {'avg us':>9s} {'%':>6s}")
     for k, (n, t) in sorted(stats.items(), key=lambda kv: -kv[1][1])[:top]:
         print(f"{k[:60]:60s} {n:6d} {t / 1e6:10.3f} {t / n / 1e3:9.1f} {100 * t / total:6.2f}")
@@ -114,8 +112,7 @@ def pmc(path, copy_bytes=None):
             print(f"solve window {label}: {c} per solve avg {sum(vals) / len(vals):.6g} KiB "
                   f"= {sum(vals) / len(vals) * 1024:.6g} B over {len(ws)} solves")
     if copy_bytes:
-        copies = [(d, n, cs) for d, n, cs in disp
-                  if "copyBuffer" in n or "copy_kernel" in n or "elementwise_kernel" in n]
+        copies = [(d, n, cs) for d, n, cs in disp if "copyBuffer" in n or "copy_kernel" in n]
         if copies:
             d, n, cs = max(copies, key=lambda x: max(x[2].values()))
             for c in counters:
