@@ -107,6 +107,72 @@ def gpu_time_to_steady(g, tol, check_every, max_steps):
     return {"s": time.perf_counter() - t0, "steps": n, "residual": res, "reached": res <= tol}
 
 
+def batched_leg(args, workload):
+    """I-V sweep on one mesh: B bias points (applied bias evenly over
+    [-0.5, 0.5] scaled volts) stepped together, every factor read once per
+    sweep for all of them (pecs_create_batch).  Reports the aggregate step
+    throughput and the time to steady state per bias point."""
+    import numpy as np
+    import torch
+    import pecs_amd
+    B = args.batch
+    prms = []
+    for v in np.linspace(-0.5, 0.5, B):
+        p = pecs_amd.workload_params(args.refine, args.degree, args.dt)
+        p.update(phi_app=float(v))
+        prms.append(p)
+    t0 = time.perf_counter()
+    g = pecs_amd.SolarCellGPU(prms)
+    setup_s = time.perf_counter() - t0
+    print(f"[bench] batch of {B}: setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+    g.initial_state()
+    g.step(3)
+    L = pecs_amd.lib()
+    import ctypes
+    L.pecs_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.pecs_phase_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int)]
+    assert L.pecs_enable_timing(g.ctx, 1) == 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.batch_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ph = (ctypes.c_double * 8)()
+    nst = ctypes.c_int()
+    L.pecs_phase_times(g.ctx, ph, 8, ctypes.byref(nst))
+    L.pecs_enable_timing(g.ctx, 0)
+    names = ["rhs_semi", "rhs_elec", "schur_transforms", "carrier_cholesky", "recoveries", "poisson_cond",
+             "chol_poisson", "poisson_recover"]
+    phase = {n: round(ph[i] / max(nst.value, 1), 4) for i, n in enumerate(names)}
+    U = 2 * g.n_semi + 2 * g.n_elec + g.n_poisson
+    ms_step = el / args.batch_steps * 1e3
+    out = {"batch": B, "workload": workload + f", {B} applied biases in [-0.5, 0.5]", "setup_s": round(setup_s, 2),
+           "ms_per_step_all_instances": ms_step, "ms_per_step_per_instance": ms_step / B,
+           "dof_updates_per_s": U * B * args.batch_steps / el, "phase_ms_per_step": phase,
+           "sweep_tiles_semi": [list(t) for t in g.sweep_config(0)]}
+    if not args.no_steady:
+        g.initial_state()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n, done = 0, np.full(B, -1)
+        while n < args.steady_max_steps:
+            k = min(args.steady_check_every, args.steady_max_steps - n)
+            g.step(k)
+            n += k
+            r = g.residuals()
+            done[(done < 0) & (r <= args.steady_tol)] = n
+            if np.all(done >= 0):
+                break
+        wall = time.perf_counter() - t0
+        out["time_to_steady_state"] = {"tol": args.steady_tol, "s_all": wall, "s_per_bias_point": wall / B,
+                                       "steps_all": n, "steps_per_instance": done.tolist(),
+                                       "reached": bool(np.all(done >= 0))}
+    g.close()
+    print(f"[bench] batched: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +187,9 @@ def main():
     ap.add_argument("--steady-max-steps", type=int, default=8000)
     ap.add_argument("--steady-check-every", type=int, default=25)
     ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady-state legs")
+    ap.add_argument("--batch", type=int, default=0, help="also run an I-V sweep of this many bias points")
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -142,6 +211,11 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n}), flush=True)
+        return
+    if args.batch_only:
+        wl = (f"PECS half cell, refinement {args.refine} ({1 << args.refine}x{1 << args.refine} quads/region), "
+              f"LDG k={args.degree}, RT{args.degree - 1}xDGQ{args.degree - 1} Poisson, dt={args.dt}")
+        print(json.dumps({"batched": batched_leg(args, wl)}), flush=True)
         return
     p = pecs_amd.workload_params(args.refine, args.degree, args.dt)
     t0 = time.perf_counter()
@@ -235,6 +309,8 @@ def main():
             out["time_to_steady_state"]["small_cpu"] = c
         print(f"[bench] small workload time to steady state: {out['time_to_steady_state']}", file=sys.stderr,
               flush=True)
+    if args.batch > 0:
+        out["batched"] = batched_leg(args, workload)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -82,6 +82,16 @@ int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out);
 /* SolarCellProblem constructor + run_full_system setup (SolarCell.cpp:15-92,
  * 1898-1970): meshes, maps, constant matrices, factorisation, device upload. */
 int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out);
+/* A batch of `batch` (1..16) instances on one mesh that share every constant
+ * matrix and factor: an I-V bias sweep, illumination or doping series.  The
+ * reference runs one SolarCellProblem per bias point (main.cpp:22-23); here the
+ * instances step together and every triangular sweep reads each factor once
+ * for all of them.  params[i] may differ only in right-hand-side data (biases,
+ * generation, recombination, transfer rates, equilibria); the matrix
+ * parameters (degree, dt, theta, penalty, mobilities, permittivities, lambda2,
+ * leaf size, Schottky location) must agree.  No counterpart in the reference. */
+int pecs_create_batch(const pecs_mesh* mesh, const pecs_params* params, int batch, pecs_ctx** out);
+int pecs_batch_size(pecs_ctx* ctx, int* batch);
 int pecs_destroy(pecs_ctx* ctx);
 const char* pecs_last_error(pecs_ctx* ctx);
 /* Error text of the last failed pecs_create / stateless call on this thread. */
@@ -116,18 +126,26 @@ int pecs_rhs_poisson(pecs_ctx* ctx, const double* u_n, const double* u_p, const 
 int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x);
 
 /* State (Carrier::solution, PoissonData::solution). poisson may be NULL in
- * set_state (then it is recomputed from the densities as in SolarCell.cpp:2033). */
+ * set_state (then it is recomputed from the densities as in SolarCell.cpp:2033).
+ * pecs_set_state / pecs_get_state address instance 0 of a batch; the _batch
+ * variants take the instance index.  The per-call functions above
+ * (pecs_rhs_*, pecs_solve) use instance 0's right-hand-side data. */
 int pecs_set_state(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* u_r, const double* u_o,
                    const double* poisson);
 int pecs_get_state(pecs_ctx* ctx, double* u_n, double* u_p, double* u_r, double* u_o, double* poisson);
-/* Equilibrium initial data + first Poisson solve (SolarCell.cpp:1996-2034). */
+int pecs_set_state_batch(pecs_ctx* ctx, int instance, const double* u_n, const double* u_p, const double* u_r,
+                         const double* u_o, const double* poisson);
+int pecs_get_state_batch(pecs_ctx* ctx, int instance, double* u_n, double* u_p, double* u_r, double* u_o,
+                         double* poisson);
+/* Equilibrium initial data + first Poisson solve (SolarCell.cpp:1996-2034), every instance. */
 int pecs_initial_state(pecs_ctx* ctx);
-/* nsteps of the IMEX loop body (SolarCell.cpp:2057-2075), device resident. */
+/* nsteps of the IMEX loop body (SolarCell.cpp:2057-2075) for every instance, device resident. */
 int pecs_step(pecs_ctx* ctx, int nsteps);
 /* Relative steady residual ||x^{n+1}-x^n|| / (||x^n|| dt) over the five state
  * vectors after the last pecs_step call (the reference keeps dead diff_* vectors,
  * SolarCell.cpp:2042-2046). */
-int pecs_last_residual(pecs_ctx* ctx, double* residual);
+int pecs_last_residual(pecs_ctx* ctx, double* residual);       /* max over the instances */
+int pecs_last_residual_batch(pecs_ctx* ctx, double* residual);  /* one per instance */
 
 /* Per-phase timing of pecs_step with hipEvents on the context stream
  * (the TimerOutput sections of SolarCell.cpp:1905-2087): phases 0 rhs semi,
@@ -156,7 +174,8 @@ int pecs_rhs_electrolyte_dev(pecs_ctx* ctx, const double* u_r, const double* u_o
 int pecs_rhs_poisson_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* u_r,
                          const double* u_o, double* rhs);
 int pecs_solve_dev(pecs_ctx* ctx, int which, const double* rhs, double* x);
-/* Device pointers of the resident state (valid until pecs_destroy). */
+/* Device pointers of the resident state (valid until pecs_destroy); a batch
+ * stores instance i at offset i * (n_semi, n_elec or n_poisson). */
 int pecs_state_dev(pecs_ctx* ctx, double** u_n, double** u_p, double** u_r, double** u_o, double** poisson);
 
 #ifdef __cplusplus

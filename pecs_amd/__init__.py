@@ -24,6 +24,8 @@ ABI_FUNCTIONS = [
     "pecs_initial_state", "pecs_step", "pecs_last_residual", "pecs_rhs_semiconductor_dev",
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config",
+    "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
+    "pecs_last_residual_batch",
 ]
 
 
@@ -85,6 +87,12 @@ def lib():
         L.pecs_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
         L.pecs_set_state.argtypes = [ctypes.c_void_p] + [dp] * 5
         L.pecs_get_state.argtypes = [ctypes.c_void_p] + [dp] * 5
+        L.pecs_create_batch.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(CParams), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.pecs_batch_size.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.pecs_set_state_batch.argtypes = [ctypes.c_void_p, ctypes.c_int] + [dp] * 5
+        L.pecs_get_state_batch.argtypes = [ctypes.c_void_p, ctypes.c_int] + [dp] * 5
+        L.pecs_last_residual_batch.argtypes = [ctypes.c_void_p, dp]
         L.pecs_export_matrix.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(CParams), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), dp,
@@ -197,14 +205,24 @@ def scale_parameters(phys):
 
 
 class SolarCellGPU:
-    """One problem instance on the GPU (SolarCellProblem roles, SolarCell.hpp:105-757)."""
+    """One problem instance on the GPU (SolarCellProblem roles, SolarCell.hpp:105-757).
+
+    Given a list of parameter dicts, a batch of instances on one mesh that
+    share the constant matrices and factors (pecs_create_batch): they may
+    differ only in right-hand-side data (biases, generation, doping, rates).
+    get_state / set_state / residuals then take the instance index."""
 
     def __init__(self, prm):
         L = lib()
-        self.prm = dict(prm)
-        m, p = to_c(prm)
+        prms = [dict(q) for q in prm] if isinstance(prm, (list, tuple)) else [dict(prm)]
+        self.prm = prms[0]
+        self.prms = prms
+        self.batch = len(prms)
+        cm = [to_c(q) for q in prms]
+        m = cm[0][0]
+        params = (CParams * len(prms))(*[q[1] for q in cm])
         ctx = ctypes.c_void_p()
-        _check(L.pecs_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(ctx)))
+        _check(L.pecs_create_batch(ctypes.byref(m), params, len(prms), ctypes.byref(ctx)))
         self.ctx = ctx
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         L.pecs_sizes(ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
@@ -275,16 +293,23 @@ class SolarCellGPU:
         self._c(lib().pecs_solve(self.ctx, which, _ptr(b), _ptr(x)))
         return x
 
-    def set_state(self, st):
+    def set_state(self, st, instance=0):
         a = [self._f(st[k]) for k in "npro"]
         P = _ptr(self._f(st["P"])) if st.get("P") is not None else None
-        self._c(lib().pecs_set_state(self.ctx, *[_ptr(x) for x in a], P))
+        self._c(lib().pecs_set_state_batch(self.ctx, int(instance), *[_ptr(x) for x in a], P))
 
-    def get_state(self):
+    def get_state(self, instance=0):
         st = {k: np.zeros(self.n_semi if k in "np" else self.n_elec) for k in "npro"}
         st["P"] = np.zeros(self.n_poisson)
-        self._c(lib().pecs_get_state(self.ctx, *[_ptr(st[k]) for k in "npro"], _ptr(st["P"])))
+        self._c(lib().pecs_get_state_batch(self.ctx, int(instance), *[_ptr(st[k]) for k in "npro"],
+                                           _ptr(st["P"])))
         return st
+
+    def residuals(self):
+        """Relative steady residual of each instance after the last step call."""
+        r = np.zeros(self.batch)
+        self._c(lib().pecs_last_residual_batch(self.ctx, _ptr(r)))
+        return r
 
     def initial_state(self):
         self._c(lib().pecs_initial_state(self.ctx))

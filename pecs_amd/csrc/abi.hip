@@ -2,6 +2,7 @@
 // Declarations and reference citations: include/pecs_abi.h.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -44,7 +45,10 @@ struct pecs_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;            // second stream for the concurrent region solves
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-  Params P;
+  Params P;                  // parameters of instance 0 (matrix parameters are common to all)
+  int B = 1;                 // instances sharing mesh, matrices and factors (a bias / illumination / doping batch)
+  std::vector<Params> Pb;    // per instance
+  std::vector<DevPhys> phs;  // per instance
   int k = 1;
   SolarCellMesh mesh;
   PoissonLayout L;
@@ -52,23 +56,28 @@ struct pecs_ctx {
   DevTabs tabs{};
   DevRegion Rs{}, Re{};
   DevPoisson Pd{};
-  DevPhys ph{};
+  DevPhys ph{};  // instance 0
   FactorDev Fs[2], Fe[2], FH;
   bool share_s = false, share_e = false;
   int64_t Ns = 0, Ne = 0, NP = 0;
+  // device vectors, instance-major (instance i at offset i * per-instance size)
   double* u[4] = {};
   double* pv = nullptr;
   double* b[4] = {};
   double* W[4] = {};
-  double* g[2][2] = {};
-  double* t[2][2] = {};
+  int64_t Wsz[4] = {};
+  // per region: the Schur right-hand sides / solutions of all carriers and
+  // instances, h-major with h = carrier * B + instance (region factor order)
+  double* g[2] = {};
+  double* t[2] = {};
   double* yP = nullptr;
-  double* rH = nullptr;
+  int64_t ysz = 0;
+  double* rH = nullptr;  // B multiplier right-hand sides, h-major
   double* tH = nullptr;
   double* old[5] = {};
-  double* acc = nullptr;
-  double* io[4] = {};  // staging for host-pointer calls
-  double last_res = -1.0;
+  double* acc = nullptr;  // 2 per instance
+  double* io[4] = {};     // staging for host-pointer calls (one instance)
+  std::vector<double> last_res;
   double setup_seconds = 0.0, factor_seconds = 0.0;
   std::vector<void*> allocs;
   // optional per-phase timing of pecs_step (hipEvents recorded between phases)
@@ -169,7 +178,8 @@ void validate(const pecs_mesh* m, const pecs_params* q) {
 }
 
 // Pick, per level and sweep direction, the tile shape (lanes per row W, row
-// passes MAXR, rows per tile) with the shortest measured sweep on this GPU.
+// passes MAXR, rows per tile) with the shortest measured sweep on this GPU
+// (W 8..64 with MAXR 1, 2, 4; W 32 and 64 also with MAXR 8).
 // Each candidate is timed on the real factor level with the number of
 // right-hand sides it will be used with; a 512 MiB fill between timed runs
 // keeps the level's factor out of the caches, as it is in a solve.  Inputs
@@ -177,7 +187,17 @@ void validate(const pecs_mesh* m, const pecs_params* q) {
 // temporary Y.  PECS_NO_TUNE=1 keeps the default shapes of pack_factor.
 void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
   const char* env = std::getenv("PECS_NO_TUNE");
-  if (env && env[0] == '1') return;
+  if (env && env[0] == '1') {
+    if (nrhs > 2) {  // the default shapes are for 1-2 right-hand sides
+      for (int l = 0; l < P.nlevels; ++l) {
+        P.fw[l] = P.bw[l] = 0;
+        P.fm[l] = P.bm[l] = 1;
+        P.frows[l] = P.brows[l] = 64;
+      }
+      rebuild_tiles(P);
+    }
+    return;
+  }
   const size_t scrub_bytes = (size_t)512 << 20;
   void* scrub = nullptr;
   double* Y = nullptr;
@@ -198,14 +218,15 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
   };
   try {
     HIPCHECK(hipMalloc(&scrub, scrub_bytes));
-    HIPCHECK(hipMalloc((void**)&Y, 2 * (size_t)P.n * sizeof(double)));
+    const int nrs = nrhs > 2 ? mfma_rhs_pad(nrhs) : nrhs;  // scratch right-hand sides
+    HIPCHECK(hipMalloc((void**)&Y, (size_t)nrs * P.n * sizeof(double)));
     HIPCHECK(hipMalloc((void**)&tbuf, std::max<size_t>(tcap, 1) * sizeof(int4)));
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     hipStream_t s = c->stream;
-    HIPCHECK(hipMemsetAsync(Y, 0, 2 * (size_t)P.n * sizeof(double), s));
-    HIPCHECK(hipMemsetAsync(d.X, 0, 2 * (size_t)P.n * sizeof(double), s));
-    HIPCHECK(hipMemsetAsync(d.Z, 0, 2 * (size_t)std::max<int64_t>(P.zlen, 1) * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(Y, 0, (size_t)nrs * P.n * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(d.X, 0, (size_t)nrs * P.n * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(d.Z, 0, (size_t)nrs * std::max<int64_t>(P.zlen, 1) * sizeof(double), s));
     std::vector<int> tl;
     std::vector<int4> t4;
     for (int l = 0; l < P.nlevels; ++l) {
@@ -213,37 +234,53 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
         const bool fwd = dir == 0;
         float best = 1e30f;
         int bw = fwd ? P.fw[l] : P.bw[l], bm = fwd ? P.fm[l] : P.bm[l], br = fwd ? P.frows[l] : P.brows[l];
-        for (int w : {8, 16, 32, 64}) {
-          for (int m : {1, 2, 4}) {
-            for (int half = 0; half < 2; ++half) {
-              const int R = 256 / w;
-              const int rows = (R * m) >> half;
-              if (rows < R) continue;
-              tl.clear();
-              level_tiles(P, l, fwd, rows, tl);
-              const int nt = (int)(tl.size() / 4);
-              if (nt == 0) continue;
-              t4.resize(nt);
-              for (int t = 0; t < nt; ++t) t4[t] = make_int4(tl[4 * t], tl[4 * t + 1], tl[4 * t + 2], 0);
-              HIPCHECK(hipMemcpyAsync(tbuf, t4.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
-              float tmin = 1e30f;
-              for (int rep = 0; rep < 3; ++rep) {
-                HIPCHECK(hipMemsetAsync(scrub, rep, scrub_bytes, s));
-                HIPCHECK(hipEventRecord(e0, s));
-                HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, Y + P.n, Y, Y + P.n, s));
-                HIPCHECK(hipEventRecord(e1, s));
-                HIPCHECK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-                tmin = std::min(tmin, ms);
-              }
-              if (tmin < best) {
-                best = tmin;
-                bw = w;
-                bm = m;
-                br = rows;
+        if (nrhs > 2) {
+          bw = 0;
+          bm = 1;
+          br = 64;
+        }
+        // candidates: (w, m, rows); w = 0 is the MFMA sweep (m 16-row blocks
+        // per wave, rows <= 64 m), used for more than 2 right-hand sides
+        std::vector<std::array<int, 3>> cand;
+        if (nrhs > 2) {
+          for (int m : {1, 2, 4})
+            for (int rows : {64 * m, 32 * m}) cand.push_back({0, m, rows});
+        } else {
+          for (int w : {8, 16, 32, 64})
+            for (int m : {1, 2, 4, 8}) {
+              if (m == 8 && w < 32) continue;
+              for (int half = 0; half < 2; ++half) {
+                const int R = 256 / w;
+                const int rows = (R * m) >> half;
+                if (rows >= R) cand.push_back({w, m, rows});
               }
             }
+        }
+        for (const auto& cd : cand) {
+          const int w = cd[0], m = cd[1], rows = cd[2];
+          tl.clear();
+          level_tiles(P, l, fwd, rows, tl);
+          const int nt = (int)(tl.size() / 4);
+          if (nt == 0) continue;
+          t4.resize(nt);
+          for (int t = 0; t < nt; ++t) t4[t] = make_int4(tl[4 * t], tl[4 * t + 1], tl[4 * t + 2], 0);
+          HIPCHECK(hipMemcpyAsync(tbuf, t4.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
+          float tmin = 1e30f;
+          for (int rep = 0; rep < 3; ++rep) {
+            HIPCHECK(hipMemsetAsync(scrub, rep, scrub_bytes, s));
+            HIPCHECK(hipEventRecord(e0, s));
+            HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, s));
+            HIPCHECK(hipEventRecord(e1, s));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            tmin = std::min(tmin, ms);
+          }
+          if (tmin < best) {
+            best = tmin;
+            bw = w;
+            bm = m;
+            br = rows;
           }
         }
         if (fwd) {
@@ -292,15 +329,37 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   d.bwd_off = upload(c, P.bwd_off);
   d.rows_off = upload(c, P.rows_off);
   d.z_off = upload(c, P.z_off);
-  d.fwd = upload(c, P.fwd);
-  d.bwd = upload(c, P.bwd);
+  if (nrhs > 2) {  // MFMA sweeps: blocked copies only
+    pack_blocked(P);
+    d.fwd = nullptr;
+    d.bwd = nullptr;
+    d.fblk = upload(c, P.fblk);
+    d.bblk = upload(c, P.bblk);
+    d.fblk_off = upload(c, P.fblk_off);
+    d.bblk_off = upload(c, P.bblk_off);
+    d.fblk0 = upload(c, P.fblk0);
+    d.bblk0 = upload(c, P.bblk0);
+    std::vector<double>().swap(P.fblk);
+    std::vector<double>().swap(P.bblk);
+  } else {
+    d.fwd = upload(c, P.fwd);
+    d.bwd = upload(c, P.bwd);
+    d.fblk = d.bblk = nullptr;
+    d.fblk_off = d.bblk_off = nullptr;
+    d.fblk0 = d.bblk0 = nullptr;
+  }
   d.rows = upload(c, P.rows);
   d.cptr = upload(c, P.cptr);
   d.czidx = upload(c, P.czidx);
   d.lrows = upload(c, P.lrows);
   d.xidx = upload(c, P.xidx);
-  d.Z = dalloc<double>(c, 2 * (size_t)std::max<int64_t>(P.zlen, 1));
-  d.X = dalloc<double>(c, 2 * (size_t)P.n);
+  // scratch for up to nrhs right-hand sides (at least 2: single-carrier
+  // calls; the MFMA sweep pads to 16 or 32 interleaved right-hand sides)
+  const int nr = nrhs > 2 ? mfma_rhs_pad(nrhs) : 2;
+  d.Z = dalloc<double>(c, (size_t)nr * std::max<int64_t>(P.zlen, 1));
+  d.X = dalloc<double>(c, (size_t)nr * P.n);
+  HIPCHECK(hipMemset(d.Z, 0, (size_t)nr * std::max<int64_t>(P.zlen, 1) * sizeof(double)));
+  HIPCHECK(hipMemset(d.X, 0, (size_t)nr * P.n * sizeof(double)));
   tune_factor(c, P, d, nrhs);
   std::vector<int4> ft(P.ftiles.size() / 4), bt(P.btiles.size() / 4);
   for (size_t t = 0; t < ft.size(); ++t)
@@ -341,7 +400,7 @@ DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, c
   return d;
 }
 
-// single-carrier solve with its own factor
+// single-carrier solve with its own factor (instance 0's scratch)
 void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
   const int reg = which / 2, car = which % 2;
   const DevRegion& R = reg == 0 ? c->Rs : c->Re;
@@ -352,25 +411,36 @@ void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
   hipStream_t s = c->stream;
   double* W = c->W[which];
   HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b, mu, W, s));
-  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b, b, W, W, c->g[reg][0], c->g[reg][0], 1, s));
-  HIPCHECK((hipError_t)launch_chol_solve(Fc.d, Fc.lv(), c->g[reg][0], nullptr, c->t[reg][0],
-                                         nullptr, 1, s));
-  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W, W, c->g[reg][0], c->g[reg][0], mu, mu, x, x, b, b, 1, s));
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b, b, W, W, c->g[reg], c->g[reg], 1, s));
+  HIPCHECK((hipError_t)launch_chol_solve(Fc.d, Fc.lv(), c->g[reg], c->t[reg], 1, s));
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W, W, c->g[reg], c->g[reg], mu, mu, x, x, b, b, 1, s));
 }
 
-void poisson_solve_from_y(pecs_ctx* c, double* out) {
+// Poisson solves of instances [i0, i0 + n) from their condensed cell data yP:
+// face assembly per instance, one multiplier solve with n right-hand sides,
+// recovery per instance into out + i * NP
+void poisson_solve_from_y(pecs_ctx* c, int i0, int n, double* out) {
   hipStream_t s = c->stream;
-  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, nullptr, c->tH, nullptr, 1,
-                                         s));
-  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, out, s));
+  const int64_t nH = c->FH.n;
+  for (int i = i0; i < i0 + n; ++i)
+    HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP + i * c->ysz, c->rH + (i - i0) * nH, s));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, n, s));
+  for (int i = i0; i < i0 + n; ++i)
+    HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH + (i - i0) * nH, c->yP + i * c->ysz,
+                                                out + (i - i0) * c->NP, s));
 }
 
-void poisson_from_densities(pecs_ctx* c, const double* un, const double* up, const double* ur, const double* uo,
-                            double* out) {
-  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->ph, un, up, ur, uo, c->mesh.semi.ncell,
-                                           c->mesh.elec.ncell, c->yP, c->stream));
-  poisson_solve_from_y(c, out);
+// Poisson condensation of instance i from its densities
+void poisson_cells(pecs_ctx* c, int i, const double* un, const double* up, const double* ur, const double* uo) {
+  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->phs[i], un, up, ur, uo, c->mesh.semi.ncell,
+                                           c->mesh.elec.ncell, c->yP + i * c->ysz, c->stream));
+}
+
+// the Poisson states of all instances from their current densities
+void poisson_all_from_densities(pecs_ctx* c) {
+  for (int i = 0; i < c->B; ++i)
+    poisson_cells(c, i, c->u[0] + i * c->Ns, c->u[1] + i * c->Ns, c->u[2] + i * c->Ne, c->u[3] + i * c->Ne);
+  poisson_solve_from_y(c, 0, c->B, c->pv);
 }
 
 // Phases of one step (timing mode records an event on the context stream
@@ -378,7 +448,7 @@ void poisson_from_densities(pecs_ctx* c, const double* un, const double* up, con
 // both regions, 3 the carrier Cholesky solves of both regions (run
 // concurrently on the context stream and a second stream, joined before the
 // mark), 4 recoveries of both regions, 5 Poisson condensation, 6 Poisson
-// multiplier Cholesky, 7 Poisson recovery.
+// multiplier Cholesky, 7 Poisson recovery.  Every phase covers all instances.
 constexpr int NPHASE = 8;
 
 struct PhaseMarks {
@@ -390,31 +460,49 @@ struct PhaseMarks {
   }
 };
 
-void region_schur(pecs_ctx* c, int reg, const double* b0, const double* b1, hipStream_t s) {
+// right-hand side slot of carrier car (0, 1) of instance i in a region's g / t
+inline int64_t gslot(const pecs_ctx* c, int reg, int car, int i) {
+  const int64_t n = reg == 0 ? c->Fs[0].n : c->Fe[0].n;
+  return ((int64_t)car * c->B + i) * n;
+}
+
+void region_schur(pecs_ctx* c, int reg, hipStream_t s) {
   const DevRegion& R = reg == 0 ? c->Rs : c->Re;
-  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, c->P.mu[2 * reg], c->W[2 * reg], s));
-  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, c->P.mu[2 * reg + 1], c->W[2 * reg + 1], s));
-  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, c->W[2 * reg], c->W[2 * reg + 1], c->g[reg][0],
-                                        c->g[reg][1], 2, s));
+  const int64_t N = reg == 0 ? c->Ns : c->Ne;
+  const int c0 = 2 * reg, c1 = 2 * reg + 1;
+  for (int i = 0; i < c->B; ++i) {
+    const double* b0 = c->b[c0] + i * N;
+    const double* b1 = c->b[c1] + i * N;
+    double* W0 = c->W[c0] + i * c->Wsz[c0];
+    double* W1 = c->W[c1] + i * c->Wsz[c1];
+    HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, c->P.mu[c0], W0, s));
+    HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, c->P.mu[c1], W1, s));
+    HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg] + gslot(c, reg, 0, i),
+                                          c->g[reg] + gslot(c, reg, 1, i), 2, s));
+  }
 }
 
 void region_chol(pecs_ctx* c, int reg, hipStream_t s) {
   FactorDev* F = reg == 0 ? c->Fs : c->Fe;
   const bool share = reg == 0 ? c->share_s : c->share_e;
-  if (share) {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], c->g[reg][1], c->t[reg][0],
-                                           c->t[reg][1], 2, s));
-  } else {
-    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg][0], nullptr, c->t[reg][0], nullptr, 1, s));
-    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].lv(), c->g[reg][1], nullptr, c->t[reg][1], nullptr, 1, s));
+  if (share) {  // both carriers of all instances against one factor read
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg], c->t[reg], 2 * c->B, s));
+  } else {  // one factor per carrier, one after the other on the same stream (t is reused)
+    const int64_t h1 = gslot(c, reg, 1, 0);
+    HIPCHECK((hipError_t)launch_chol_solve(F[0].d, F[0].lv(), c->g[reg], c->t[reg], c->B, s));
+    HIPCHECK((hipError_t)launch_chol_solve(F[1].d, F[1].lv(), c->g[reg] + h1, c->t[reg], c->B, s));
   }
 }
 
-void region_recover(pecs_ctx* c, int reg, const double* b0, const double* b1, double* x0, double* x1,
-                    hipStream_t s) {
+void region_recover(pecs_ctx* c, int reg, hipStream_t s) {
   const DevRegion& R = reg == 0 ? c->Rs : c->Re;
-  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, c->W[2 * reg], c->W[2 * reg + 1], c->g[reg][0], c->g[reg][1],
-                                            c->P.mu[2 * reg], c->P.mu[2 * reg + 1], x0, x1, b0, b1, 2, s));
+  const int64_t N = reg == 0 ? c->Ns : c->Ne;
+  const int c0 = 2 * reg, c1 = 2 * reg + 1;
+  for (int i = 0; i < c->B; ++i)
+    HIPCHECK((hipError_t)launch_schur_recover(
+        c->k, R, c->W[c0] + i * c->Wsz[c0], c->W[c1] + i * c->Wsz[c1], c->g[reg] + gslot(c, reg, 0, i),
+        c->g[reg] + gslot(c, reg, 1, i), c->P.mu[c0], c->P.mu[c1], c->u[c0] + i * N, c->u[c1] + i * N,
+        c->b[c0] + i * N, c->b[c1] + i * N, 2, s));
 }
 
 void step_once(pecs_ctx* c) {
@@ -425,14 +513,19 @@ void step_once(pecs_ctx* c) {
     c->ev_used += NPHASE + 1;
     pm.mark();
   }
-  HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->ph, c->u[0], c->u[1],
-                                          c->u[2], c->u[3], c->pv, c->b[0], c->b[1], s));
+  const int64_t Ns = c->Ns, Ne = c->Ne, NP = c->NP;
+  for (int i = 0; i < c->B; ++i)
+    HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->phs[i], c->u[0] + i * Ns,
+                                            c->u[1] + i * Ns, c->u[2] + i * Ne, c->u[3] + i * Ne, c->pv + i * NP,
+                                            c->b[0] + i * Ns, c->b[1] + i * Ns, s));
   pm.mark();
-  HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->ph, c->u[2], c->u[3],
-                                          c->u[0], c->u[1], c->pv, c->b[2], c->b[3], s));
+  for (int i = 0; i < c->B; ++i)
+    HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->phs[i], c->u[2] + i * Ne,
+                                            c->u[3] + i * Ne, c->u[0] + i * Ns, c->u[1] + i * Ns, c->pv + i * NP,
+                                            c->b[2] + i * Ne, c->b[3] + i * Ne, s));
   pm.mark();
-  region_schur(c, 0, c->b[0], c->b[1], s);
-  region_schur(c, 1, c->b[2], c->b[3], s);
+  region_schur(c, 0, s);
+  region_schur(c, 1, s);
   pm.mark();
   // the two regions' solves share no data: the electrolyte runs on the side
   // stream so its launches fill the GPU where the semiconductor's leave gaps
@@ -443,16 +536,19 @@ void step_once(pecs_ctx* c) {
   HIPCHECK(hipEventRecord(c->join_ev, c->side));
   HIPCHECK(hipStreamWaitEvent(s, c->join_ev, 0));
   pm.mark();
-  region_recover(c, 0, c->b[0], c->b[1], c->u[0], c->u[1], s);
-  region_recover(c, 1, c->b[2], c->b[3], c->u[2], c->u[3], s);
+  region_recover(c, 0, s);
+  region_recover(c, 1, s);
   pm.mark();
-  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->ph, c->u[0], c->u[1], c->u[2], c->u[3],
-                                           c->mesh.semi.ncell, c->mesh.elec.ncell, c->yP, s));
+  for (int i = 0; i < c->B; ++i)
+    poisson_cells(c, i, c->u[0] + i * Ns, c->u[1] + i * Ns, c->u[2] + i * Ne, c->u[3] + i * Ne);
   pm.mark();
-  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, nullptr, c->tH, nullptr, 1, s));
+  const int64_t nH = c->FH.n;
+  for (int i = 0; i < c->B; ++i)
+    HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP + i * c->ysz, c->rH + i * nH, s));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, c->B, s));
   pm.mark();
-  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s));
+  for (int i = 0; i < c->B; ++i)
+    HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH + i * nH, c->yP + i * c->ysz, c->pv + i * NP, s));
   pm.mark();
 }
 
@@ -478,45 +574,64 @@ int fail(pecs_ctx* c, const std::exception& e) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out) {
-  // Parameters.hpp:164-251 (constants Parameters.hpp:7-12)
-  if (!in || !out) return -1;
-  const double UT = 0.02585, q = 1.62e-19, eps0 = 8.85e-14;
-  const double L = in->char_length, t = in->char_time, C = in->char_density;
-  const double mob = (t * UT) / (L * L);
-  out->mobility[0] = in->electron_mobility * mob;
-  out->mobility[1] = in->hole_mobility * mob;
-  out->mobility[2] = in->reductant_mobility * mob;
-  out->mobility[3] = in->oxidant_mobility * mob;
-  out->recombo_t[0] = in->electron_recombo_t / t;
-  out->recombo_t[1] = in->hole_recombo_t / t;
-  out->recombo_v[0] = in->electron_recombo_v * (t / L);
-  out->recombo_v[1] = in->hole_recombo_v * (t / L);
-  out->intrinsic_density = in->intrinsic_density / C;
-  out->photon_flux = in->photon_flux * (t / C);
-  out->absorption_coeff = in->absorption_coeff * L;
-  out->debye_length = (UT * eps0) / (q * C * L * L);
-  out->rescaled_k_et = q * in->k_et * C * C;
-  out->rescaled_k_ht = q * in->k_ht * C * C;
-  out->k_et = in->k_et * (t * C / L);
-  out->k_ht = in->k_ht * (t * C / L);
-  out->applied_bias = in->applied_bias / UT;
-  out->built_in_bias = in->built_in_bias / UT;
-  out->schottky_bias = in->schottky_bias / UT;
-  out->rescale_current = (q * C * L) / t;
-  return 0;
+DevPhys make_phys(const Params& P, double H) {
+  DevPhys ph{};
+  ph.dt = P.dt;
+  ph.penalty = P.penalty;
+  ph.k_et = P.k_et;
+  ph.k_ht = P.k_ht;
+  ph.v_rec_n = P.v_rec_n;
+  ph.v_rec_p = P.v_rec_p;
+  ph.ne = P.ne;
+  ph.pe = P.pe;
+  ph.rinf = P.rinf;
+  ph.oinf = P.oinf;
+  ph.eps_s = P.eps_s;
+  ph.eps_e = P.eps_e;
+  ph.G0 = P.G0;
+  ph.alpha = P.alpha;
+  ph.H = H;
+  ph.ni = P.ni;
+  ph.tau_n = P.tau_n;
+  ph.tau_p = P.tau_p;
+  ph.phi_bi = P.phi_bi;
+  ph.phi_app = P.phi_app;
+  ph.phi_sch = P.phi_sch;
+  ph.gen_mode = P.gen_mode;
+  ph.srh = P.srh;
+  return ph;
 }
 
-const char* pecs_global_error(void) { return g_error.c_str(); }
+// members of a batch share the mesh and every constant matrix, so the
+// parameters that enter the matrices (or the Schottky data's location) must agree
+void validate_batch(const pecs_params* q, int B) {
+  for (int i = 1; i < B; ++i) {
+    const pecs_params& a = q[0];
+    const pecs_params& b = q[i];
+    bool same = a.degree == b.degree && a.dt == b.dt && a.theta == b.theta && a.penalty == b.penalty &&
+                a.eps_s == b.eps_s && a.eps_e == b.eps_e && a.lambda2 == b.lambda2 && a.leaf_cells == b.leaf_cells &&
+                a.schottky_location_set == b.schottky_location_set &&
+                (!a.schottky_location_set || a.schottky_location == b.schottky_location);
+    for (int j = 0; j < 4; ++j) same = same && a.mu[j] == b.mu[j];
+    if (!same)
+      throw std::runtime_error("batch member " + std::to_string(i) +
+                               " differs from member 0 in a parameter that enters the matrices (degree, dt, theta, "
+                               "penalty, mobilities, permittivities, lambda2, leaf size, Schottky location)");
+  }
+}
 
-int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out) {
+int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ctx** out) {
   if (!out) return -1;
   *out = nullptr;
   auto c = std::make_unique<pecs_ctx>();
   try {
-    validate(mesh, params);
+    if (B < 1 || B > 16) throw std::runtime_error("batch size must be 1..16");
+    if (!params) throw std::runtime_error("null params");
+    for (int i = 0; i < B; ++i) validate(mesh, &params[i]);
+    validate_batch(params, B);
+    c->B = B;
     auto t0 = std::chrono::steady_clock::now();
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -525,6 +640,7 @@ int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out
     HIPCHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     c->P = to_params(params);
+    for (int i = 0; i < B; ++i) c->Pb.push_back(to_params(&params[i]));
     c->k = params->degree;
     const int leaf = params->leaf_cells > 0 ? params->leaf_cells : 16;
     c->mesh = make_solarcell_mesh(mesh->n_refine, to_geo(mesh));
@@ -541,11 +657,11 @@ int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out
     auto tf = std::chrono::steady_clock::now();
     c->share_s = c->P.mu[0] == c->P.mu[1];
     c->share_e = c->P.mu[2] == c->P.mu[3];
-    c->Fs[0] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[0]), leaf, c->share_s ? 2 : 1);
-    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[1]), leaf, 1);
-    c->Fe[0] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[2]), leaf, c->share_e ? 2 : 1);
-    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[3]), leaf, 1);
-    c->FH = make_factor(c.get(), multiplier_matrix(c->mesh, op, T), leaf, 1);
+    c->Fs[0] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[0]), leaf, c->share_s ? 2 * B : B);
+    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[1]), leaf, B);
+    c->Fe[0] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[2]), leaf, c->share_e ? 2 * B : B);
+    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[3]), leaf, B);
+    c->FH = make_factor(c.get(), multiplier_matrix(c->mesh, op, T), leaf, B);
     c->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
     // reference tables
     DevTabs& tb = c->tabs;
@@ -607,52 +723,38 @@ int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out
     P.mloc = upload(c.get(), mloc);
     P.mcoef = upload(c.get(), mcoef);
     P.gw = upload(c.get(), gw);
-    // physics
-    DevPhys& ph = c->ph;
-    ph.dt = c->P.dt;
-    ph.penalty = c->P.penalty;
-    ph.k_et = c->P.k_et;
-    ph.k_ht = c->P.k_ht;
-    ph.v_rec_n = c->P.v_rec_n;
-    ph.v_rec_p = c->P.v_rec_p;
-    ph.ne = c->P.ne;
-    ph.pe = c->P.pe;
-    ph.rinf = c->P.rinf;
-    ph.oinf = c->P.oinf;
-    ph.eps_s = c->P.eps_s;
-    ph.eps_e = c->P.eps_e;
-    ph.G0 = c->P.G0;
-    ph.alpha = c->P.alpha;
-    ph.H = m.geo.H;
-    ph.ni = c->P.ni;
-    ph.tau_n = c->P.tau_n;
-    ph.tau_p = c->P.tau_p;
-    ph.gen_mode = c->P.gen_mode;
-    ph.srh = c->P.srh;
-    // state and scratch
+    // physics, per instance
+    for (int i = 0; i < B; ++i) c->phs.push_back(make_phys(c->Pb[i], m.geo.H));
+    c->ph = c->phs[0];
+    // state and scratch (instance-major)
     const int64_t nsv[4] = {c->Ns, c->Ns, c->Ne, c->Ne};
     for (int i = 0; i < 4; ++i) {
-      c->u[i] = dalloc<double>(c.get(), nsv[i]);
-      c->b[i] = dalloc<double>(c.get(), nsv[i]);
-      c->W[i] = dalloc<double>(c.get(), 2LL * nb * (i < 2 ? m.semi.ncell : m.elec.ncell));
-      c->old[i] = dalloc<double>(c.get(), nsv[i]);
+      c->Wsz[i] = 2LL * nb * (i < 2 ? m.semi.ncell : m.elec.ncell);
+      c->u[i] = dalloc<double>(c.get(), B * nsv[i]);
+      c->b[i] = dalloc<double>(c.get(), B * nsv[i]);
+      c->W[i] = dalloc<double>(c.get(), B * c->Wsz[i]);
+      c->old[i] = dalloc<double>(c.get(), B * nsv[i]);
       c->io[i] = dalloc<double>(c.get(), std::max(nsv[i], c->NP));
-      HIPCHECK(hipMemset(c->u[i], 0, nsv[i] * sizeof(double)));
+      HIPCHECK(hipMemset(c->u[i], 0, B * nsv[i] * sizeof(double)));
     }
-    c->pv = dalloc<double>(c.get(), c->NP);
-    c->old[4] = dalloc<double>(c.get(), c->NP);
-    HIPCHECK(hipMemset(c->pv, 0, c->NP * sizeof(double)));
+    c->pv = dalloc<double>(c.get(), B * c->NP);
+    c->old[4] = dalloc<double>(c.get(), B * c->NP);
+    HIPCHECK(hipMemset(c->pv, 0, B * c->NP * sizeof(double)));
+    // scratch right-hand sides of a solve: 2B (or B) h-major, or the padded
+    // interleaved block of the MFMA sweep
+    auto scratch_rhs = [](int nrhs) { return nrhs > 2 ? mfma_rhs_pad(nrhs) : nrhs; };
     for (int r = 0; r < 2; ++r) {
-      const int n = r == 0 ? c->Fs[0].n : c->Fe[0].n;
-      for (int j = 0; j < 2; ++j) {
-        c->g[r][j] = dalloc<double>(c.get(), n);
-        c->t[r][j] = dalloc<double>(c.get(), n);
-      }
+      const int64_t n = r == 0 ? c->Fs[0].n : c->Fe[0].n;
+      c->g[r] = dalloc<double>(c.get(), 2 * B * n);
+      c->t[r] = dalloc<double>(c.get(), std::max(2 * B, scratch_rhs(2 * B)) * n);
+      HIPCHECK(hipMemset(c->t[r], 0, std::max(2 * B, scratch_rhs(2 * B)) * n * sizeof(double)));
     }
-    c->yP = dalloc<double>(c.get(), (size_t)m.ncellP * op.nloc);
-    c->rH = dalloc<double>(c.get(), c->FH.n);
-    c->tH = dalloc<double>(c.get(), c->FH.n);
-    c->acc = dalloc<double>(c.get(), 2);
+    c->ysz = (int64_t)m.ncellP * op.nloc;
+    c->yP = dalloc<double>(c.get(), B * c->ysz);
+    c->rH = dalloc<double>(c.get(), B * (size_t)c->FH.n);
+    c->tH = dalloc<double>(c.get(), std::max(B, scratch_rhs(B)) * (size_t)c->FH.n);
+    c->acc = dalloc<double>(c.get(), 2 * B);
+    c->last_res.assign(B, -1.0);
     HIPCHECK(hipDeviceSynchronize());
     c->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   } catch (const std::exception& e) {
@@ -660,6 +762,56 @@ int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out
     return -1;
   }
   *out = c.release();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out) {
+  // Parameters.hpp:164-251 (constants Parameters.hpp:7-12)
+  if (!in || !out) return -1;
+  const double UT = 0.02585, q = 1.62e-19, eps0 = 8.85e-14;
+  const double L = in->char_length, t = in->char_time, C = in->char_density;
+  const double mob = (t * UT) / (L * L);
+  out->mobility[0] = in->electron_mobility * mob;
+  out->mobility[1] = in->hole_mobility * mob;
+  out->mobility[2] = in->reductant_mobility * mob;
+  out->mobility[3] = in->oxidant_mobility * mob;
+  out->recombo_t[0] = in->electron_recombo_t / t;
+  out->recombo_t[1] = in->hole_recombo_t / t;
+  out->recombo_v[0] = in->electron_recombo_v * (t / L);
+  out->recombo_v[1] = in->hole_recombo_v * (t / L);
+  out->intrinsic_density = in->intrinsic_density / C;
+  out->photon_flux = in->photon_flux * (t / C);
+  out->absorption_coeff = in->absorption_coeff * L;
+  out->debye_length = (UT * eps0) / (q * C * L * L);
+  out->rescaled_k_et = q * in->k_et * C * C;
+  out->rescaled_k_ht = q * in->k_ht * C * C;
+  out->k_et = in->k_et * (t * C / L);
+  out->k_ht = in->k_ht * (t * C / L);
+  out->applied_bias = in->applied_bias / UT;
+  out->built_in_bias = in->built_in_bias / UT;
+  out->schottky_bias = in->schottky_bias / UT;
+  out->rescale_current = (q * C * L) / t;
+  return 0;
+}
+
+const char* pecs_global_error(void) { return g_error.c_str(); }
+
+
+int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out) {
+  return create_impl(mesh, params, 1, out);
+}
+
+int pecs_create_batch(const pecs_mesh* mesh, const pecs_params* params, int batch, pecs_ctx** out) {
+  return create_impl(mesh, params, batch, out);
+}
+
+int pecs_batch_size(pecs_ctx* ctx, int* batch) {
+  if (!ctx || !batch) return -1;
+  *batch = ctx->B;
   return 0;
 }
 
@@ -787,7 +939,7 @@ int pecs_solve_dev(pecs_ctx* ctx, int which, const double* rhs, double* x) {
       solve_carrier(ctx, which, rhs, x);
     } else if (which == PECS_POISSON) {
       HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, rhs, ctx->yP, ctx->stream));
-      poisson_solve_from_y(ctx, x);
+      poisson_solve_from_y(ctx, 0, 1, x);
     } else {
       throw std::runtime_error("unknown system id");
     }
@@ -851,7 +1003,7 @@ int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x) {
     h2d(ctx, ctx->io[0], rhs, n);
     if (which == PECS_POISSON) {
       HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, ctx->io[0], ctx->yP, ctx->stream));
-      poisson_solve_from_y(ctx, ctx->io[1]);
+      poisson_solve_from_y(ctx, 0, 1, ctx->io[1]);
     } else {
       solve_carrier(ctx, which, ctx->io[0], ctx->io[1]);
     }
@@ -859,44 +1011,62 @@ int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x) {
   })
 }
 
-int pecs_set_state(pecs_ctx* ctx, const double* un, const double* up, const double* ur, const double* uo,
-                   const double* pv) {
+int pecs_set_state_batch(pecs_ctx* ctx, int i, const double* un, const double* up, const double* ur,
+                         const double* uo, const double* pv) {
   PECS_TRY(ctx, {
-    h2d(ctx, ctx->u[0], un, ctx->Ns);
-    h2d(ctx, ctx->u[1], up, ctx->Ns);
-    h2d(ctx, ctx->u[2], ur, ctx->Ne);
-    h2d(ctx, ctx->u[3], uo, ctx->Ne);
-    if (pv) h2d(ctx, ctx->pv, pv, ctx->NP);
-    else poisson_from_densities(ctx, ctx->u[0], ctx->u[1], ctx->u[2], ctx->u[3], ctx->pv);
+    if (i < 0 || i >= ctx->B) throw std::runtime_error("instance index out of range");
+    const int64_t Ns = ctx->Ns, Ne = ctx->Ne, NP = ctx->NP;
+    h2d(ctx, ctx->u[0] + i * Ns, un, Ns);
+    h2d(ctx, ctx->u[1] + i * Ns, up, Ns);
+    h2d(ctx, ctx->u[2] + i * Ne, ur, Ne);
+    h2d(ctx, ctx->u[3] + i * Ne, uo, Ne);
+    if (pv) {
+      h2d(ctx, ctx->pv + i * NP, pv, NP);
+    } else {
+      poisson_cells(ctx, i, ctx->u[0] + i * Ns, ctx->u[1] + i * Ns, ctx->u[2] + i * Ne, ctx->u[3] + i * Ne);
+      poisson_solve_from_y(ctx, i, 1, ctx->pv + i * NP);
+    }
     HIPCHECK(hipStreamSynchronize(ctx->stream));
   })
 }
 
-int pecs_get_state(pecs_ctx* ctx, double* un, double* up, double* ur, double* uo, double* pv) {
+int pecs_set_state(pecs_ctx* ctx, const double* un, const double* up, const double* ur, const double* uo,
+                   const double* pv) {
+  return pecs_set_state_batch(ctx, 0, un, up, ur, uo, pv);
+}
+
+int pecs_get_state_batch(pecs_ctx* ctx, int i, double* un, double* up, double* ur, double* uo, double* pv) {
   PECS_TRY(ctx, {
-    if (un) d2h(ctx, un, ctx->u[0], ctx->Ns);
-    if (up) d2h(ctx, up, ctx->u[1], ctx->Ns);
-    if (ur) d2h(ctx, ur, ctx->u[2], ctx->Ne);
-    if (uo) d2h(ctx, uo, ctx->u[3], ctx->Ne);
-    if (pv) d2h(ctx, pv, ctx->pv, ctx->NP);
+    if (i < 0 || i >= ctx->B) throw std::runtime_error("instance index out of range");
+    const int64_t Ns = ctx->Ns, Ne = ctx->Ne, NP = ctx->NP;
+    if (un) d2h(ctx, un, ctx->u[0] + i * Ns, Ns);
+    if (up) d2h(ctx, up, ctx->u[1] + i * Ns, Ns);
+    if (ur) d2h(ctx, ur, ctx->u[2] + i * Ne, Ne);
+    if (uo) d2h(ctx, uo, ctx->u[3] + i * Ne, Ne);
+    if (pv) d2h(ctx, pv, ctx->pv + i * NP, NP);
   })
+}
+
+int pecs_get_state(pecs_ctx* ctx, double* un, double* up, double* ur, double* uo, double* pv) {
+  return pecs_get_state_batch(ctx, 0, un, up, ur, uo, pv);
 }
 
 int pecs_initial_state(pecs_ctx* ctx) {
   PECS_TRY(ctx, {
     // VectorTools::project of the constant equilibria (SolarCell.cpp:1999-2021):
     // the L2 projection of a constant onto FE_DGQ is that constant, J = 0.
-    const int nb = ctx->T->nb;
-    const double vals[4] = {ctx->P.ne, ctx->P.pe, ctx->P.rinf, ctx->P.oinf};
     const int64_t nsv[4] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne};
-    for (int i = 0; i < 4; ++i) {
-      std::vector<double> h(nsv[i], 0.0);
-      const int64_t n3 = nsv[i] / 3;
-      for (int64_t j = 2 * n3; j < nsv[i]; ++j) h[j] = vals[i];
-      HIPCHECK(hipMemcpy(ctx->u[i], h.data(), nsv[i] * sizeof(double), hipMemcpyHostToDevice));
+    for (int i = 0; i < ctx->B; ++i) {
+      const Params& P = ctx->Pb[i];
+      const double vals[4] = {P.ne, P.pe, P.rinf, P.oinf};
+      for (int c = 0; c < 4; ++c) {
+        std::vector<double> h(nsv[c], 0.0);
+        const int64_t n3 = nsv[c] / 3;
+        for (int64_t j = 2 * n3; j < nsv[c]; ++j) h[j] = vals[c];
+        HIPCHECK(hipMemcpy(ctx->u[c] + i * nsv[c], h.data(), nsv[c] * sizeof(double), hipMemcpyHostToDevice));
+      }
     }
-    (void)nb;
-    poisson_from_densities(ctx, ctx->u[0], ctx->u[1], ctx->u[2], ctx->u[3], ctx->pv);
+    poisson_all_from_densities(ctx);
     HIPCHECK(hipStreamSynchronize(ctx->stream));
   })
 }
@@ -904,28 +1074,34 @@ int pecs_initial_state(pecs_ctx* ctx) {
 int pecs_step(pecs_ctx* ctx, int nsteps) {
   PECS_TRY(ctx, {
     if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
+    const int B = ctx->B;
+    const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
     for (int i = 0; i < nsteps; ++i) {
       if (i == nsteps - 1) {
-        const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
         for (int j = 0; j < 4; ++j)
-          HIPCHECK(hipMemcpyAsync(ctx->old[j], ctx->u[j], nsv[j] * sizeof(double), hipMemcpyDeviceToDevice,
+          HIPCHECK(hipMemcpyAsync(ctx->old[j], ctx->u[j], B * nsv[j] * sizeof(double), hipMemcpyDeviceToDevice,
                                   ctx->stream));
-        HIPCHECK(hipMemcpyAsync(ctx->old[4], ctx->pv, nsv[4] * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(hipMemcpyAsync(ctx->old[4], ctx->pv, B * nsv[4] * sizeof(double), hipMemcpyDeviceToDevice,
+                                ctx->stream));
       }
       step_once(ctx);
       if (ctx->timing && ctx->ev_used + NPHASE + 1 > (int)ctx->events.size()) collect_timing(ctx);
     }
     collect_timing(ctx);
     if (nsteps > 0) {
-      const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
-      HIPCHECK(hipMemsetAsync(ctx->acc, 0, 2 * sizeof(double), ctx->stream));
-      for (int j = 0; j < 4; ++j)
-        HIPCHECK((hipError_t)launch_diff_norms(ctx->u[j], ctx->old[j], nsv[j], ctx->acc, ctx->stream));
-      HIPCHECK((hipError_t)launch_diff_norms(ctx->pv, ctx->old[4], nsv[4], ctx->acc, ctx->stream));
-      double h[2];
-      HIPCHECK(hipMemcpyAsync(h, ctx->acc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipMemsetAsync(ctx->acc, 0, 2 * B * sizeof(double), ctx->stream));
+      for (int b = 0; b < B; ++b) {
+        for (int j = 0; j < 4; ++j)
+          HIPCHECK((hipError_t)launch_diff_norms(ctx->u[j] + b * nsv[j], ctx->old[j] + b * nsv[j], nsv[j],
+                                                 ctx->acc + 2 * b, ctx->stream));
+        HIPCHECK((hipError_t)launch_diff_norms(ctx->pv + b * nsv[4], ctx->old[4] + b * nsv[4], nsv[4],
+                                               ctx->acc + 2 * b, ctx->stream));
+      }
+      std::vector<double> h(2 * B);
+      HIPCHECK(hipMemcpyAsync(h.data(), ctx->acc, 2 * B * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
       HIPCHECK(hipStreamSynchronize(ctx->stream));
-      ctx->last_res = (h[1] > 0) ? std::sqrt(h[0] / h[1]) / ctx->P.dt : 0.0;
+      for (int b = 0; b < B; ++b)
+        ctx->last_res[b] = (h[2 * b + 1] > 0) ? std::sqrt(h[2 * b] / h[2 * b + 1]) / ctx->P.dt : 0.0;
     }
   })
 }
@@ -975,7 +1151,15 @@ int pecs_factor_bytes(pecs_ctx* ctx, int which, double* bytes) {
 
 int pecs_last_residual(pecs_ctx* ctx, double* r) {
   if (!ctx || !r) return -1;
-  *r = ctx->last_res;
+  double m = -1.0;
+  for (double v : ctx->last_res) m = std::max(m, v);
+  *r = m;
+  return 0;
+}
+
+int pecs_last_residual_batch(pecs_ctx* ctx, double* r) {
+  if (!ctx || !r) return -1;
+  for (int i = 0; i < ctx->B; ++i) r[i] = ctx->last_res[i];
   return 0;
 }
 

@@ -502,6 +502,71 @@ PackedFactor pack_factor(const SupernodalFactor& F) {
   return P;
 }
 
+void pack_blocked(PackedFactor& P) {
+  const int nn = (int)P.p0.size();
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool fwd = dir == 0;
+    std::vector<int>& b0 = fwd ? P.fblk0 : P.bblk0;
+    std::vector<int64_t>& boff = fwd ? P.fblk_off : P.bblk_off;
+    b0.assign(nn + 1, 0);
+    boff.clear();
+    int64_t total = 0;
+    for (int s = 0; s < nn; ++s) {
+      b0[s] = (int)boff.size();
+      const int rows = fwd ? P.ns[s] + P.ms[s] : P.ns[s];
+      for (int b = 0; b * 16 < rows; ++b) {
+        boff.push_back(total);
+        const int nsteps = (blk_chi(fwd, b, P.ns[s], P.ms[s]) - blk_clo(fwd, b) + 15) / 16;
+        total += (int64_t)nsteps * 256;
+      }
+    }
+    b0[nn] = (int)boff.size();
+    boff.push_back(total);
+    std::vector<double>& dst = fwd ? P.fblk : P.bblk;
+    dst.assign(total, 0.0);
+    const std::vector<double>& src = fwd ? P.fwd : P.bwd;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nn; ++s) {
+      const int ns = P.ns[s], ms = P.ms[s];
+      const int rows = fwd ? ns + ms : ns;
+      const int64_t base = fwd ? P.fwd_off[s] : P.bwd_off[s];
+      for (int b = 0; b * 16 < rows; ++b) {
+        const int clo = blk_clo(fwd, b), chi = blk_chi(fwd, b, ns, ms);
+        const int nsteps = (chi - clo + 15) / 16;
+        double* d = &dst[boff[b0[s] + b]];
+        for (int i = 0; i < 16; ++i) {
+          const int lr = 16 * b + i;
+          if (lr >= rows) continue;
+          int64_t off;
+          int rs, re;
+          if (fwd) {
+            rs = 0;
+            if (lr < ns) {
+              off = (int64_t)lr * (lr + 1) / 2;
+              re = lr + 1;
+            } else {
+              off = (int64_t)ns * (ns + 1) / 2 + (int64_t)(lr - ns) * ns;
+              re = ns;
+            }
+          } else {
+            off = (int64_t)lr * (ns + ms) - (int64_t)lr * (lr - 1) / 2 - lr;
+            rs = lr;
+            re = ns + ms;
+          }
+          for (int st = 0; st < nsteps; ++st)
+            for (int k = 0; k < 4; ++k)
+              for (int q = 0; q < 4; ++q) {
+                const int col = clo + 16 * st + 4 * k + q;
+                if (col >= rs && col < re) d[((int64_t)st * 4 + q) * 64 + i + 16 * k] = src[base + off + col];
+              }
+        }
+      }
+    }
+  }
+  std::vector<double>().swap(P.fwd);
+  std::vector<double>().swap(P.bwd);
+}
+
 void level_tiles(const PackedFactor& P, int l, bool fwd, int rows, std::vector<int>& out) {
   if (rows < 1) throw std::runtime_error("level_tiles: rows must be >= 1");
   for (int q = P.lnoff[l]; q < P.lnoff[l + 1]; ++q) {
@@ -517,7 +582,10 @@ void rebuild_tiles(PackedFactor& P) {
   P.foff.assign(P.nlevels + 1, 0);
   P.boff.assign(P.nlevels + 1, 0);
   for (int l = 0; l < P.nlevels; ++l) {
-    if (P.frows[l] > (256 / P.fw[l]) * P.fm[l] || P.brows[l] > (256 / P.bw[l]) * P.bm[l])
+    // lanes per row W > 0: (256 / W) * MAXR rows per tile; W == 0 (the MFMA
+    // sweep): 64 * MAXR rows per tile
+    auto cap = [](int w, int m) { return w > 0 ? (256 / w) * m : 64 * m; };
+    if (P.frows[l] > cap(P.fw[l], P.fm[l]) || P.brows[l] > cap(P.bw[l], P.bm[l]))
       throw std::runtime_error("rebuild_tiles: tile height exceeds the rows one workgroup covers");
     P.foff[l] = (int)(P.ftiles.size() / 4);
     P.boff[l] = (int)(P.btiles.size() / 4);

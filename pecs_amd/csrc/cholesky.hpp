@@ -71,8 +71,29 @@ struct PackedFactor {
   std::vector<int> frows, brows;      // tile height per level (<= (256 / W) * MAXR)
   std::vector<int> lnodes, lnoff;     // nodes of each level + offsets
   double tri_bytes = 0.0;             // bytes read by one forward + backward sweep
+  // MFMA-blocked copies (see pack_blocked): per node, blocks of 16 rows; a
+  // block holds its columns [clo, clo + 16 nsteps) in 16-column steps of four
+  // 64-entry groups in MFMA A-operand lane order
+  std::vector<double> fblk, bblk;
+  std::vector<int64_t> fblk_off, bblk_off;  // per block: offset of its data
+  std::vector<int> fblk0, bblk0;            // per node: index of its first block
 };
 PackedFactor pack_factor(const SupernodalFactor& F);
+
+// Build the MFMA-blocked copies from fwd / bwd and release the row-major
+// arrays.  Block b of node s covers node rows [16b, 16b + 16) (forward: the
+// ns + ms rows of [Linv; W21]; backward: the ns rows of [Linv^T | -W21^T]);
+// its columns start at clo = 0 (forward) or 16b (backward, the first row's own
+// column) and end at the block's longest row.  Entry (step st, group q, lane
+// l = i + 16 k) is row 16b + i, column clo + 16 st + 4 k + q, zero outside the
+// row.  A wave streaming one block reads 512 contiguous bytes per load.
+void pack_blocked(PackedFactor& P);
+inline int blk_clo(bool fwd, int b) { return fwd ? 0 : 16 * b; }
+inline int blk_chi(bool fwd, int b, int ns, int ms) {
+  if (!fwd) return ns + ms;
+  const int last = std::min(16 * b + 15, ns + ms - 1);
+  return last < ns ? last + 1 : ns;
+}
 
 // Row tiles of level l (forward: all ns + ms rows of each node; backward: the
 // ns own rows) with `rows` rows per tile, appended as (node, row0, nrows, 0).

@@ -422,33 +422,28 @@ struct SweepShape {
   static constexpr int RB = R * MAXR;
 };
 
-// forward, part 1: t_p = b_p - sum of the updates z owed to row p, into X
-template <int NRHS>
+// forward, part 1: t_p = b_p - sum of the updates z owed to row p, into X.
+// Right-hand side h = blockIdx.y; vectors are h-major (b[h n + p], Z[h zlen + z]).
 __global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int* __restrict__ lrows, int nrow,
-                                                    const double* __restrict__ b0, const double* __restrict__ b1) {
+                                                    const double* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrow) return;
+  const int h = blockIdx.y;
   const int p = lrows[i];
-  double a0 = b0[p], a1 = (NRHS > 1) ? b1[p] : 0.0;
+  const double* __restrict__ Zh = F.Z + (int64_t)h * F.zlen;
+  double a = b[(int64_t)h * F.n + p];
   const int64_t c1 = F.cptr[p + 1];
-  for (int64_t c = F.cptr[p]; c < c1; ++c) {
-    const int64_t z = F.czidx[c];
-    a0 -= F.Z[z];
-    if (NRHS > 1) a1 -= F.Z[F.zlen + z];
-  }
-  F.X[p] = a0;
-  if (NRHS > 1) F.X[F.n + p] = a1;
+  for (int64_t c = F.cptr[p]; c < c1; ++c) a -= Zh[F.czidx[c]];
+  F.X[(int64_t)h * F.n + p] = a;
 }
 
 // backward, part 1: Z[q] = x[rows[q]] for the panel rows of the level's nodes
-template <int NRHS>
 __global__ __launch_bounds__(256) void k_bwd_gather(DevFactor F, const int64_t* __restrict__ idx, int nidx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nidx) return;
+  const int h = blockIdx.y;
   const int64_t q = idx[i];
-  const int p = F.rows[q];
-  F.Z[q] = F.X[p];
-  if (NRHS > 1) F.Z[F.zlen + q] = F.X[F.n + p];
+  F.Z[(int64_t)h * F.zlen + q] = F.X[(int64_t)h * F.n + F.rows[q]];
 }
 
 template <int NRHS, int W, int MAXR_, bool FWD>
@@ -568,6 +563,156 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
   }
 }
 
+// Many right-hand sides (a batch of instances sharing the factor): the row
+// tile times the vector block is a skinny GEMM, done with FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  These solves keep their vectors RHS-interleaved
+// (entry p of right-hand side h at [p * NRP + h], NRP = 16 or 32 padded
+// right-hand sides, the padding zero), so the B operand of a 16-column step is
+// four 128-byte rows straight from L2: no LDS staging and no barriers.  Each
+// wave owns MB blocks of 16 rows; per step a lane loads 4 consecutive entries
+// of one row (lane l: row l%16, columns 4(l/16)..4(l/16)+3) and issues 4 MFMAs
+// whose B operand holds the vector at the same columns (lane l: column
+// 4(l/16)+e, right-hand side l%16), so the k order of A and B agree.
+// Accumulator lane l, register r holds row l/16 + 4r, right-hand side l%16.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// t_p = b_p - sum of the updates z owed to row p, interleaved (h fastest)
+template <int NRP>
+__global__ __launch_bounds__(256) void k_fwd_gather_il(DevFactor F, const int* __restrict__ lrows, int nrow,
+                                                       const double* __restrict__ b, int nrhs) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nrow * NRP) return;
+  const int i = (int)(e / NRP), h = (int)(e % NRP);
+  const int p = lrows[i];
+  double a = 0.0;
+  if (h < nrhs) {
+    a = b[(int64_t)h * F.n + p];
+    const int64_t c1 = F.cptr[p + 1];
+    for (int64_t c = F.cptr[p]; c < c1; ++c) a -= F.Z[F.czidx[c] * NRP + h];
+  }
+  F.X[(int64_t)p * NRP + h] = a;
+}
+
+template <int NRP>
+__global__ __launch_bounds__(256) void k_bwd_gather_il(DevFactor F, const int64_t* __restrict__ idx, int nidx) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nidx * NRP) return;
+  const int i = (int)(e / NRP), h = (int)(e % NRP);
+  const int64_t q = idx[i];
+  F.Z[q * NRP + h] = F.X[(int64_t)F.rows[q] * NRP + h];
+}
+
+// x (h-major, nrhs right-hand sides) from the interleaved result
+template <int NRP>
+__global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restrict__ v, int nrhs) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)F.n * NRP) return;
+  const int64_t p = e / NRP;
+  const int h = (int)(e % NRP);
+  if (h < nrhs) v[(int64_t)h * F.n + p] = F.X[e];
+}
+
+template <int NRP, int MB, bool FWD>
+__global__ __launch_bounds__(256) void k_sweep_mfma(DevFactor F, const int4* __restrict__ tiles,
+                                                    double* __restrict__ Y) {
+  constexpr int NH = NRP / 16;
+  const int4 tl = tiles[blockIdx.x];
+  const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
+  const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
+  const int VL = FWD ? ns : ns + ms;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ri = lane & 15, kq = lane >> 4;
+  const int first = row0 + wave * MB * 16;               // wave-uniform
+  if (first >= row0 + nr) return;                        // no rows for this wave (no barriers below)
+  const int last = min(row0 + nr, first + MB * 16) - 1;
+  const int lo = FWD ? 0 : first;
+  const int hi = FWD ? (last < ns ? last + 1 : ns) : ns + ms;
+  // the wave's blocks in the MFMA-blocked copy: data base, first column, steps
+  const double* __restrict__ blk = FWD ? F.fblk : F.bblk;
+  const int64_t* __restrict__ boff = FWD ? F.fblk_off : F.bblk_off;
+  const int bnode = FWD ? F.fblk0[s] : F.bblk0[s];
+  int64_t bbase[MB];
+  int bclo[MB], bsteps[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) {
+    const int b = first / 16 + mb;
+    bbase[mb] = 0;
+    bclo[mb] = 0;
+    bsteps[mb] = 0;
+    if (first + mb * 16 < row0 + nr) {
+      bbase[mb] = boff[bnode + b];
+      bclo[mb] = FWD ? 0 : 16 * b;
+      bsteps[mb] = (int)((boff[bnode + b + 1] - bbase[mb]) >> 8);
+    }
+  }
+  v4d acc[MB][NH];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int hg = 0; hg < NH; ++hg) acc[mb][hg] = v4d{0.0, 0.0, 0.0, 0.0};
+  // vector entry at column col of right-hand side h (interleaved storage)
+  auto vecv = [&](int col, int h) -> double {
+    if (col >= VL) return 0.0;
+    if (FWD) return F.X[(int64_t)(p0 + col) * NRP + h];
+    if (col < ns) return Y[(int64_t)(p0 + col) * NRP + h];
+    return F.Z[(F.rows_off[s] + (col - ns)) * NRP + h];
+  };
+  const int c0 = (lo >> 4) << 4;
+  const int nstep = (hi - c0 + 15) >> 4;
+  auto load = [&](int st, double (&L)[MB][4], double (&Bv)[4][NH]) {
+    const int cc = c0 + 16 * st;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int idx = (cc - bclo[mb]) >> 4;  // the block's own step (wave-uniform)
+      const bool in = cc >= bclo[mb] && idx < bsteps[mb];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        L[mb][q] = in ? __builtin_nontemporal_load(blk + bbase[mb] + ((int64_t)idx * 4 + q) * 64 + lane) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int col = cc + 4 * kq + q;
+#pragma unroll
+      for (int hg = 0; hg < NH; ++hg) Bv[q][hg] = vecv(col, hg * 16 + ri);
+    }
+  };
+  auto mma = [&](const double (&L)[MB][4], const double (&Bv)[4][NH]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int hg = 0; hg < NH; ++hg)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[mb][hg] = __builtin_amdgcn_mfma_f64_16x16x4f64(L[mb][q], Bv[q][hg], acc[mb][hg], 0, 0, 0);
+  };
+  // loads run one step ahead of the MFMAs that use them (two register sets)
+  double LA[MB][4], LB[MB][4], BA[4][NH], BB[4][NH];
+  if (nstep > 0) load(0, LA, BA);
+  for (int st = 0; st < nstep; st += 2) {
+    if (st + 1 < nstep) load(st + 1, LB, BB);
+    mma(LA, BA);
+    if (st + 2 < nstep) load(st + 2, LA, BA);
+    if (st + 1 < nstep) mma(LB, BB);
+  }
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int hg = 0; hg < NH; ++hg)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lr = first + mb * 16 + kq + 4 * r;
+        const int h = hg * 16 + ri;
+        if (lr >= row0 + nr) continue;
+        const double v = acc[mb][hg][r];
+        if (FWD) {
+          if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
+          else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
+        } else {
+          F.X[(int64_t)(p0 + lr) * NRP + h] = v;
+        }
+      }
+}
+
 // ------------------------------------------------------------------------
 // Hybridised mixed Poisson
 // ------------------------------------------------------------------------
@@ -610,6 +755,14 @@ __device__ __forceinline__ void poisson_fphi(const DevPoisson& P, const DevTabs&
   }
 }
 
+// constant Dirichlet / Schottky data of D row i from the cell's unit parts
+template <int K>
+__device__ __forceinline__ double poisson_bias(const double* tb, const DevPhys& ph, int i) {
+  constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
+  const double* U = tb + NL * NL;
+  return ph.phi_bi * U[i] - ph.phi_app * U[NRT + i] - ph.phi_sch * U[2 * NRT + i];
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, DevPhys ph, const double* __restrict__ un,
                                                      const double* __restrict__ up, const double* __restrict__ ur,
@@ -622,7 +775,7 @@ __global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, De
   double f[NL];
   const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
 #pragma unroll
-  for (int i = 0; i < NRT; ++i) f[i] = tb[NL * NL + i];
+  for (int i = 0; i < NRT; ++i) f[i] = poisson_bias<K>(tb, ph, i);
   poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, f + NRT);
   for (int i = 0; i < NL; ++i) {
     double s = 0.0;
@@ -747,7 +900,7 @@ __global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTa
     int m;
     const int fc = face_of_local<K>(i, m);
     if (fc >= 0 && !owns_face<K>(P, Kc, fc)) continue;
-    b[id[i]] = tb[NL * NL + i];
+    b[id[i]] = poisson_bias<K>(tb, ph, i);
   }
 #pragma unroll
   for (int l = 0; l < NPHI; ++l) b[id[NRT + l]] = fphi[l];
@@ -844,54 +997,90 @@ static int sweep_dispatch(const DevFactor& F, const int4* tl, int nt, int w, int
   PECS_SWEEP_CASE(16, 1) PECS_SWEEP_CASE(16, 2) PECS_SWEEP_CASE(16, 4)
   PECS_SWEEP_CASE(32, 1) PECS_SWEEP_CASE(32, 2) PECS_SWEEP_CASE(32, 4)
   PECS_SWEEP_CASE(64, 1) PECS_SWEEP_CASE(64, 2) PECS_SWEEP_CASE(64, 4)
+  PECS_SWEEP_CASE(32, 8) PECS_SWEEP_CASE(64, 8)
 #undef PECS_SWEEP_CASE
   return (int)hipErrorInvalidValue;
 }
 
-int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 const double* v0, const double* v1, double* t0, double* t1, hipStream_t s) {
-  if (nt <= 0) return 0;
-  if (nrhs != 2) {
-    v1 = v0;
-    t1 = t0;
+template <bool FWD>
+static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int mb, int nrp, double* Y,
+                               hipStream_t s) {
+#define PECS_MFMA_CASE(NN, MM)                                          \
+  if (nrp == NN && mb == MM) {                                          \
+    k_sweep_mfma<NN, MM, FWD><<<nt, 256, 0, s>>>(F, tl, Y);             \
+    return 0;                                                           \
   }
-  if (fwd) {
-    return nrhs == 2 ? sweep_dispatch<2, true>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s)
-                     : sweep_dispatch<1, true>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s);
-  }
-  return nrhs == 2 ? sweep_dispatch<2, false>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s)
-                   : sweep_dispatch<1, false>(F, tiles, nt, w, maxr, v0, v1, t0, t1, s);
+  PECS_MFMA_CASE(16, 1) PECS_MFMA_CASE(16, 2) PECS_MFMA_CASE(16, 4)
+  PECS_MFMA_CASE(32, 1) PECS_MFMA_CASE(32, 2) PECS_MFMA_CASE(32, 4)
+#undef PECS_MFMA_CASE
+  return (int)hipErrorInvalidValue;
 }
 
-int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, double* v1, double* t0, double* t1,
-                      int nrhs, hipStream_t s) {
-  if (nrhs != 2) {
-    v1 = v0;
-    t1 = t0;
+int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : 32; }
+
+int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
+                 double* t, hipStream_t s) {
+  if (nt <= 0) return 0;
+  if (w == 0) {
+    if (nrhs > 32) return (int)hipErrorInvalidValue;
+    const int nrp = mfma_rhs_pad(nrhs);
+    return fwd ? sweep_mfma_dispatch<true>(F, tiles, nt, maxr, nrp, t, s)
+               : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, t, s);
   }
+  if (nrhs > 2) return (int)hipErrorInvalidValue;
+  const double* v0 = nullptr;
+  double* t1 = nrhs == 2 ? t + F.n : t;
+  if (fwd) {
+    return nrhs == 2 ? sweep_dispatch<2, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s)
+                     : sweep_dispatch<1, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s);
+  }
+  return nrhs == 2 ? sweep_dispatch<2, false>(F, tiles, nt, w, maxr, v0, v0, t, t1, s)
+                   : sweep_dispatch<1, false>(F, tiles, nt, w, maxr, v0, v0, t, t1, s);
+}
+
+template <int NRP>
+static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s) {
   for (int l = 0; l < L.nlevels; ++l) {
     const int ng = L.goff[l + 1] - L.goff[l];
-    if (ng > 0) {
-      if (nrhs == 2) k_fwd_gather<2><<<nblk(ng, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v0, v1);
-      else k_fwd_gather<1><<<nblk(ng, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v0, v1);
-    }
+    if (ng > 0) k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v, nrhs);
     const int nt = L.foff[l + 1] - L.foff[l];
-    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, v0, v1, t0, t1, s);
+    if (L.fw[l] != 0) return (int)hipErrorInvalidValue;
+    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, 0, L.fm[l], true, nrhs, t, s);
     if (e) return e;
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
     const int nx = L.xoff[l + 1] - L.xoff[l];
-    if (nx > 0) {
-      if (nrhs == 2) k_bwd_gather<2><<<nblk(nx, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
-      else k_bwd_gather<1><<<nblk(nx, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
-    }
+    if (nx > 0) k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     const int nt = L.boff[l + 1] - L.boff[l];
-    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, v0, v1, t0, t1, s);
+    if (L.bw[l] != 0) return (int)hipErrorInvalidValue;
+    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, 0, L.bm[l], false, nrhs, t, s);
     if (e) return e;
   }
-  hipError_t e = hipMemcpyAsync(v0, F.X, (size_t)F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess && nrhs == 2)
-    e = hipMemcpyAsync(v1, F.X + F.n, (size_t)F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
+  k_unpack_il<NRP><<<nblk((int64_t)F.n * NRP, 256), 256, 0, s>>>(F, v, nrhs);
+  return (int)hipGetLastError();
+}
+
+int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s) {
+  if (L.nlevels > 0 && L.fw[0] == 0) {  // MFMA-shaped factor: interleaved vectors
+    if (nrhs > 32) return (int)hipErrorInvalidValue;
+    return mfma_rhs_pad(nrhs) == 16 ? chol_solve_il<16>(F, L, v, t, nrhs, s) : chol_solve_il<32>(F, L, v, t, nrhs, s);
+  }
+  if (nrhs > 2) return (int)hipErrorInvalidValue;
+  for (int l = 0; l < L.nlevels; ++l) {
+    const int ng = L.goff[l + 1] - L.goff[l];
+    if (ng > 0) k_fwd_gather<<<dim3(nblk(ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
+    const int nt = L.foff[l + 1] - L.foff[l];
+    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s);
+    if (e) return e;
+  }
+  for (int l = L.nlevels - 1; l >= 0; --l) {
+    const int nx = L.xoff[l + 1] - L.xoff[l];
+    if (nx > 0) k_bwd_gather<<<dim3(nblk(nx, 256), nrhs), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
+    const int nt = L.boff[l + 1] - L.boff[l];
+    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s);
+    if (e) return e;
+  }
+  hipError_t e = hipMemcpyAsync(v, F.X, (size_t)nrhs * F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return (int)e;
   return (int)hipGetLastError();
 }

@@ -29,7 +29,7 @@ struct DevPoisson {
   const double* V;
   const int* idx;        // nloc global DoFs per cell
   const int* type;
-  const double* table;   // per type: Ainv | fpsi
+  const double* table;   // per type: Ainv | U0 | U1 | U2 (unit Dirichlet data of phi_bi, phi_app, phi_sch)
   int stride;
   const int* pbid;       // 4 per cell
   const int* pI;
@@ -46,6 +46,7 @@ struct DevPoisson {
 struct DevPhys {
   double dt, penalty, k_et, k_ht, v_rec_n, v_rec_p;
   double ne, pe, rinf, oinf, eps_s, eps_e, G0, alpha, H, ni, tau_n, tau_p;
+  double phi_bi, phi_app, phi_sch;
   int gen_mode, srh;
 };
 
@@ -76,6 +77,14 @@ struct DevFactor {
   const int4* ftiles;       // forward row tiles (node, first row, rows, 0), grouped by level
   const int4* btiles;       // backward row tiles, grouped by level
   const int64_t* xidx;      // panel-row positions (into rows) of each level's nodes, grouped by level
+  // MFMA-blocked factor copies (batched contexts; fwd / bwd are then null):
+  // per node blocks of 16 rows in MFMA A-operand lane order (pack_blocked)
+  const double* fblk;
+  const double* bblk;
+  const int64_t* fblk_off;  // per block: data offset (one extra entry at the end)
+  const int64_t* bblk_off;
+  const int* fblk0;         // per node: first block index
+  const int* bblk0;
   double* Z;                // 2 * zlen scratch: forward updates, then backward gathered x
   double* X;                // 2 * n scratch: forward t, then backward output
 };
@@ -102,13 +111,22 @@ int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* 
 int launch_schur_recover(int k, const DevRegion& R, const double* W1, const double* W2, const double* xs1,
                          const double* xs2, double mu1, double mu2, double* out1, double* out2, const double* b1,
                          const double* b2, int nrhs, hipStream_t s);
-// One level's sweep over `nt` row tiles with W lanes per row and `maxr` row
-// passes per tile (W in {8,16,32,64}, maxr in {1,2,4}; every tile has at most
-// (256 / W) * maxr rows).
+// One level's sweep over `nt` row tiles.  w > 0: W = w lanes per row and
+// `maxr` row passes per tile (W in {8,16,32,64} with maxr in {1,2,4}, or W in
+// {32,64} with maxr 8; every tile has at most (256 / W) * maxr rows), for 1 or
+// 2 right-hand sides.  w == 0: the FP64-MFMA sweep with maxr (1, 2 or 4)
+// 16-row blocks per wave (tiles of at most 64 * maxr rows), for up to 32
+// right-hand sides.
+// Vectors are h-major: right-hand side h at t[h * n].
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 const double* v0, const double* v1, double* t0, double* t1, hipStream_t s);
-int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v0, double* v1, double* tmp0,
-                      double* tmp1, int nrhs, hipStream_t s);
+                 double* t, hipStream_t s);
+// Padded right-hand-side count of the MFMA sweep (16 or 32).
+int mfma_rhs_pad(int nrhs);
+// x = S^{-1} v for nrhs h-major right-hand sides in the factor's permuted
+// order (v is overwritten with x).  t is scratch: nrhs * n doubles for a
+// factor shaped for 1-2 right-hand sides, mfma_rhs_pad(nrhs) * n for an
+// MFMA-shaped one (whose scratch X, Z and t hold RHS-interleaved vectors).
+int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s);
 int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
                         const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
                         hipStream_t s);

@@ -305,7 +305,7 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
   const int nc = m.ncellP;
   O.Ainv.assign((size_t)nc * nl * nl, 0);
   O.A.assign((size_t)nc * nl * nl, 0);
-  O.fpsi.assign((size_t)nc * nrt, 0);
+  O.ubias.assign((size_t)nc * 3 * nrt, 0);
   std::vector<double> psx(nrt), psy(nrt);
   for (int K = 0; K < nc; ++K) {
     const double* v = &m.pV[8 * K];
@@ -342,16 +342,22 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
           face_point(f, T.fs[q], xi, eta);
           Geo g = bilinear(v, xi, eta);
           const double jw = T.fw[q] * len;
-          const double bi = (g.x == 0.0) ? P.phi_bi : 0.0;  // BiasValues.cpp:11-27
-          double other;
-          if (bid == DIRICHLET) other = (g.x == 0.0) ? P.phi_app : 0.0;  // BiasValues.cpp:74-91
-          else other = (P.schottky_location_set && g.y == P.schottky_location) ? P.phi_sch : 0.0;
-          const double datum = bi - other;
-          if (datum == 0.0) continue;
+          // datum = bi - other (BiasValues.cpp:11-27, 74-91): bi = phi_bi on
+          // x = 0; other = phi_app on Dirichlet points with x = 0, phi_sch on
+          // Schottky points at the Schottky location
+          const double wbi = (g.x == 0.0) ? 1.0 : 0.0;
+          double wapp = 0.0, wsch = 0.0;
+          if (bid == DIRICHLET) wapp = (g.x == 0.0) ? 1.0 : 0.0;
+          else wsch = (P.schottky_location_set && g.y == P.schottky_location) ? 1.0 : 0.0;
+          if (wbi == 0.0 && wapp == 0.0 && wsch == 0.0) continue;
           for (int i = 0; i < nrt; ++i) {
             const double hx = T.fpsix[(f * nrt + i) * nfq + q], hy = T.fpsiy[(f * nrt + i) * nfq + q];
             const double px = (g.J00 * hx + g.J01 * hy) / g.det, py = (g.J10 * hx + g.J11 * hy) / g.det;
-            O.fpsi[(size_t)K * nrt + i] += -(px * nx + py * ny) * datum * jw;
+            const double unit = -(px * nx + py * ny) * jw;
+            double* U = &O.ubias[(size_t)K * 3 * nrt];
+            U[i] += unit * wbi;
+            U[nrt + i] += unit * wapp;
+            U[2 * nrt + i] += unit * wsch;
           }
         }
       }
@@ -406,11 +412,12 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
     for (int f = 0; f < 4; ++f)
       if (m.pbid[4 * K + f] == NEUMANN)
         for (int mm = 0; mm < kf; ++mm) O.constrained[Lay.idx[(size_t)K * nl + T.rt_face_dof(f, mm)]] = 1;
-  O.type_stride = nl * nl + nrt;
+  O.type_stride = nl * nl + 3 * nrt;
   std::vector<double> rec((size_t)nc * O.type_stride);
   for (int K = 0; K < nc; ++K) {
     std::copy(&O.Ainv[(size_t)K * nl * nl], &O.Ainv[(size_t)(K + 1) * nl * nl], &rec[(size_t)K * O.type_stride]);
-    std::copy(&O.fpsi[(size_t)K * nrt], &O.fpsi[(size_t)(K + 1) * nrt], &rec[(size_t)K * O.type_stride + nl * nl]);
+    std::copy(&O.ubias[(size_t)K * 3 * nrt], &O.ubias[(size_t)(K + 1) * 3 * nrt],
+              &rec[(size_t)K * O.type_stride + nl * nl]);
   }
   dedup(rec, nc, O.type_stride, O.type, O.table, O.ntype);
   return O;

@@ -91,14 +91,19 @@ void ldg_csr(const Region& R, const LdgRegionOps& ops, double mu, double dt, std
 struct PoissonOps {
   int nloc = 0, nrt = 0, nphi = 0, k = 0;
   std::vector<double> Ainv;   // per cell nloc x nloc
-  std::vector<double> fpsi;   // per cell nrt (constant Dirichlet / Schottky part)
+  // constant Dirichlet / Schottky data of the D rows, split by the bias that
+  // multiplies it (BiasValues.cpp): per cell 3 x nrt, the unit-datum parts of
+  // phi_bi (points on x = 0), phi_app (Dirichlet points on x = 0) and phi_sch
+  // (Schottky points at the Schottky location); the right-hand side is
+  // phi_bi U0 - phi_app U1 - phi_sch U2, so the operators hold no bias value
+  std::vector<double> ubias;
   std::vector<double> A;      // per cell nloc x nloc (kept for CSR export)
   // multipliers
   std::vector<int> mult_face;  // per cell, 4 faces: multiplier index base (global, unpermuted) or -1
   int nmult = 0;
   std::vector<int> ndof_owner;      // per grid cell: number of owned multiplier DoFs
   std::vector<int> owner_base;      // per grid cell: first owned multiplier index
-  // de-duplicated device table: per type Ainv | fpsi
+  // de-duplicated device table: per type Ainv | U0 | U1 | U2
   std::vector<int> type;
   std::vector<double> table;
   int ntype = 0, type_stride = 0;

@@ -114,6 +114,151 @@ static void packed_solve_host(const PackedFactor& P, const double* b, double* x)
   for (int i = 0; i < P.n; ++i) x[i] = X[i];
 }
 
+// Host emulation of the FP64-MFMA solve (k_fwd_gather_il, k_sweep_mfma,
+// k_bwd_gather_il, k_unpack_il) lane by lane: the same tiles, the
+// RHS-interleaved vectors ([p * NRP + h]), the same per-lane loads and
+// v_mfma_f64_16x16x4f64 operand / result layouts (A: lane l = row l%16, k
+// l/16; B: lane l = k l/16, column l%16; D: lane l register r = row l/16 + 4r,
+// column l%16), with every global index checked against its array.  nrhs
+// h-major right-hand sides in b, h-major solutions in x.
+#define EMU_CHECK(cond, what)                                              \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      std::fprintf(stderr, "mfma emulation: out of bounds: %s\n", what);  \
+      std::exit(3);                                                        \
+    }                                                                      \
+  } while (0)
+
+static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, const std::vector<double>& b,
+                               std::vector<double>& x) {
+  const int NRP = nrhs <= 16 ? 16 : 32, NH = NRP / 16;
+  const int64_t n = P.n, zl = P.zlen;
+  std::vector<double> X(NRP * n, 0.0), Y(NRP * n, 0.0), Z(NRP * std::max<int64_t>(zl, 1), 0.0);
+  std::vector<int> tl;
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool FWD = dir == 0;
+    for (int li = 0; li < P.nlevels; ++li) {
+      const int l = FWD ? li : P.nlevels - 1 - li;
+      if (FWD) {
+        for (int i = P.goff[l]; i < P.goff[l + 1]; ++i)
+          for (int h = 0; h < NRP; ++h) {
+            const int p = P.lrows[i];
+            double a = 0.0;
+            if (h < nrhs) {
+              a = b[h * n + p];
+              for (int64_t c = P.cptr[p]; c < P.cptr[p + 1]; ++c) {
+                EMU_CHECK(P.czidx[c] >= 0 && P.czidx[c] < zl, "gathered z");
+                a -= Z[P.czidx[c] * NRP + h];
+              }
+            }
+            X[(int64_t)p * NRP + h] = a;
+          }
+      } else {
+        for (int i = P.xoff[l]; i < P.xoff[l + 1]; ++i)
+          for (int h = 0; h < NRP; ++h) {
+            const int64_t q = P.xidx[i];
+            EMU_CHECK(q >= 0 && q < zl, "backward gather");
+            Z[q * NRP + h] = X[(int64_t)P.rows[q] * NRP + h];
+          }
+      }
+      tl.clear();
+      level_tiles(P, l, FWD, 64 * MB, tl);
+      for (size_t t = 0; t < tl.size() / 4; ++t) {
+        const int s = tl[4 * t], row0 = tl[4 * t + 1], nr = tl[4 * t + 2];
+        EMU_CHECK(nr <= 64 * MB, "tile rows");
+        const int ns = P.ns[s], ms = P.ms[s], p0 = P.p0[s];
+        const int VL = FWD ? ns : ns + ms;
+        const std::vector<double>& blk = FWD ? P.fblk : P.bblk;
+        const std::vector<int64_t>& boffs = FWD ? P.fblk_off : P.bblk_off;
+        const int bnode = FWD ? P.fblk0[s] : P.bblk0[s], bnext = FWD ? P.fblk0[s + 1] : P.bblk0[s + 1];
+        auto vecv = [&](int col, int h) -> double {
+          if (col >= VL) return 0.0;
+          if (FWD) {
+            EMU_CHECK(p0 + col < n, "B operand X");
+            return X[(int64_t)(p0 + col) * NRP + h];
+          }
+          if (col < ns) return Y[(int64_t)(p0 + col) * NRP + h];
+          const int64_t q = P.rows_off[s] + (col - ns);
+          EMU_CHECK(q >= 0 && q < zl, "B operand Z");
+          return Z[q * NRP + h];
+        };
+        for (int wave = 0; wave < 4; ++wave) {
+          const int first = row0 + wave * MB * 16;
+          if (first >= row0 + nr) continue;
+          const int last = std::min(row0 + nr, first + MB * 16) - 1;
+          const int lo = FWD ? 0 : first, hi = FWD ? (last < ns ? last + 1 : ns) : ns + ms;
+          const int c0 = (lo >> 4) << 4, nstep = (hi - c0 + 15) >> 4;
+          EMU_CHECK(row0 % 16 == 0, "tile start on a 16-row block");
+          std::vector<double> C((size_t)MB * NH * 256, 0.0);
+          for (int st = 0; st < nstep; ++st) {
+            const int cc = c0 + 16 * st;
+            for (int q = 0; q < 4; ++q)
+              for (int mb = 0; mb < MB; ++mb) {
+                double A[16][4] = {}, Bm[NH][4][16] = {};
+                const int b = first / 16 + mb;
+                const bool exists = first + mb * 16 < row0 + nr;
+                int64_t bbase = 0;
+                int bclo = 0, bsteps = 0;
+                if (exists) {
+                  EMU_CHECK(bnode + b < bnext, "block of the node");
+                  bbase = boffs[bnode + b];
+                  bclo = FWD ? 0 : 16 * b;
+                  bsteps = (int)((boffs[bnode + b + 1] - bbase) >> 8);
+                }
+                const int idx = (cc - bclo) >> 4;
+                const bool in = exists && cc >= bclo && idx < bsteps;
+                for (int lane = 0; lane < 64; ++lane) {
+                  const int ri = lane & 15, kq = lane >> 4;
+                  const int col = cc + 4 * kq + q;
+                  double L = 0.0;
+                  if (in) {
+                    const int64_t e = bbase + ((int64_t)idx * 4 + q) * 64 + lane;
+                    EMU_CHECK(e >= 0 && e < (int64_t)blk.size(), "blocked factor entry");
+                    L = blk[e];
+                  }
+                  A[ri][kq] = L;
+                  for (int hg = 0; hg < NH; ++hg) Bm[hg][kq][ri] = vecv(col, hg * 16 + ri);
+                }
+                for (int hg = 0; hg < NH; ++hg) {
+                  double* Cm = &C[(size_t)(mb * NH + hg) * 256];
+                  for (int i = 0; i < 16; ++i)
+                    for (int j = 0; j < 16; ++j) {
+                      double v = 0;
+                      for (int k = 0; k < 4; ++k) v += A[i][k] * Bm[hg][k][j];
+                      Cm[i * 16 + j] += v;
+                    }
+                }
+              }
+          }
+          for (int mb = 0; mb < MB; ++mb)
+            for (int hg = 0; hg < NH; ++hg)
+              for (int lane = 0; lane < 64; ++lane)
+                for (int r = 0; r < 4; ++r) {
+                  const int ri = lane & 15, kq = lane >> 4;
+                  const int row = kq + 4 * r, lr = first + mb * 16 + row, h = hg * 16 + ri;
+                  if (lr >= row0 + nr) continue;
+                  const double v = C[(size_t)(mb * NH + hg) * 256 + row * 16 + ri];
+                  if (FWD) {
+                    if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
+                    else {
+                      const int64_t z = P.z_off[s] + (lr - ns);
+                      EMU_CHECK(z >= 0 && z < zl, "Z output");
+                      Z[z * NRP + h] = v;
+                    }
+                  } else {
+                    EMU_CHECK(p0 + lr < n, "X output");
+                    X[(int64_t)(p0 + lr) * NRP + h] = v;
+                  }
+                }
+        }
+      }
+    }
+  }
+  x.assign((size_t)nrhs * n, 0.0);
+  for (int64_t p = 0; p < n; ++p)
+    for (int h = 0; h < nrhs; ++h) x[h * n + p] = X[p * NRP + h];
+}
+
 static double check(const char* name, const BlockSPD& S, int leaf) {
   auto t0 = std::chrono::steady_clock::now();
   SupernodalFactor F = cholesky_factor(S, leaf);
@@ -140,8 +285,29 @@ static double check(const char* name, const BlockSPD& S, int leaf) {
     block_matvec(S, F.cell_base, xp.data(), rp.data());
     double nr2 = 0;
     for (int i = 0; i < F.n; ++i) nr2 += (rp[i] - b[i]) * (rp[i] - b[i]);
-    const double rel2 = std::sqrt(nr2 / nb);
+    double rel2 = std::sqrt(nr2 / nb);
     std::printf("   packed sweeps: resid=%.3e\n", rel2);
+    // MFMA sweep emulation, several right-hand sides (RHS h = b scaled by h+1)
+    PackedFactor PB = P;
+    pack_blocked(PB);
+    for (int nrhs : {3, 17}) {
+      for (int MB : {1, 2, 4}) {
+        std::vector<double> bb((size_t)nrhs * F.n), xm;
+        for (int h = 0; h < nrhs; ++h)
+          for (int i = 0; i < F.n; ++i) bb[(size_t)h * F.n + i] = (h + 1) * b[i];
+        mfma_solve_emulate(PB, nrhs, MB, bb, xm);
+        double e2 = 0, n2 = 0;
+        for (int h = 0; h < nrhs; ++h)
+          for (int i = 0; i < F.n; ++i) {
+            const double d = xm[(size_t)h * F.n + i] - (h + 1) * xs[i];
+            e2 += d * d;
+            n2 += (h + 1) * (h + 1) * xs[i] * xs[i];
+          }
+        const double relm = std::sqrt(e2 / n2);
+        std::printf("   mfma emulation nrhs=%d MB=%d: rel diff to host solve %.3e\n", nrhs, MB, relm);
+        rel2 = std::max(rel2, relm);
+      }
+    }
     if (std::getenv("PECS_LEVELS"))
       for (int l = 0; l < P.nlevels; ++l) {
         double fb = 0, bb = 0;
