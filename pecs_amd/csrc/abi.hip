@@ -27,14 +27,14 @@ thread_local std::string g_error;
 
 struct FactorDev {
   DevFactor d{};
-  std::vector<int> foff, boff, goff, xoff, fw, bw, fm, bm, frows, brows;  // per level (host)
+  std::vector<int> foff, boff, goff, xoff, fw, bw, fm, bm, frows, brows, fks, bks, froff, broff;  // per level
   int nlevels = 0, n = 0;
   double nnz = 0.0;
   double tri_bytes = 0.0;      // algorithmic bytes of a forward + backward sweep
   std::vector<int> cell_base;  // host copy: grid cell -> permuted base
   FactorLevels lv() const {
-    return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(),
-                        bw.data(),   fm.data(),   bm.data(),   nlevels};
+    return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(), bw.data(), fm.data(),
+                        bm.data(),   fks.data(),  bks.data(),  froff.data(), broff.data(), nlevels};
   }
 };
 
@@ -202,6 +202,9 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
   void* scrub = nullptr;
   double* Y = nullptr;
   int4* tbuf = nullptr;
+  int4* rtbuf = nullptr;
+  double* part = nullptr;
+  size_t part_cap = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   size_t tcap = 0;
   for (int l = 0; l < P.nlevels; ++l) {
@@ -215,36 +218,46 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
     if (scrub) (void)hipFree(scrub);
     if (Y) (void)hipFree(Y);
     if (tbuf) (void)hipFree(tbuf);
+    if (rtbuf) (void)hipFree(rtbuf);
+    if (part) (void)hipFree(part);
+    d.Part = nullptr;
   };
   try {
     HIPCHECK(hipMalloc(&scrub, scrub_bytes));
     const int nrs = nrhs > 2 ? mfma_rhs_pad(nrhs) : nrhs;  // scratch right-hand sides
     HIPCHECK(hipMalloc((void**)&Y, (size_t)nrs * P.n * sizeof(double)));
+    tcap *= 4;  // split tiles: at most 63 segments per row tile of >= 16 rows
     HIPCHECK(hipMalloc((void**)&tbuf, std::max<size_t>(tcap, 1) * sizeof(int4)));
+    HIPCHECK(hipMalloc((void**)&rtbuf, std::max<size_t>(tcap, 1) * sizeof(int4)));
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     hipStream_t s = c->stream;
     HIPCHECK(hipMemsetAsync(Y, 0, (size_t)nrs * P.n * sizeof(double), s));
     HIPCHECK(hipMemsetAsync(d.X, 0, (size_t)nrs * P.n * sizeof(double), s));
     HIPCHECK(hipMemsetAsync(d.Z, 0, (size_t)nrs * std::max<int64_t>(P.zlen, 1) * sizeof(double), s));
-    std::vector<int> tl;
-    std::vector<int4> t4;
+    std::vector<int> tl, rtl;
+    std::vector<int4> t4, r4;
+    P.fks.assign(P.nlevels, 0);
+    P.bks.assign(P.nlevels, 0);
     for (int l = 0; l < P.nlevels; ++l) {
       for (int dir = 0; dir < 2; ++dir) {
         const bool fwd = dir == 0;
         float best = 1e30f;
         int bw = fwd ? P.fw[l] : P.bw[l], bm = fwd ? P.fm[l] : P.bm[l], br = fwd ? P.frows[l] : P.brows[l];
+        int bk = 0;
         if (nrhs > 2) {
           bw = 0;
           bm = 1;
           br = 64;
         }
-        // candidates: (w, m, rows); w = 0 is the MFMA sweep (m 16-row blocks
-        // per wave, rows <= 64 m), used for more than 2 right-hand sides
-        std::vector<std::array<int, 3>> cand;
+        // candidates: (w, m, rows, ks); w = 0 is the MFMA sweep (m 16-row
+        // blocks per wave, rows <= 64 m, column segments of ks), used for more
+        // than 2 right-hand sides
+        std::vector<std::array<int, 4>> cand;
         if (nrhs > 2) {
           for (int m : {1, 2, 4})
-            for (int rows : {64 * m, 32 * m}) cand.push_back({0, m, rows});
+            for (int rows : {64 * m, 32 * m})
+              for (int ks : {0, 4096, 2048, 1024, 512, 256}) cand.push_back({0, m, rows, ks});
         } else {
           for (int w : {8, 16, 32, 64})
             for (int m : {1, 2, 4, 8}) {
@@ -252,24 +265,41 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
               for (int half = 0; half < 2; ++half) {
                 const int R = 256 / w;
                 const int rows = (R * m) >> half;
-                if (rows >= R) cand.push_back({w, m, rows});
+                if (rows >= R) cand.push_back({w, m, rows, 0});
               }
             }
         }
         for (const auto& cd : cand) {
-          const int w = cd[0], m = cd[1], rows = cd[2];
+          const int w = cd[0], m = cd[1], rows = cd[2], ks = cd[3];
           tl.clear();
-          level_tiles(P, l, fwd, rows, tl);
-          const int nt = (int)(tl.size() / 4);
+          rtl.clear();
+          if (!level_tiles_split(P, l, fwd, rows, ks, tl, rtl)) continue;
+          const int nt = (int)(tl.size() / 4), nrt = (int)(rtl.size() / 4);
           if (nt == 0) continue;
+          if (ks > 0 && nt == nrt) continue;  // no row tile is split: same as ks = 0
+          if ((size_t)nt > tcap) continue;
           t4.resize(nt);
-          for (int t = 0; t < nt; ++t) t4[t] = make_int4(tl[4 * t], tl[4 * t + 1], tl[4 * t + 2], 0);
+          for (int t = 0; t < nt; ++t) t4[t] = make_int4(tl[4 * t], tl[4 * t + 1], tl[4 * t + 2], tl[4 * t + 3]);
           HIPCHECK(hipMemcpyAsync(tbuf, t4.data(), nt * sizeof(int4), hipMemcpyHostToDevice, s));
+          if (ks > 0) {
+            const size_t need = (size_t)nt * 64 * m * nrs;
+            if (need > part_cap) {
+              HIPCHECK(hipStreamSynchronize(s));
+              if (part) HIPCHECK(hipFree(part));
+              part = nullptr;
+              HIPCHECK(hipMalloc((void**)&part, need * sizeof(double)));
+              part_cap = need;
+            }
+            d.Part = part;
+            r4.resize(nrt);
+            for (int t = 0; t < nrt; ++t) r4[t] = make_int4(rtl[4 * t], rtl[4 * t + 1], rtl[4 * t + 2], rtl[4 * t + 3]);
+            HIPCHECK(hipMemcpyAsync(rtbuf, r4.data(), nrt * sizeof(int4), hipMemcpyHostToDevice, s));
+          }
           float tmin = 1e30f;
           for (int rep = 0; rep < 3; ++rep) {
             HIPCHECK(hipMemsetAsync(scrub, rep, scrub_bytes, s));
             HIPCHECK(hipEventRecord(e0, s));
-            HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, s));
+            HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, s, ks, rtbuf, nrt));
             HIPCHECK(hipEventRecord(e1, s));
             HIPCHECK(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -281,16 +311,19 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
             bw = w;
             bm = m;
             br = rows;
+            bk = ks;
           }
         }
         if (fwd) {
           P.fw[l] = bw;
           P.fm[l] = bm;
           P.frows[l] = br;
+          P.fks[l] = bk;
         } else {
           P.bw[l] = bw;
           P.bm[l] = bm;
           P.brows[l] = br;
+          P.bks[l] = bk;
         }
       }
     }
@@ -361,13 +394,23 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   HIPCHECK(hipMemset(d.Z, 0, (size_t)nr * std::max<int64_t>(P.zlen, 1) * sizeof(double)));
   HIPCHECK(hipMemset(d.X, 0, (size_t)nr * P.n * sizeof(double)));
   tune_factor(c, P, d, nrhs);
-  std::vector<int4> ft(P.ftiles.size() / 4), bt(P.btiles.size() / 4);
-  for (size_t t = 0; t < ft.size(); ++t)
-    ft[t] = make_int4(P.ftiles[4 * t], P.ftiles[4 * t + 1], P.ftiles[4 * t + 2], 0);
-  for (size_t t = 0; t < bt.size(); ++t)
-    bt[t] = make_int4(P.btiles[4 * t], P.btiles[4 * t + 1], P.btiles[4 * t + 2], 0);
-  d.ftiles = upload(c, ft);
-  d.btiles = upload(c, bt);
+  auto to4 = [](const std::vector<int>& v) {
+    std::vector<int4> o(v.size() / 4);
+    for (size_t t = 0; t < o.size(); ++t) o[t] = make_int4(v[4 * t], v[4 * t + 1], v[4 * t + 2], v[4 * t + 3]);
+    return o;
+  };
+  d.ftiles = upload(c, to4(P.ftiles));
+  d.btiles = upload(c, to4(P.btiles));
+  d.frtiles = upload(c, to4(P.frtiles));
+  d.brtiles = upload(c, to4(P.brtiles));
+  // partial sums of the split levels: one slot of 64 MAXR rows per tile
+  size_t part = 0;
+  const int nrs = nrhs > 2 ? mfma_rhs_pad(nrhs) : nrhs;
+  for (int l = 0; l < P.nlevels; ++l) {
+    if (P.fks[l] > 0) part = std::max(part, (size_t)(P.foff[l + 1] - P.foff[l]) * 64 * P.fm[l] * nrs);
+    if (P.bks[l] > 0) part = std::max(part, (size_t)(P.boff[l + 1] - P.boff[l]) * 64 * P.bm[l] * nrs);
+  }
+  d.Part = part > 0 ? dalloc<double>(c, part) : nullptr;
   D.foff = P.foff;
   D.boff = P.boff;
   D.fw = P.fw;
@@ -376,6 +419,10 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   D.bm = P.bm;
   D.frows = P.frows;
   D.brows = P.brows;
+  D.fks = P.fks;
+  D.bks = P.bks;
+  D.froff = P.froff;
+  D.broff = P.broff;
   return D;
 }
 

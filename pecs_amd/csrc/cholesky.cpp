@@ -572,28 +572,69 @@ void level_tiles(const PackedFactor& P, int l, bool fwd, int rows, std::vector<i
   for (int q = P.lnoff[l]; q < P.lnoff[l + 1]; ++q) {
     const int s = P.lnodes[q];
     const int nr = fwd ? P.ns[s] + P.ms[s] : P.ns[s];
-    for (int r0 = 0; r0 < nr; r0 += rows) out.insert(out.end(), {s, r0, std::min(rows, nr - r0), 0});
+    for (int r0 = 0; r0 < nr; r0 += rows) out.insert(out.end(), {s, r0, std::min(rows, nr - r0), -1});
   }
+}
+
+bool level_tiles_split(const PackedFactor& P, int l, bool fwd, int rows, int ks, std::vector<int>& tiles,
+                       std::vector<int>& rtiles) {
+  if (ks <= 0) {
+    level_tiles(P, l, fwd, rows, tiles);
+    return true;
+  }
+  if (ks % 16 != 0) throw std::runtime_error("level_tiles_split: segment length must be a multiple of 16");
+  const int t_base = (int)(tiles.size() / 4);
+  for (int q = P.lnoff[l]; q < P.lnoff[l + 1]; ++q) {
+    const int s = P.lnodes[q];
+    const int ns = P.ns[s], ms = P.ms[s];
+    const int nr = fwd ? ns + ms : ns;
+    for (int r0 = 0; r0 < nr; r0 += rows) {
+      const int n = std::min(rows, nr - r0);
+      const int last = r0 + n - 1;
+      // columns the row tile touches: forward [0, longest row), backward [r0, ns + ms)
+      const int cbeg = fwd ? 0 : r0;
+      const int cend = fwd ? (last < ns ? last + 1 : ns) : ns + ms;
+      const int k0 = cbeg / ks, k1 = (cend + ks - 1) / ks;
+      if (k1 - k0 > 63) return false;
+      const int t0 = (int)(tiles.size() / 4) - t_base;
+      for (int k = k0; k < k1; ++k) tiles.insert(tiles.end(), {s, r0, n, k});
+      rtiles.insert(rtiles.end(), {s, r0, n, 64 * t0 + (k1 - k0)});
+    }
+  }
+  return true;
 }
 
 void rebuild_tiles(PackedFactor& P) {
   P.ftiles.clear();
   P.btiles.clear();
+  P.frtiles.clear();
+  P.brtiles.clear();
   P.foff.assign(P.nlevels + 1, 0);
   P.boff.assign(P.nlevels + 1, 0);
+  P.froff.assign(P.nlevels + 1, 0);
+  P.broff.assign(P.nlevels + 1, 0);
+  if ((int)P.fks.size() != P.nlevels) P.fks.assign(P.nlevels, 0);
+  if ((int)P.bks.size() != P.nlevels) P.bks.assign(P.nlevels, 0);
   for (int l = 0; l < P.nlevels; ++l) {
     // lanes per row W > 0: (256 / W) * MAXR rows per tile; W == 0 (the MFMA
-    // sweep): 64 * MAXR rows per tile
+    // sweep): 64 * MAXR rows per tile; only the MFMA sweep splits columns
     auto cap = [](int w, int m) { return w > 0 ? (256 / w) * m : 64 * m; };
     if (P.frows[l] > cap(P.fw[l], P.fm[l]) || P.brows[l] > cap(P.bw[l], P.bm[l]))
       throw std::runtime_error("rebuild_tiles: tile height exceeds the rows one workgroup covers");
+    if ((P.fks[l] && P.fw[l]) || (P.bks[l] && P.bw[l]))
+      throw std::runtime_error("rebuild_tiles: only the MFMA sweep splits columns");
     P.foff[l] = (int)(P.ftiles.size() / 4);
     P.boff[l] = (int)(P.btiles.size() / 4);
-    level_tiles(P, l, true, P.frows[l], P.ftiles);
-    level_tiles(P, l, false, P.brows[l], P.btiles);
+    P.froff[l] = (int)(P.frtiles.size() / 4);
+    P.broff[l] = (int)(P.brtiles.size() / 4);
+    if (!level_tiles_split(P, l, true, P.frows[l], P.fks[l], P.ftiles, P.frtiles) ||
+        !level_tiles_split(P, l, false, P.brows[l], P.bks[l], P.btiles, P.brtiles))
+      throw std::runtime_error("rebuild_tiles: too many column segments");
   }
   P.foff[P.nlevels] = (int)(P.ftiles.size() / 4);
   P.boff[P.nlevels] = (int)(P.btiles.size() / 4);
+  P.froff[P.nlevels] = (int)(P.frtiles.size() / 4);
+  P.broff[P.nlevels] = (int)(P.brtiles.size() / 4);
 }
 
 }  // namespace pecs

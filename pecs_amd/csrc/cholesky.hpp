@@ -69,6 +69,9 @@ struct PackedFactor {
   std::vector<int> fw, bw;            // lanes per row of the sweeps per level
   std::vector<int> fm, bm;            // row passes per tile per level
   std::vector<int> frows, brows;      // tile height per level (<= (256 / W) * MAXR)
+  std::vector<int> fks, bks;          // per level column-segment length of split tiles (0: no split)
+  std::vector<int> frtiles, brtiles;  // split levels: row tiles (node, row0, nrows, 64 * first tile + nseg)
+  std::vector<int> froff, broff;      // per level offsets into frtiles / brtiles
   std::vector<int> lnodes, lnoff;     // nodes of each level + offsets
   double tri_bytes = 0.0;             // bytes read by one forward + backward sweep
   // MFMA-blocked copies (see pack_blocked): per node, blocks of 16 rows; a
@@ -96,8 +99,15 @@ inline int blk_chi(bool fwd, int b, int ns, int ms) {
 }
 
 // Row tiles of level l (forward: all ns + ms rows of each node; backward: the
-// ns own rows) with `rows` rows per tile, appended as (node, row0, nrows, 0).
+// ns own rows) with `rows` rows per tile, appended as (node, row0, nrows, -1).
 void level_tiles(const PackedFactor& P, int l, bool fwd, int rows, std::vector<int>& out);
+// Split tiles: each row tile is cut into column segments of ks columns
+// (multiple of 16), one tile (node, row0, nrows, segment) per non-empty
+// segment, and one row tile (node, row0, nrows, 64 * first tile + nseg) per
+// row tile for the deterministic reduction of the segments' partial sums.
+// Returns false if a row tile would need more than 63 segments.
+bool level_tiles_split(const PackedFactor& P, int l, bool fwd, int rows, int ks, std::vector<int>& tiles,
+                       std::vector<int>& rtiles);
 // Rebuild ftiles/btiles and their level offsets from fw/fm/frows, bw/bm/brows.
 void rebuild_tiles(PackedFactor& P);
 

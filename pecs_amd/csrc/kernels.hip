@@ -612,9 +612,12 @@ __global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restri
   if (h < nrhs) v[(int64_t)h * F.n + p] = F.X[e];
 }
 
+// ks > 0: the tile is column segment tl.w (columns [tl.w ks, (tl.w + 1) ks))
+// of its row tile and writes its partial sums to Part (slot blockIdx.x, rows
+// relative to row0, 64 MB rows per slot); k_sweep_reduce adds the segments.
 template <int NRP, int MB, bool FWD>
 __global__ __launch_bounds__(256) void k_sweep_mfma(DevFactor F, const int4* __restrict__ tiles,
-                                                    double* __restrict__ Y) {
+                                                    double* __restrict__ Y, int ks, double* __restrict__ Part) {
   constexpr int NH = NRP / 16;
   const int4 tl = tiles[blockIdx.x];
   const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
@@ -657,8 +660,12 @@ __global__ __launch_bounds__(256) void k_sweep_mfma(DevFactor F, const int4* __r
     if (col < ns) return Y[(int64_t)(p0 + col) * NRP + h];
     return F.Z[(F.rows_off[s] + (col - ns)) * NRP + h];
   };
-  const int c0 = (lo >> 4) << 4;
-  const int nstep = (hi - c0 + 15) >> 4;
+  int c0 = (lo >> 4) << 4, cend = hi;
+  if (ks > 0) {  // this tile's column segment (ks and the segment start are multiples of 16)
+    c0 = max(c0, tl.w * ks);
+    cend = min(cend, (tl.w + 1) * ks);
+  }
+  const int nstep = cend > c0 ? (cend - c0 + 15) >> 4 : 0;
   auto load = [&](int st, double (&L)[MB][4], double (&Bv)[4][NH]) {
     const int cc = c0 + 16 * st;
 #pragma unroll
@@ -704,13 +711,38 @@ __global__ __launch_bounds__(256) void k_sweep_mfma(DevFactor F, const int4* __r
         const int h = hg * 16 + ri;
         if (lr >= row0 + nr) continue;
         const double v = acc[mb][hg][r];
-        if (FWD) {
+        if (ks > 0) {
+          Part[((int64_t)blockIdx.x * (64 * MB) + (lr - row0)) * NRP + h] = v;
+        } else if (FWD) {
           if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
           else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
         } else {
           F.X[(int64_t)(p0 + lr) * NRP + h] = v;
         }
       }
+}
+
+// Sum of the column segments of split row tiles, in segment order: row tile
+// rt = (node, row0, nrows, 64 * first tile + nseg); partial sums of tile t at
+// Part[(t * cap + r) * NRP + h] (cap = 64 MB rows per slot).
+template <int NRP, bool FWD>
+__global__ __launch_bounds__(256) void k_sweep_reduce(DevFactor F, const int4* __restrict__ rtiles, int cap,
+                                                      const double* __restrict__ Part, double* __restrict__ Y) {
+  const int4 rt = rtiles[blockIdx.x];
+  const int s = rt.x, row0 = rt.y, nr = rt.z, t0 = rt.w >> 6, nseg = rt.w & 63;
+  const int ns = F.ns[s], p0 = F.p0[s];
+  for (int e = threadIdx.x; e < nr * NRP; e += blockDim.x) {
+    const int r = e / NRP, h = e % NRP;
+    double v = 0.0;
+    for (int k = 0; k < nseg; ++k) v += Part[((int64_t)(t0 + k) * cap + r) * NRP + h];
+    const int lr = row0 + r;
+    if (FWD) {
+      if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
+      else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
+    } else {
+      F.X[(int64_t)(p0 + lr) * NRP + h] = v;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -1003,12 +1035,14 @@ static int sweep_dispatch(const DevFactor& F, const int4* tl, int nt, int w, int
 }
 
 template <bool FWD>
-static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int mb, int nrp, double* Y,
-                               hipStream_t s) {
-#define PECS_MFMA_CASE(NN, MM)                                          \
-  if (nrp == NN && mb == MM) {                                          \
-    k_sweep_mfma<NN, MM, FWD><<<nt, 256, 0, s>>>(F, tl, Y);             \
-    return 0;                                                           \
+static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int mb, int nrp, double* Y, int ks,
+                               const int4* rtl, int nrt, hipStream_t s) {
+  if (ks > 0 && (!rtl || !F.Part)) return (int)hipErrorInvalidValue;
+#define PECS_MFMA_CASE(NN, MM)                                                              \
+  if (nrp == NN && mb == MM) {                                                              \
+    k_sweep_mfma<NN, MM, FWD><<<nt, 256, 0, s>>>(F, tl, Y, ks, F.Part);                     \
+    if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<nrt, 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
+    return 0;                                                                               \
   }
   PECS_MFMA_CASE(16, 1) PECS_MFMA_CASE(16, 2) PECS_MFMA_CASE(16, 4)
   PECS_MFMA_CASE(32, 1) PECS_MFMA_CASE(32, 2) PECS_MFMA_CASE(32, 4)
@@ -1019,15 +1053,15 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
 int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : 32; }
 
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 double* t, hipStream_t s) {
+                 double* t, hipStream_t s, int ks, const int4* rtiles, int nrt) {
   if (nt <= 0) return 0;
   if (w == 0) {
     if (nrhs > 32) return (int)hipErrorInvalidValue;
     const int nrp = mfma_rhs_pad(nrhs);
-    return fwd ? sweep_mfma_dispatch<true>(F, tiles, nt, maxr, nrp, t, s)
-               : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, t, s);
+    return fwd ? sweep_mfma_dispatch<true>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s)
+               : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s);
   }
-  if (nrhs > 2) return (int)hipErrorInvalidValue;
+  if (nrhs > 2 || ks > 0) return (int)hipErrorInvalidValue;
   const double* v0 = nullptr;
   double* t1 = nrhs == 2 ? t + F.n : t;
   if (fwd) {
@@ -1045,7 +1079,8 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
     if (ng > 0) k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v, nrhs);
     const int nt = L.foff[l + 1] - L.foff[l];
     if (L.fw[l] != 0) return (int)hipErrorInvalidValue;
-    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, 0, L.fm[l], true, nrhs, t, s);
+    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, 0, L.fm[l], true, nrhs, t, s, L.fks[l],
+                               F.frtiles + L.froff[l], L.froff[l + 1] - L.froff[l]);
     if (e) return e;
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
@@ -1053,7 +1088,8 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
     if (nx > 0) k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     const int nt = L.boff[l + 1] - L.boff[l];
     if (L.bw[l] != 0) return (int)hipErrorInvalidValue;
-    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, 0, L.bm[l], false, nrhs, t, s);
+    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, 0, L.bm[l], false, nrhs, t, s, L.bks[l],
+                               F.brtiles + L.broff[l], L.broff[l + 1] - L.broff[l]);
     if (e) return e;
   }
   k_unpack_il<NRP><<<nblk((int64_t)F.n * NRP, 256), 256, 0, s>>>(F, v, nrhs);

@@ -85,6 +85,9 @@ struct DevFactor {
   const int64_t* bblk_off;
   const int* fblk0;         // per node: first block index
   const int* bblk0;
+  const int4* frtiles;      // split row tiles of the MFMA sweeps, grouped by level
+  const int4* brtiles;
+  double* Part;             // partial sums of split tiles (MFMA sweeps)
   double* Z;                // 2 * zlen scratch: forward updates, then backward gathered x
   double* X;                // 2 * n scratch: forward t, then backward output
 };
@@ -98,6 +101,10 @@ struct FactorLevels {
   const int* bw;    // host: lanes per row of the backward sweep per level
   const int* fm;    // host: row passes per tile (tile rows <= (256 / fw) * fm) per level
   const int* bm;    // host: the same for the backward sweep
+  const int* fks;   // host: column-segment length of split forward tiles per level (0: no split)
+  const int* bks;   // host: the same for the backward sweep
+  const int* froff; // host: per level offsets into frtiles (nlevels+1)
+  const int* broff; // host: per level offsets into brtiles
   int nlevels;
 };
 
@@ -116,10 +123,11 @@ int launch_schur_recover(int k, const DevRegion& R, const double* W1, const doub
 // {32,64} with maxr 8; every tile has at most (256 / W) * maxr rows), for 1 or
 // 2 right-hand sides.  w == 0: the FP64-MFMA sweep with maxr (1, 2 or 4)
 // 16-row blocks per wave (tiles of at most 64 * maxr rows), for up to 32
-// right-hand sides.
+// right-hand sides; ks > 0 splits its tiles into column segments of ks
+// columns whose partial sums (F.Part) are added by a reduction over rtiles.
 // Vectors are h-major: right-hand side h at t[h * n].
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 double* t, hipStream_t s);
+                 double* t, hipStream_t s, int ks = 0, const int4* rtiles = nullptr, int nrt = 0);
 // Padded right-hand-side count of the MFMA sweep (16 or 32).
 int mfma_rhs_pad(int nrhs);
 // x = S^{-1} v for nrhs h-major right-hand sides in the factor's permuted

@@ -129,7 +129,7 @@ static void packed_solve_host(const PackedFactor& P, const double* b, double* x)
     }                                                                      \
   } while (0)
 
-static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, const std::vector<double>& b,
+static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, int KS, const std::vector<double>& b,
                                std::vector<double>& x) {
   const int NRP = nrhs <= 16 ? 16 : 32, NH = NRP / 16;
   const int64_t n = P.n, zl = P.zlen;
@@ -162,9 +162,15 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, const st
           }
       }
       tl.clear();
-      level_tiles(P, l, FWD, 64 * MB, tl);
+      std::vector<int> rtl;
+      if (!level_tiles_split(P, l, FWD, 64 * MB, KS, tl, rtl)) {
+        std::fprintf(stderr, "mfma emulation: too many segments\n");
+        std::exit(3);
+      }
+      const int cap = 64 * MB;
+      std::vector<double> part(KS > 0 ? tl.size() / 4 * cap * NRP : 0, 0.0);
       for (size_t t = 0; t < tl.size() / 4; ++t) {
-        const int s = tl[4 * t], row0 = tl[4 * t + 1], nr = tl[4 * t + 2];
+        const int s = tl[4 * t], row0 = tl[4 * t + 1], nr = tl[4 * t + 2], kseg = tl[4 * t + 3];
         EMU_CHECK(nr <= 64 * MB, "tile rows");
         const int ns = P.ns[s], ms = P.ms[s], p0 = P.p0[s];
         const int VL = FWD ? ns : ns + ms;
@@ -187,7 +193,13 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, const st
           if (first >= row0 + nr) continue;
           const int last = std::min(row0 + nr, first + MB * 16) - 1;
           const int lo = FWD ? 0 : first, hi = FWD ? (last < ns ? last + 1 : ns) : ns + ms;
-          const int c0 = (lo >> 4) << 4, nstep = (hi - c0 + 15) >> 4;
+          int c0 = (lo >> 4) << 4, cend = hi;
+          if (KS > 0) {
+            EMU_CHECK(kseg >= 0, "segment index");
+            c0 = std::max(c0, kseg * KS);
+            cend = std::min(cend, (kseg + 1) * KS);
+          }
+          const int nstep = cend > c0 ? (cend - c0 + 15) >> 4 : 0;
           EMU_CHECK(row0 % 16 == 0, "tile start on a 16-row block");
           std::vector<double> C((size_t)MB * NH * 256, 0.0);
           for (int st = 0; st < nstep; ++st) {
@@ -238,7 +250,11 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, const st
                   const int row = kq + 4 * r, lr = first + mb * 16 + row, h = hg * 16 + ri;
                   if (lr >= row0 + nr) continue;
                   const double v = C[(size_t)(mb * NH + hg) * 256 + row * 16 + ri];
-                  if (FWD) {
+                  if (KS > 0) {
+                    const size_t e = ((size_t)t * cap + (lr - row0)) * NRP + h;
+                    EMU_CHECK(e < part.size(), "partial slot");
+                    part[e] = v;
+                  } else if (FWD) {
                     if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
                     else {
                       const int64_t z = P.z_off[s] + (lr - ns);
@@ -251,6 +267,28 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, const st
                   }
                 }
         }
+      }
+      // reduction of split row tiles, segments in order (k_sweep_reduce)
+      for (size_t q = 0; q < rtl.size() / 4; ++q) {
+        const int s = rtl[4 * q], row0 = rtl[4 * q + 1], nr = rtl[4 * q + 2];
+        const int t0 = rtl[4 * q + 3] >> 6, nseg = rtl[4 * q + 3] & 63;
+        const int ns = P.ns[s], p0 = P.p0[s];
+        for (int r = 0; r < nr; ++r)
+          for (int h = 0; h < NRP; ++h) {
+            double v = 0.0;
+            for (int k = 0; k < nseg; ++k) {
+              const size_t e = ((size_t)(t0 + k) * cap + r) * NRP + h;
+              EMU_CHECK(e < part.size(), "reduce slot");
+              v += part[e];
+            }
+            const int lr = row0 + r;
+            if (FWD) {
+              if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
+              else Z[(P.z_off[s] + (lr - ns)) * NRP + h] = v;
+            } else {
+              X[(int64_t)(p0 + lr) * NRP + h] = v;
+            }
+          }
       }
     }
   }
@@ -291,11 +329,12 @@ static double check(const char* name, const BlockSPD& S, int leaf) {
     PackedFactor PB = P;
     pack_blocked(PB);
     for (int nrhs : {3, 17}) {
-      for (int MB : {1, 2, 4}) {
+      for (int MB : {1, 2, 4})
+      for (int KS : {0, 32, 256}) {
         std::vector<double> bb((size_t)nrhs * F.n), xm;
         for (int h = 0; h < nrhs; ++h)
           for (int i = 0; i < F.n; ++i) bb[(size_t)h * F.n + i] = (h + 1) * b[i];
-        mfma_solve_emulate(PB, nrhs, MB, bb, xm);
+        mfma_solve_emulate(PB, nrhs, MB, KS, bb, xm);
         double e2 = 0, n2 = 0;
         for (int h = 0; h < nrhs; ++h)
           for (int i = 0; i < F.n; ++i) {
@@ -304,7 +343,7 @@ static double check(const char* name, const BlockSPD& S, int leaf) {
             n2 += (h + 1) * (h + 1) * xs[i] * xs[i];
           }
         const double relm = std::sqrt(e2 / n2);
-        std::printf("   mfma emulation nrhs=%d MB=%d: rel diff to host solve %.3e\n", nrhs, MB, relm);
+        std::printf("   mfma emulation nrhs=%d MB=%d KS=%d: rel diff to host solve %.3e\n", nrhs, MB, KS, relm);
         rel2 = std::max(rel2, relm);
       }
     }
