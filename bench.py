@@ -19,6 +19,10 @@ the relative steady residual ||x^{n+1}-x^n|| / (||x^n|| dt) over the five
 state vectors falls to 1e-8 (the reference has no criterion of its own),
 on the measured workload and, beside the oracle on one core, on the small
 workload (refinement 4, k = 1, dt = 1e-3).
+
+batched: an I-V sweep of 16 applied biases on the measured mesh stepped
+together (one read of each factor per sweep for all of them, FP64 MFMA):
+aggregate step throughput and time to steady state per bias point.
 """
 import argparse
 import json
@@ -187,7 +191,7 @@ def main():
     ap.add_argument("--steady-max-steps", type=int, default=8000)
     ap.add_argument("--steady-check-every", type=int, default=25)
     ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady-state legs")
-    ap.add_argument("--batch", type=int, default=0, help="also run an I-V sweep of this many bias points")
+    ap.add_argument("--batch", type=int, default=16, help="also run an I-V sweep of this many bias points (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,

@@ -49,6 +49,7 @@ struct pecs_ctx {
   int B = 1;                 // instances sharing mesh, matrices and factors (a bias / illumination / doping batch)
   std::vector<Params> Pb;    // per instance
   std::vector<DevPhys> phs;  // per instance
+  DevPhys* phs_d = nullptr;  // the same on the device
   int k = 1;
   SolarCellMesh mesh;
   PoissonLayout L;
@@ -64,8 +65,7 @@ struct pecs_ctx {
   double* u[4] = {};
   double* pv = nullptr;
   double* b[4] = {};
-  double* W[4] = {};
-  int64_t Wsz[4] = {};
+
   // per region: the Schur right-hand sides / solutions of all carriers and
   // instances, h-major with h = carrier * B + instance (region factor order)
   double* g[2] = {};
@@ -456,11 +456,9 @@ void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
   const FactorDev& Fc = (car == 1 && !share) ? F[1] : F[0];
   const double mu = c->P.mu[which];
   hipStream_t s = c->stream;
-  double* W = c->W[which];
-  HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b, mu, W, s));
-  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b, b, W, W, c->g[reg], c->g[reg], 1, s));
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b, b, mu, mu, c->g[reg], c->g[reg], 1, s));
   HIPCHECK((hipError_t)launch_chol_solve(Fc.d, Fc.lv(), c->g[reg], c->t[reg], 1, s));
-  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, W, W, c->g[reg], c->g[reg], mu, mu, x, x, b, b, 1, s));
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, b, b, c->g[reg], c->g[reg], mu, mu, x, x, 1, s));
 }
 
 // Poisson solves of instances [i0, i0 + n) from their condensed cell data yP:
@@ -469,24 +467,26 @@ void solve_carrier(pecs_ctx* c, int which, const double* b, double* x) {
 void poisson_solve_from_y(pecs_ctx* c, int i0, int n, double* out) {
   hipStream_t s = c->stream;
   const int64_t nH = c->FH.n;
-  for (int i = i0; i < i0 + n; ++i)
-    HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP + i * c->ysz, c->rH + (i - i0) * nH, s));
+  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP + i0 * c->ysz, c->rH, s, n, c->ysz, nH));
   HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, n, s));
-  for (int i = i0; i < i0 + n; ++i)
-    HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH + (i - i0) * nH, c->yP + i * c->ysz,
-                                                out + (i - i0) * c->NP, s));
+  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP + i0 * c->ysz, out, s, n, nH, c->ysz, c->NP));
 }
 
 // Poisson condensation of instance i from its densities
 void poisson_cells(pecs_ctx* c, int i, const double* un, const double* up, const double* ur, const double* uo) {
-  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->phs[i], un, up, ur, uo, c->mesh.semi.ncell,
+  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->phs_d + i, un, up, ur, uo, c->mesh.semi.ncell,
                                            c->mesh.elec.ncell, c->yP + i * c->ysz, c->stream));
 }
 
 // the Poisson states of all instances from their current densities
+void poisson_all_cells(pecs_ctx* c) {
+  HIPCHECK((hipError_t)launch_poisson_cell(c->k, c->Pd, c->tabs, c->phs_d, c->u[0], c->u[1], c->u[2], c->u[3],
+                                           c->mesh.semi.ncell, c->mesh.elec.ncell, c->yP, c->stream, c->B, c->Ns,
+                                           c->Ne, c->ysz));
+}
+
 void poisson_all_from_densities(pecs_ctx* c) {
-  for (int i = 0; i < c->B; ++i)
-    poisson_cells(c, i, c->u[0] + i * c->Ns, c->u[1] + i * c->Ns, c->u[2] + i * c->Ne, c->u[3] + i * c->Ne);
+  poisson_all_cells(c);
   poisson_solve_from_y(c, 0, c->B, c->pv);
 }
 
@@ -516,17 +516,12 @@ inline int64_t gslot(const pecs_ctx* c, int reg, int car, int i) {
 void region_schur(pecs_ctx* c, int reg, hipStream_t s) {
   const DevRegion& R = reg == 0 ? c->Rs : c->Re;
   const int64_t N = reg == 0 ? c->Ns : c->Ne;
-  const int c0 = 2 * reg, c1 = 2 * reg + 1;
-  for (int i = 0; i < c->B; ++i) {
-    const double* b0 = c->b[c0] + i * N;
-    const double* b1 = c->b[c1] + i * N;
-    double* W0 = c->W[c0] + i * c->Wsz[c0];
-    double* W1 = c->W[c1] + i * c->Wsz[c1];
-    HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b0, c->P.mu[c0], W0, s));
-    HIPCHECK((hipError_t)launch_schur_prep(c->k, R, b1, c->P.mu[c1], W1, s));
-    HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, b0, b1, W0, W1, c->g[reg] + gslot(c, reg, 0, i),
-                                          c->g[reg] + gslot(c, reg, 1, i), 2, s));
-  }
+  const int64_t n = reg == 0 ? c->Fs[0].n : c->Fe[0].n;
+  const int c0 = 2 * reg, c1 = 2 * reg + 1, B = c->B;
+  // the instance stride of the g slots is n (slot = carrier * B + instance)
+  HIPCHECK((hipError_t)launch_schur_rhs(c->k, R, c->b[c0], c->b[c1], c->P.mu[c0], c->P.mu[c1],
+                                        c->g[reg] + gslot(c, reg, 0, 0), c->g[reg] + gslot(c, reg, 1, 0), 2, s, B, N,
+                                        n));
 }
 
 void region_chol(pecs_ctx* c, int reg, hipStream_t s) {
@@ -544,12 +539,11 @@ void region_chol(pecs_ctx* c, int reg, hipStream_t s) {
 void region_recover(pecs_ctx* c, int reg, hipStream_t s) {
   const DevRegion& R = reg == 0 ? c->Rs : c->Re;
   const int64_t N = reg == 0 ? c->Ns : c->Ne;
+  const int64_t n = reg == 0 ? c->Fs[0].n : c->Fe[0].n;
   const int c0 = 2 * reg, c1 = 2 * reg + 1;
-  for (int i = 0; i < c->B; ++i)
-    HIPCHECK((hipError_t)launch_schur_recover(
-        c->k, R, c->W[c0] + i * c->Wsz[c0], c->W[c1] + i * c->Wsz[c1], c->g[reg] + gslot(c, reg, 0, i),
-        c->g[reg] + gslot(c, reg, 1, i), c->P.mu[c0], c->P.mu[c1], c->u[c0] + i * N, c->u[c1] + i * N,
-        c->b[c0] + i * N, c->b[c1] + i * N, 2, s));
+  HIPCHECK((hipError_t)launch_schur_recover(c->k, R, c->b[c0], c->b[c1], c->g[reg] + gslot(c, reg, 0, 0),
+                                            c->g[reg] + gslot(c, reg, 1, 0), c->P.mu[c0], c->P.mu[c1], c->u[c0],
+                                            c->u[c1], 2, s, c->B, N, n, N));
 }
 
 void step_once(pecs_ctx* c) {
@@ -561,15 +555,12 @@ void step_once(pecs_ctx* c) {
     pm.mark();
   }
   const int64_t Ns = c->Ns, Ne = c->Ne, NP = c->NP;
-  for (int i = 0; i < c->B; ++i)
-    HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->phs[i], c->u[0] + i * Ns,
-                                            c->u[1] + i * Ns, c->u[2] + i * Ne, c->u[3] + i * Ne, c->pv + i * NP,
-                                            c->b[0] + i * Ns, c->b[1] + i * Ns, s));
+  const int B = c->B;
+  HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->phs_d, c->u[0], c->u[1],
+                                          c->u[2], c->u[3], c->pv, c->b[0], c->b[1], s, B, Ns, Ne, NP));
   pm.mark();
-  for (int i = 0; i < c->B; ++i)
-    HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->phs[i], c->u[2] + i * Ne,
-                                            c->u[3] + i * Ne, c->u[0] + i * Ns, c->u[1] + i * Ns, c->pv + i * NP,
-                                            c->b[2] + i * Ne, c->b[3] + i * Ne, s));
+  HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->phs_d, c->u[2], c->u[3],
+                                          c->u[0], c->u[1], c->pv, c->b[2], c->b[3], s, B, Ne, Ns, NP));
   pm.mark();
   region_schur(c, 0, s);
   region_schur(c, 1, s);
@@ -586,16 +577,13 @@ void step_once(pecs_ctx* c) {
   region_recover(c, 0, s);
   region_recover(c, 1, s);
   pm.mark();
-  for (int i = 0; i < c->B; ++i)
-    poisson_cells(c, i, c->u[0] + i * Ns, c->u[1] + i * Ns, c->u[2] + i * Ne, c->u[3] + i * Ne);
+  poisson_all_cells(c);
   pm.mark();
   const int64_t nH = c->FH.n;
-  for (int i = 0; i < c->B; ++i)
-    HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP + i * c->ysz, c->rH + i * nH, s));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, c->B, s));
+  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s, B, c->ysz, nH));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, B, s));
   pm.mark();
-  for (int i = 0; i < c->B; ++i)
-    HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH + i * nH, c->yP + i * c->ysz, c->pv + i * NP, s));
+  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s, B, nH, c->ysz, NP));
   pm.mark();
 }
 
@@ -773,13 +761,12 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     // physics, per instance
     for (int i = 0; i < B; ++i) c->phs.push_back(make_phys(c->Pb[i], m.geo.H));
     c->ph = c->phs[0];
+    c->phs_d = upload(c.get(), c->phs);
     // state and scratch (instance-major)
     const int64_t nsv[4] = {c->Ns, c->Ns, c->Ne, c->Ne};
     for (int i = 0; i < 4; ++i) {
-      c->Wsz[i] = 2LL * nb * (i < 2 ? m.semi.ncell : m.elec.ncell);
       c->u[i] = dalloc<double>(c.get(), B * nsv[i]);
       c->b[i] = dalloc<double>(c.get(), B * nsv[i]);
-      c->W[i] = dalloc<double>(c.get(), B * c->Wsz[i]);
       c->old[i] = dalloc<double>(c.get(), B * nsv[i]);
       c->io[i] = dalloc<double>(c.get(), std::max(nsv[i], c->NP));
       HIPCHECK(hipMemset(c->u[i], 0, B * nsv[i] * sizeof(double)));
@@ -958,7 +945,7 @@ static void d2h(pecs_ctx* c, double* h, const double* d, int64_t n) {
 int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* un, const double* up, const double* pv,
                                const double* ur, const double* uo, double* bn, double* bp) {
   PECS_TRY(ctx, {
-    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->ph, un, up, ur,
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->phs_d, un, up, ur,
                                             uo, pv, bn, bp, ctx->stream));
   })
 }
@@ -966,7 +953,7 @@ int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* un, const double* up
 int pecs_rhs_electrolyte_dev(pecs_ctx* ctx, const double* ur, const double* uo, const double* pv,
                              const double* un, const double* up, double* br, double* bo) {
   PECS_TRY(ctx, {
-    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->ph, ur, uo,
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->phs_d, ur, uo,
                                             un, up, pv, br, bo, ctx->stream));
   })
 }
@@ -975,7 +962,7 @@ int pecs_rhs_poisson_dev(pecs_ctx* ctx, const double* un, const double* up, cons
                          double* rhs) {
   PECS_TRY(ctx, {
     HIPCHECK(hipMemsetAsync(rhs, 0, ctx->NP * sizeof(double), ctx->stream));
-    HIPCHECK((hipError_t)launch_poisson_rhs_assemble(ctx->k, ctx->Pd, ctx->tabs, ctx->ph, un, up, ur, uo,
+    HIPCHECK((hipError_t)launch_poisson_rhs_assemble(ctx->k, ctx->Pd, ctx->tabs, ctx->phs_d, un, up, ur, uo,
                                                      ctx->mesh.semi.ncell, ctx->mesh.elec.ncell, rhs, ctx->stream));
   })
 }
@@ -1001,7 +988,7 @@ int pecs_rhs_semiconductor(pecs_ctx* ctx, const double* un, const double* up, co
     h2d(ctx, ctx->io[2], ur, ctx->Ne);
     h2d(ctx, ctx->io[3], uo, ctx->Ne);
     h2d(ctx, ctx->old[4], pv, ctx->NP);
-    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->ph, ctx->io[0],
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->phs_d, ctx->io[0],
                                             ctx->io[1], ctx->io[2], ctx->io[3], ctx->old[4], ctx->b[0], ctx->b[1],
                                             ctx->stream));
     d2h(ctx, bn, ctx->b[0], ctx->Ns);
@@ -1017,7 +1004,7 @@ int pecs_rhs_electrolyte(pecs_ctx* ctx, const double* ur, const double* uo, cons
     h2d(ctx, ctx->io[2], ur, ctx->Ne);
     h2d(ctx, ctx->io[3], uo, ctx->Ne);
     h2d(ctx, ctx->old[4], pv, ctx->NP);
-    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->ph, ctx->io[2],
+    HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->phs_d, ctx->io[2],
                                             ctx->io[3], ctx->io[0], ctx->io[1], ctx->old[4], ctx->b[2], ctx->b[3],
                                             ctx->stream));
     d2h(ctx, br, ctx->b[2], ctx->Ne);
@@ -1033,7 +1020,7 @@ int pecs_rhs_poisson(pecs_ctx* ctx, const double* un, const double* up, const do
     h2d(ctx, ctx->io[2], ur, ctx->Ne);
     h2d(ctx, ctx->io[3], uo, ctx->Ne);
     HIPCHECK(hipMemsetAsync(ctx->old[4], 0, ctx->NP * sizeof(double), ctx->stream));
-    HIPCHECK((hipError_t)launch_poisson_rhs_assemble(ctx->k, ctx->Pd, ctx->tabs, ctx->ph, ctx->io[0], ctx->io[1],
+    HIPCHECK((hipError_t)launch_poisson_rhs_assemble(ctx->k, ctx->Pd, ctx->tabs, ctx->phs_d, ctx->io[0], ctx->io[1],
                                                      ctx->io[2], ctx->io[3], ctx->mesh.semi.ncell,
                                                      ctx->mesh.elec.ncell, ctx->old[4], ctx->stream));
     d2h(ctx, rhs, ctx->old[4], ctx->NP);

@@ -5,8 +5,8 @@
 //                   cell, geometry rebuilt from the 4 vertices (MappingQ1),
 //                   old densities, Poisson field (Piola) and lagged traces of
 //                   the other region evaluated at QGauss(k+2) points.
-//   k_schur_prep    W = A^{-1} f_p (cell-local inverse mass, times mobility).
-//   k_schur_rhs     g = f_v + B^T W (cell + 4 face neighbours).
+//   k_schur_rhs     g = f_v + B^T W (cell + 4 face neighbours), with
+//                   W = mu A^{-1} f_p recomputed per cell from the RHS.
 //   k_fwd_gather / k_sweep / k_bwd_gather  supernodal triangular solves
 //                   with S = L L^T, level by level on the elimination tree:
 //                   row tiles of dense [Linv; L21 Linv] panels, vector chunks
@@ -83,15 +83,28 @@ __device__ __forceinline__ int rt_face(int f, int m) {
 // Carrier right-hand sides (SolarCell.cpp:1073-1414 and 1451-1726)
 // ------------------------------------------------------------------------
 template <int K, bool SEMI>
-__global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, DevPoisson P, DevTabs T, DevPhys ph,
-                                                    const double* __restrict__ u1, const double* __restrict__ u2,
-                                                    const double* __restrict__ ou1, const double* __restrict__ ou2,
-                                                    const double* __restrict__ pvec, double* __restrict__ b1,
-                                                    double* __restrict__ b2) {
+__global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, DevPoisson P, DevTabs T,
+                                                    const DevPhys* __restrict__ phs, const double* __restrict__ u1,
+                                                    const double* __restrict__ u2, const double* __restrict__ ou1,
+                                                    const double* __restrict__ ou2, const double* __restrict__ pvec,
+                                                    double* __restrict__ b1, double* __restrict__ b2, int64_t sR,
+                                                    int64_t sO, int64_t sP) {
   using S = Sz<K>;
   constexpr int NB = S::NB, NQ = S::NQ, NF = S::NF, NRT = S::NRT;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R.ncell) return;
+  // instance blockIdx.y of a batch: its physics and vectors
+  const DevPhys ph = phs[blockIdx.y];
+  {
+    const int64_t i = blockIdx.y;
+    u1 += i * sR;
+    u2 += i * sR;
+    b1 += i * sR;
+    b2 += i * sR;
+    ou1 += i * sO;
+    ou2 += i * sO;
+    pvec += i * sP;
+  }
   const int nc = R.ncell;
   double v[8];
 #pragma unroll
@@ -244,19 +257,19 @@ __global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, De
 // ------------------------------------------------------------------------
 // Schur complement transforms
 // ------------------------------------------------------------------------
+// W_M = mu A_M^{-1} f_p,M of cell M (the current part of carrier RHS b):
+// the cell-local inverse mass block times the cell's two current components.
 template <int K>
-__global__ __launch_bounds__(64) void k_schur_prep(DevRegion R, const double* __restrict__ b, double mu,
-                                                   double* __restrict__ W) {
+__device__ __forceinline__ void schur_w(const DevRegion& R, const double* __restrict__ b, double mu, int M,
+                                        double* w) {
   constexpr int NB = Sz<K>::NB;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= R.ncell) return;
   const size_t nc = R.ncell;
-  const double* Minv = R.table + (size_t)R.type[c] * R.stride;
+  const double* Minv = R.table + (size_t)R.type[M] * R.stride;
   double fx[NB], fy[NB];
 #pragma unroll
   for (int a = 0; a < NB; ++a) {
-    fx[a] = b[(size_t)c * NB + a];
-    fy[a] = b[nc * NB + (size_t)c * NB + a];
+    fx[a] = b[(size_t)M * NB + a];
+    fy[a] = b[nc * NB + (size_t)M * NB + a];
   }
 #pragma unroll
   for (int a = 0; a < NB; ++a) {
@@ -267,19 +280,28 @@ __global__ __launch_bounds__(64) void k_schur_prep(DevRegion R, const double* __
       sx += m * fx[t];
       sy += m * fy[t];
     }
-    W[(size_t)c * 2 * NB + a] = mu * sx;
-    W[(size_t)c * 2 * NB + NB + a] = mu * sy;
+    w[a] = mu * sx;
+    w[NB + a] = mu * sy;
   }
 }
 
+// g = f_v + B^T W of cell X: its own block and its 4 face neighbours' blocks,
+// each neighbour's W recomputed from its right-hand side (no W in memory)
 template <int K, int NRHS>
 __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __restrict__ b1,
-                                                  const double* __restrict__ b2, const double* __restrict__ W1,
-                                                  const double* __restrict__ W2, double* __restrict__ g1,
-                                                  double* __restrict__ g2) {
+                                                  const double* __restrict__ b2, double mu1, double mu2,
+                                                  double* __restrict__ g1, double* __restrict__ g2, int64_t sb,
+                                                  int64_t sg) {
   constexpr int NB = Sz<K>::NB;
   const int X = blockIdx.x * blockDim.x + threadIdx.x;
   if (X >= R.ncell) return;
+  {
+    const int64_t i = blockIdx.y;
+    b1 += i * sb;
+    b2 += i * sb;
+    g1 += i * sg;
+    g2 += i * sg;
+  }
   const size_t nc = R.ncell;
   double a1[NB], a2[NB];
 #pragma unroll
@@ -287,29 +309,19 @@ __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __r
     a1[a] = b1[2 * nc * NB + (size_t)X * NB + a];
     a2[a] = (NRHS > 1) ? b2[2 * nc * NB + (size_t)X * NB + a] : 0.0;
   }
-  {
-    const double* B = R.table + (size_t)R.type[X] * R.stride + NB * NB;
-    for (int i = 0; i < 2 * NB; ++i) {
-      const double w1 = W1[(size_t)X * 2 * NB + i];
-      const double w2 = (NRHS > 1) ? W2[(size_t)X * 2 * NB + i] : 0.0;
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        a1[a] += B[i * NB + a] * w1;
-        if (NRHS > 1) a2[a] += B[i * NB + a] * w2;
-      }
-    }
-  }
-  for (int f = 0; f < 4; ++f) {
-    const int M = R.nbr[4 * (size_t)X + f];
+  for (int f = -1; f < 4; ++f) {
+    const int M = f < 0 ? X : R.nbr[4 * (size_t)X + f];
     if (M < 0) continue;
-    const double* B = R.table + (size_t)R.type[M] * R.stride + NB * NB + 2 * NB * NB + (f ^ 1) * 2 * NB * NB;
+    const double* B = R.table + (size_t)R.type[M] * R.stride + NB * NB +
+                      (f < 0 ? 0 : 2 * NB * NB + (f ^ 1) * 2 * NB * NB);
+    double w1[2 * NB], w2[2 * NB];
+    schur_w<K>(R, b1, mu1, M, w1);
+    if (NRHS > 1) schur_w<K>(R, b2, mu2, M, w2);
     for (int i = 0; i < 2 * NB; ++i) {
-      const double w1 = W1[(size_t)M * 2 * NB + i];
-      const double w2 = (NRHS > 1) ? W2[(size_t)M * 2 * NB + i] : 0.0;
 #pragma unroll
       for (int a = 0; a < NB; ++a) {
-        a1[a] += B[i * NB + a] * w1;
-        if (NRHS > 1) a2[a] += B[i * NB + a] * w2;
+        a1[a] += B[i * NB + a] * w1[i];
+        if (NRHS > 1) a2[a] += B[i * NB + a] * w2[i];
       }
     }
   }
@@ -321,14 +333,25 @@ __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __r
   }
 }
 
+// q = W - A^{-1} B u; writes [Jx | Jy | u] (W recomputed from the RHS b)
 template <int K, int NRHS>
-__global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double* __restrict__ W1,
-                                                      const double* __restrict__ W2, const double* __restrict__ x1,
+__global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double* __restrict__ b1,
+                                                      const double* __restrict__ b2, const double* __restrict__ x1,
                                                       const double* __restrict__ x2, double mu1, double mu2,
-                                                      double* __restrict__ o1, double* __restrict__ o2) {
+                                                      double* __restrict__ o1, double* __restrict__ o2, int64_t sb,
+                                                      int64_t sx, int64_t so) {
   constexpr int NB = Sz<K>::NB;
   const int M = blockIdx.x * blockDim.x + threadIdx.x;
   if (M >= R.ncell) return;
+  {
+    const int64_t i = blockIdx.y;
+    b1 += i * sb;
+    b2 += i * sb;
+    x1 += i * sx;
+    x2 += i * sx;
+    o1 += i * so;
+    o2 += i * so;
+  }
   const size_t nc = R.ncell;
   const double* tb = R.table + (size_t)R.type[M] * R.stride;
   double s1[2 * NB], s2[2 * NB], u1[NB], u2[NB];
@@ -370,6 +393,9 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
       s2[i] += t2;
     }
   }
+  double w1[2 * NB], w2[2 * NB];
+  schur_w<K>(R, b1, mu1, M, w1);
+  if (NRHS > 1) schur_w<K>(R, b2, mu2, M, w2);
   const double* Minv = tb;
 #pragma unroll
   for (int h = 0; h < 2; ++h)
@@ -382,8 +408,8 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
         if (NRHS > 1) t2 += Minv[a * NB + t] * s2[h * NB + t];
       }
       const size_t off = (size_t)h * nc * NB + (size_t)M * NB + a;
-      o1[off] = W1[(size_t)M * 2 * NB + h * NB + a] - mu1 * t1;
-      if (NRHS > 1) o2[off] = W2[(size_t)M * 2 * NB + h * NB + a] - mu2 * t2;
+      o1[off] = w1[h * NB + a] - mu1 * t1;
+      if (NRHS > 1) o2[off] = w2[h * NB + a] - mu2 * t2;
     }
 #pragma unroll
   for (int a = 0; a < NB; ++a) {
@@ -796,10 +822,20 @@ __device__ __forceinline__ double poisson_bias(const double* tb, const DevPhys& 
 }
 
 template <int K>
-__global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, DevPhys ph, const double* __restrict__ un,
-                                                     const double* __restrict__ up, const double* __restrict__ ur,
-                                                     const double* __restrict__ uo, int ncs, int nce,
-                                                     double* __restrict__ y) {
+__global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, const DevPhys* __restrict__ phs,
+                                                     const double* __restrict__ un, const double* __restrict__ up,
+                                                     const double* __restrict__ ur, const double* __restrict__ uo,
+                                                     int ncs, int nce, double* __restrict__ y, int64_t ss, int64_t se,
+                                                     int64_t sy) {
+  const DevPhys ph = phs[blockIdx.y];
+  {
+    const int64_t i = blockIdx.y;
+    un += i * ss;
+    up += i * ss;
+    ur += i * se;
+    uo += i * se;
+    y += i * sy;
+  }
   using S = Sz<K>;
   constexpr int NRT = S::NRT, NL = S::NL, NPHI = S::NPHI;
   const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
@@ -867,9 +903,11 @@ __global__ __launch_bounds__(64) void k_poisson_cell_from_b(DevPoisson P, const 
 }
 
 __global__ __launch_bounds__(256) void k_poisson_face(DevPoisson P, const double* __restrict__ y,
-                                                      double* __restrict__ r) {
+                                                      double* __restrict__ r, int64_t sy, int64_t sr) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= P.nmult) return;
+  y += blockIdx.y * sy;
+  r += blockIdx.y * sr;
   double s = 0.0;
   for (int e = 0; e < 2; ++e) {
     const int c = P.mcell[2 * m + e];
@@ -880,10 +918,14 @@ __global__ __launch_bounds__(256) void k_poisson_face(DevPoisson P, const double
 
 template <int K>
 __global__ __launch_bounds__(64) void k_poisson_recover(DevPoisson P, const double* __restrict__ lam,
-                                                        const double* __restrict__ y, double* __restrict__ out) {
+                                                        const double* __restrict__ y, double* __restrict__ out,
+                                                        int64_t sl, int64_t sy, int64_t so) {
   constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
   const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
   if (Kc >= P.ncell) return;
+  lam += blockIdx.y * sl;
+  y += blockIdx.y * sy;
+  out += blockIdx.y * so;
   double e[4 * K];
   int ej[4 * K];
   int ne = 0;
@@ -915,7 +957,7 @@ __global__ __launch_bounds__(64) void k_poisson_recover(DevPoisson P, const doub
 }
 
 template <int K>
-__global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTabs T, DevPhys ph,
+__global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTabs T, const DevPhys* __restrict__ phs,
                                                              const double* __restrict__ un,
                                                              const double* __restrict__ up,
                                                              const double* __restrict__ ur,
@@ -924,6 +966,7 @@ __global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTa
   constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT, NPHI = Sz<K>::NPHI;
   const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
   if (Kc >= P.ncell) return;
+  const DevPhys ph = phs[0];
   double fphi[NPHI];
   poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, fphi);
   const int* id = P.idx + (size_t)Kc * NL;
@@ -976,43 +1019,42 @@ inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
   }
 
 int launch_rhs_carrier(int k, bool semi, const DevRegion& R, const DevRegion& O, const DevPoisson& P,
-                       const DevTabs& T, const DevPhys& ph, const double* u1, const double* u2, const double* ou1,
-                       const double* ou2, const double* pvec, double* b1, double* b2, hipStream_t s) {
+                       const DevTabs& T, const DevPhys* phs, const double* u1, const double* u2, const double* ou1,
+                       const double* ou2, const double* pvec, double* b1, double* b2, hipStream_t s, int nb,
+                       int64_t sR, int64_t sO, int64_t sP) {
   if (R.ncell == 0) return 0;
+  const dim3 g(nblk(R.ncell, 64), nb);
   if (semi) {
-    PECS_DISPATCH_K(k, (k_rhs_carrier<KK, true><<<nblk(R.ncell, 64), 64, 0, s>>>(R, O, P, T, ph, u1, u2, ou1, ou2,
-                                                                                  pvec, b1, b2)));
+    PECS_DISPATCH_K(k, (k_rhs_carrier<KK, true><<<g, 64, 0, s>>>(R, O, P, T, phs, u1, u2, ou1, ou2, pvec, b1, b2, sR,
+                                                                 sO, sP)));
   } else {
-    PECS_DISPATCH_K(k, (k_rhs_carrier<KK, false><<<nblk(R.ncell, 64), 64, 0, s>>>(R, O, P, T, ph, u1, u2, ou1, ou2,
-                                                                                   pvec, b1, b2)));
+    PECS_DISPATCH_K(k, (k_rhs_carrier<KK, false><<<g, 64, 0, s>>>(R, O, P, T, phs, u1, u2, ou1, ou2, pvec, b1, b2, sR,
+                                                                  sO, sP)));
   }
   return (int)hipGetLastError();
 }
 
-int launch_schur_prep(int k, const DevRegion& R, const double* b, double mu, double* W, hipStream_t s) {
-  PECS_DISPATCH_K(k, (k_schur_prep<KK><<<nblk(R.ncell, 64), 64, 0, s>>>(R, b, mu, W)));
-  return (int)hipGetLastError();
-}
-
-int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, const double* W1,
-                     const double* W2, double* g1, double* g2, int nrhs, hipStream_t s) {
+int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, double mu1, double mu2,
+                     double* g1, double* g2, int nrhs, hipStream_t s, int nb, int64_t sb, int64_t sg) {
+  const dim3 g(nblk(R.ncell, 64), nb);
   if (nrhs == 2) {
-    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 2><<<nblk(R.ncell, 64), 64, 0, s>>>(R, b1, b2, W1, W2, g1, g2)));
+    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, mu1, mu2, g1, g2, sb, sg)));
   } else {
-    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 1><<<nblk(R.ncell, 64), 64, 0, s>>>(R, b1, b1, W1, W1, g1, g1)));
+    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, mu1, mu1, g1, g1, sb, sg)));
   }
   return (int)hipGetLastError();
 }
 
-int launch_schur_recover(int k, const DevRegion& R, const double* W1, const double* W2, const double* xs1,
-                         const double* xs2, double mu1, double mu2, double* out1, double* out2, const double*,
-                         const double*, int nrhs, hipStream_t s) {
+int launch_schur_recover(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
+                         const double* xs2, double mu1, double mu2, double* out1, double* out2, int nrhs,
+                         hipStream_t s, int nb, int64_t sb, int64_t sx, int64_t so) {
+  const dim3 g(nblk(R.ncell, 64), nb);
   if (nrhs == 2) {
-    PECS_DISPATCH_K(k, (k_schur_recover<KK, 2><<<nblk(R.ncell, 64), 64, 0, s>>>(R, W1, W2, xs1, xs2, mu1, mu2, out1,
-                                                                               out2)));
+    PECS_DISPATCH_K(k, (k_schur_recover<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, xs1, xs2, mu1, mu2, out1, out2, sb, sx,
+                                                                so)));
   } else {
-    PECS_DISPATCH_K(k, (k_schur_recover<KK, 1><<<nblk(R.ncell, 64), 64, 0, s>>>(R, W1, W1, xs1, xs1, mu1, mu1, out1,
-                                                                               out1)));
+    PECS_DISPATCH_K(k, (k_schur_recover<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, xs1, xs1, mu1, mu1, out1, out1, sb, sx,
+                                                                so)));
   }
   return (int)hipGetLastError();
 }
@@ -1121,10 +1163,11 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
   return (int)hipGetLastError();
 }
 
-int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
+int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                         const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
-                        hipStream_t s) {
-  PECS_DISPATCH_K(k, (k_poisson_cell<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, T, ph, un, up, ur, uo, ncs, nce, y)));
+                        hipStream_t s, int nb, int64_t ss, int64_t se, int64_t sy) {
+  PECS_DISPATCH_K(k, (k_poisson_cell<KK><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(P, T, phs, un, up, ur, uo, ncs,
+                                                                                    nce, y, ss, se, sy)));
   return (int)hipGetLastError();
 }
 
@@ -1134,23 +1177,23 @@ int launch_poisson_cell_from_b(const DevPoisson& P, const double* b, double* y, 
   return (int)hipGetLastError();
 }
 
-int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s) {
-  if (P.nmult == 0) return 0;
-  k_poisson_face<<<nblk(P.nmult, 256), 256, 0, s>>>(P, y, r);
+int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s, int nb, int64_t sy,
+                        int64_t sr) {
+  k_poisson_face<<<dim3(nblk(P.nmult, 256), nb), 256, 0, s>>>(P, y, r, sy, sr);
   return (int)hipGetLastError();
 }
 
 int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
-                           hipStream_t s) {
-  PECS_DISPATCH_K(k, (k_poisson_recover<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, lam, y, out)));
+                           hipStream_t s, int nb, int64_t sl, int64_t sy, int64_t so) {
+  PECS_DISPATCH_K(k, (k_poisson_recover<KK><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl, sy, so)));
   return (int)hipGetLastError();
 }
 
-int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
+int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                                 const double* up, const double* ur, const double* uo, int ncs, int nce, double* b,
                                 hipStream_t s) {
-  PECS_DISPATCH_K(k, (k_poisson_rhs_assemble<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, T, ph, un, up, ur, uo, ncs, nce,
-                                                                                  b)));
+  PECS_DISPATCH_K(k, (k_poisson_rhs_assemble<KK><<<nblk(P.ncell, 64), 64, 0, s>>>(P, T, phs, un, up, ur, uo, ncs,
+                                                                                   nce, b)));
   return (int)hipGetLastError();
 }
 

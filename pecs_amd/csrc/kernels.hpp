@@ -108,16 +108,21 @@ struct FactorLevels {
   int nlevels;
 };
 
-// launchers (return hipError_t as int)
+// launchers (return hipError_t as int).  The cell-local launchers run nb
+// instances of a batch at once (grid dimension y); each vector argument of
+// instance i is at its pointer + i * (its stride), physics at phs[i].
 int launch_rhs_carrier(int k, bool semi, const DevRegion& R, const DevRegion& O, const DevPoisson& P,
-                       const DevTabs& T, const DevPhys& ph, const double* u1, const double* u2, const double* ou1,
-                       const double* ou2, const double* pvec, double* b1, double* b2, hipStream_t s);
-int launch_schur_prep(int k, const DevRegion& R, const double* b, double mu, double* W, hipStream_t s);
-int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, const double* W1,
-                     const double* W2, double* g1, double* g2, int nrhs, hipStream_t s);
-int launch_schur_recover(int k, const DevRegion& R, const double* W1, const double* W2, const double* xs1,
-                         const double* xs2, double mu1, double mu2, double* out1, double* out2, const double* b1,
-                         const double* b2, int nrhs, hipStream_t s);
+                       const DevTabs& T, const DevPhys* phs, const double* u1, const double* u2, const double* ou1,
+                       const double* ou2, const double* pvec, double* b1, double* b2, hipStream_t s, int nb = 1,
+                       int64_t sR = 0, int64_t sO = 0, int64_t sP = 0);
+// Schur transform g = f_v + B^T (mu A^{-1} f_p) and recovery
+// q = mu A^{-1} f_p - mu A^{-1} B u of one region's carriers (b: the LDG RHS
+// [f_p | f_v] of each carrier; g / xs in the factor's permuted order).
+int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, double mu1, double mu2,
+                     double* g1, double* g2, int nrhs, hipStream_t s, int nb = 1, int64_t sb = 0, int64_t sg = 0);
+int launch_schur_recover(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
+                         const double* xs2, double mu1, double mu2, double* out1, double* out2, int nrhs,
+                         hipStream_t s, int nb = 1, int64_t sb = 0, int64_t sx = 0, int64_t so = 0);
 // One level's sweep over `nt` row tiles.  w > 0: W = w lanes per row and
 // `maxr` row passes per tile (W in {8,16,32,64} with maxr in {1,2,4}, or W in
 // {32,64} with maxr 8; every tile has at most (256 / W) * maxr rows), for 1 or
@@ -135,14 +140,15 @@ int mfma_rhs_pad(int nrhs);
 // factor shaped for 1-2 right-hand sides, mfma_rhs_pad(nrhs) * n for an
 // MFMA-shaped one (whose scratch X, Z and t hold RHS-interleaved vectors).
 int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s);
-int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
+int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                         const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
-                        hipStream_t s);
+                        hipStream_t s, int nb = 1, int64_t ss = 0, int64_t se = 0, int64_t sy = 0);
 int launch_poisson_cell_from_b(const DevPoisson& P, const double* b, double* y, hipStream_t s);
-int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s);
+int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s, int nb = 1, int64_t sy = 0,
+                        int64_t sr = 0);
 int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
-                           hipStream_t s);
-int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, const DevPhys& ph, const double* un,
+                           hipStream_t s, int nb = 1, int64_t sl = 0, int64_t sy = 0, int64_t so = 0);
+int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                                 const double* up, const double* ur, const double* uo, int ncs, int nce, double* b,
                                 hipStream_t s);
 int launch_diff_norms(const double* a, const double* b, int64_t n, double* acc, hipStream_t s);
