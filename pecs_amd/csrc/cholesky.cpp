@@ -179,13 +179,24 @@ void lower_inverse(const double* L, int n, int ld, double* X) {
 
 }  // namespace
 
-SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
+namespace {
+
+// Ordering, supernodes, row structure and levels (no numeric values).
+struct Symbolic {
+  SupernodalFactor F;
+  std::vector<NDNode> nodes;
+  std::vector<int> pos, node_pos0, node_pos1;
+  std::vector<std::vector<int>> structc;
+};
+
+Symbolic symbolic(const BlockSPD& S, int leaf_cells) {
   const int NG = S.NXc * S.NYc;
   const int no = S.noff();
   NDBuilder nd(S, leaf_cells);
   nd.build(0, S.NXc, 0, S.NYc);
   const int nn = (int)nd.nodes.size();
-  SupernodalFactor F;
+  Symbolic Y;
+  SupernodalFactor& F = Y.F;
   // cell positions and DoF bases
   std::vector<int> pos(NG, -1), node_of(NG, -1);
   std::vector<int> node_pos0(nn), node_pos1(nn);
@@ -255,14 +266,55 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
     l21 += (int64_t)ms * N.ns;
     rows += ms;
   }
-  F.Linv.assign(linv, 0.0);
-  F.L21.assign(l21, 0.0);
   F.rows.assign(rows, 0);
   for (int s = 0; s < nn; ++s) {
     int t = 0;
     for (int c : structc[s])
       for (int a = 0; a < S.ndof[c]; ++a) F.rows[F.nodes[s].rows_off + t++] = F.cell_base[c] + a;
   }
+  F.linv_size = linv;
+  F.l21_size = l21;
+  // forward contribution lists
+  std::vector<int64_t> cnt(F.n + 1, 0);
+  for (int s = 0; s < nn; ++s)
+    for (int r = 0; r < F.nodes[s].ms; ++r) cnt[F.rows[F.nodes[s].rows_off + r] + 1]++;
+  for (int i = 0; i < F.n; ++i) cnt[i + 1] += cnt[i];
+  F.contrib_ptr = cnt;
+  F.contrib_node.assign(cnt[F.n], 0);
+  F.contrib_row.assign(cnt[F.n], 0);
+  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int s = 0; s < nn; ++s)
+    for (int r = 0; r < F.nodes[s].ms; ++r) {
+      const int pp = F.rows[F.nodes[s].rows_off + r];
+      F.contrib_node[fill[pp]] = s;
+      F.contrib_row[fill[pp]] = r;
+      fill[pp]++;
+    }
+  Y.nodes = nd.nodes;
+  Y.pos = pos;
+  Y.node_pos0 = node_pos0;
+  Y.node_pos1 = node_pos1;
+  Y.structc = structc;
+  return Y;
+}
+
+}  // namespace
+
+SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
+  const int NG = S.NXc * S.NYc;
+  const int no = S.noff();
+  Symbolic Y = symbolic(S, leaf_cells);
+  SupernodalFactor& F = Y.F;
+  const int nn = (int)F.nodes.size();
+  const auto& pos = Y.pos;
+  const auto& node_pos0 = Y.node_pos0;
+  const auto& node_pos1 = Y.node_pos1;
+  const auto& structc = Y.structc;
+  struct {
+    const std::vector<NDNode>& nodes;
+  } nd{Y.nodes};
+  F.Linv.assign(F.linv_size, 0.0);
+  F.L21.assign(F.l21_size, 0.0);
   // numeric multifrontal factorisation, level by level on the elimination tree:
   // nodes of one level are independent (their children are in lower levels).
   std::vector<std::vector<double>> upd(nn);
@@ -365,23 +417,167 @@ SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells) {
     F.t_factor += tf_t[t];
     F.t_inverse += ti_t[t];
   }
-  // forward contribution lists
-  std::vector<int64_t> cnt(F.n + 1, 0);
-  for (int s = 0; s < nn; ++s)
-    for (int r = 0; r < F.nodes[s].ms; ++r) cnt[F.rows[F.nodes[s].rows_off + r] + 1]++;
-  for (int i = 0; i < F.n; ++i) cnt[i + 1] += cnt[i];
-  F.contrib_ptr = cnt;
-  F.contrib_node.assign(cnt[F.n], 0);
-  F.contrib_row.assign(cnt[F.n], 0);
-  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-  for (int s = 0; s < nn; ++s)
-    for (int r = 0; r < F.nodes[s].ms; ++r) {
-      const int pp = F.rows[F.nodes[s].rows_off + r];
-      F.contrib_node[fill[pp]] = s;
-      F.contrib_row[fill[pp]] = r;
-      fill[pp]++;
-    }
   return F;
+}
+
+SupernodalFactor cholesky_symbolic(const BlockSPD& S, int leaf_cells, FrontPlan& plan) {
+  const int no = S.noff();
+  Symbolic Y = symbolic(S, leaf_cells);
+  SupernodalFactor& F = Y.F;
+  const int nn = (int)F.nodes.size();
+  const int NG = S.NXc * S.NYc;
+  plan = FrontPlan{};
+  plan.nlevels = F.nlevels;
+  // S blocks, concatenated
+  plan.sblk_off.assign((size_t)NG * no, -1);
+  int64_t sv = 0;
+  for (int X = 0; X < NG; ++X)
+    for (int o = 0; o < no; ++o) {
+      const auto& B = S.blk[(size_t)X * no + o];
+      if (B.empty()) continue;
+      plan.sblk_off[(size_t)X * no + o] = sv;
+      sv += (int64_t)B.size();
+    }
+  plan.svals.resize(sv);
+  for (int X = 0; X < NG; ++X)
+    for (int o = 0; o < no; ++o) {
+      const int64_t off = plan.sblk_off[(size_t)X * no + o];
+      if (off >= 0) std::copy(S.blk[(size_t)X * no + o].begin(), S.blk[(size_t)X * no + o].end(), &plan.svals[off]);
+    }
+  // fronts, level by level (each level's fronts contiguous)
+  plan.front_off.assign(nn, 0);
+  plan.level_front.assign(F.nlevels + 1, 0);
+  int64_t fo = 0;
+  for (int l = 0; l < F.nlevels; ++l) {
+    plan.level_front[l] = fo;
+    for (int s : F.level_nodes[l]) {
+      const int64_t nf = F.nodes[s].ns + F.nodes[s].ms;
+      plan.front_off[s] = fo;
+      fo += nf * nf;
+    }
+  }
+  plan.level_front[F.nlevels] = fo;
+  plan.front_total = fo;
+  // local front position of each cell of a node's front (own cells, then struct)
+  std::vector<int> loc(NG, -1);
+  plan.sc_level.assign(F.nlevels + 1, 0);
+  plan.ea_level.assign(2 * F.nlevels + 1, 0);
+  plan.ea_map_off.assign(nn + 1, 0);
+  // child row maps into the parent's front, per child (ms entries)
+  std::vector<std::vector<int>> cmap(nn);
+  for (int l = 0; l < F.nlevels; ++l) {
+    plan.sc_level[l] = (int)plan.sc_src.size();
+    for (int s : F.level_nodes[l]) {
+      const int nf = F.nodes[s].ns + F.nodes[s].ms;
+      int off = 0;
+      for (int c : Y.nodes[s].cells) {
+        loc[c] = off;
+        off += S.ndof[c];
+      }
+      for (int c : Y.structc[s]) {
+        loc[c] = off;
+        off += S.ndof[c];
+      }
+      for (int X : Y.nodes[s].cells) {
+        const int nx = S.ndof[X];
+        if (nx == 0) continue;
+        const int xi = X % S.NXc, xj = X / S.NXc;
+        for (int o = 0; o < no; ++o) {
+          const int yi = xi + S.offi[o], yj = xj + S.offj[o];
+          if (yi < 0 || yi >= S.NXc || yj < 0 || yj >= S.NYc) continue;
+          const int Yc = yi + S.NXc * yj;
+          const int64_t so = plan.sblk_off[(size_t)X * no + o];
+          if (so < 0 || S.ndof[Yc] == 0) continue;
+          if (Y.pos[Yc] < Y.node_pos0[s]) continue;
+          const bool own = Y.pos[Yc] < Y.node_pos1[s];
+          const int rx = loc[X], ry = loc[Yc];
+          plan.sc_src.push_back(so);
+          plan.sc_dst.push_back(plan.front_off[s] + (own ? (int64_t)rx * nf + ry : (int64_t)ry * nf + rx));
+          plan.sc_nx.push_back(nx);
+          plan.sc_ny.push_back(S.ndof[Yc]);
+          plan.sc_nf.push_back(nf);
+          plan.sc_mode.push_back(own ? 0 : 1);
+          plan.sc_rx.push_back(rx);
+          plan.sc_ry.push_back(ry);
+        }
+      }
+      for (int c : Y.nodes[s].children) {
+        std::vector<int>& m = cmap[c];
+        m.clear();
+        for (int Yc : Y.structc[c])
+          for (int a = 0; a < S.ndof[Yc]; ++a) m.push_back(loc[Yc] + a);
+      }
+      for (int c : Y.nodes[s].cells) loc[c] = -1;
+      for (int c : Y.structc[s]) loc[c] = -1;
+    }
+  }
+  plan.sc_level[F.nlevels] = (int)plan.sc_src.size();
+  for (int s = 0; s < nn; ++s) plan.ea_map_off[s + 1] = plan.ea_map_off[s] + (int64_t)cmap[s].size();
+  plan.ea_map.resize(plan.ea_map_off[nn]);
+  for (int s = 0; s < nn; ++s) std::copy(cmap[s].begin(), cmap[s].end(), &plan.ea_map[plan.ea_map_off[s]]);
+  // extend-add rows, per parent level and pass (child 0, then child 1 of every parent)
+  for (int l = 0; l < F.nlevels; ++l)
+    for (int pass = 0; pass < 2; ++pass) {
+      plan.ea_level[2 * l + pass] = (int)plan.ea_child.size();
+      for (int s : F.level_nodes[l]) {
+        if ((int)Y.nodes[s].children.size() <= pass) continue;
+        const int c = Y.nodes[s].children[pass];
+        for (int r = 0; r < F.nodes[c].ms; ++r) {
+          plan.ea_child.push_back(c);
+          plan.ea_row.push_back(r);
+          plan.ea_parent.push_back(s);
+        }
+      }
+    }
+  plan.ea_level[2 * F.nlevels] = (int)plan.ea_child.size();
+  for (int s = 0; s < nn; ++s)
+    if (Y.nodes[s].children.size() > 2) throw std::runtime_error("cholesky_symbolic: more than two children");
+  return F;
+}
+
+void execute_plan_host(const FrontPlan& plan, const SupernodalFactor& F, std::vector<double>& Linv,
+                       std::vector<double>& L21) {
+  const int nn = (int)F.nodes.size();
+  std::vector<double> fr(plan.front_total, 0.0);
+  Linv.assign(F.linv_size, 0.0);
+  L21.assign(F.l21_size, 0.0);
+  double flops = 0.0;
+  for (int l = 0; l < F.nlevels; ++l) {
+    for (int i = plan.sc_level[l]; i < plan.sc_level[l + 1]; ++i) {
+      const int nx = plan.sc_nx[i], ny = plan.sc_ny[i], nf = plan.sc_nf[i];
+      for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b) {
+          const double v = plan.svals.at(plan.sc_src[i] + (int64_t)a * ny + b);
+          if (plan.sc_mode[i] == 0) {
+            if (plan.sc_rx[i] + a < plan.sc_ry[i] + b) continue;
+            fr.at(plan.sc_dst[i] + (int64_t)a * nf + b) += v;
+          } else {
+            fr.at(plan.sc_dst[i] + (int64_t)b * nf + a) += v;
+          }
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = plan.ea_level[2 * l + pass]; i < plan.ea_level[2 * l + pass + 1]; ++i) {
+        const int c = plan.ea_child[i], r = plan.ea_row[i], s = plan.ea_parent[i];
+        const int nsc = F.nodes[c].ns, nfc = nsc + F.nodes[c].ms, nfp = F.nodes[s].ns + F.nodes[s].ms;
+        const int* m = &plan.ea_map[plan.ea_map_off[c]];
+        for (int q = 0; q <= r; ++q)
+          fr.at(plan.front_off[s] + (int64_t)m[r] * nfp + m[q]) +=
+              fr.at(plan.front_off[c] + (int64_t)(nsc + r) * nfc + nsc + q);
+      }
+    for (int s : F.level_nodes[l]) {
+      const SNode& N = F.nodes[s];
+      const int nf = N.ns + N.ms;
+      double* Fr = &fr[plan.front_off[s]];
+      if (N.ns > 0) {
+        partial_cholesky(Fr, nf, N.ns, flops);
+        lower_inverse(Fr, N.ns, nf, &Linv[N.linv_off]);
+      }
+      for (int r = 0; r < N.ms; ++r)
+        for (int c2 = 0; c2 < N.ns; ++c2) L21[N.l21_off + (int64_t)r * N.ns + c2] = Fr[(size_t)(N.ns + r) * nf + c2];
+    }
+  }
+  (void)nn;
 }
 
 PackedFactor pack_factor(const SupernodalFactor& F) {

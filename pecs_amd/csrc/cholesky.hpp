@@ -35,12 +35,39 @@ struct SupernodalFactor {
   // forward contributions: for each permuted row p the (node, panel row) pairs
   std::vector<int64_t> contrib_ptr;
   std::vector<int> contrib_node, contrib_row;
+  int64_t linv_size = 0, l21_size = 0;  // lengths of Linv / L21 when filled
   double flops = 0.0;
   double t_assemble = 0.0, t_factor = 0.0, t_inverse = 0.0;
   int64_t nnz_L() const { return (int64_t)Linv.size() + (int64_t)L21.size(); }
 };
 
 SupernodalFactor cholesky_factor(const BlockSPD& S, int leaf_cells);
+
+// Index plan of the multifrontal factorisation for execution on the device:
+// each node's dense nf x nf row-major front (nf = ns + ms; fronts of one level
+// contiguous), the S blocks scattered into the fronts (mode 0: own-own block,
+// lower triangle kept; mode 1: block of a panel row, stored transposed), and
+// the extend-add rows (child c's update row r into its parent's front through
+// the child's row map), in two passes per level (first, then second child).
+struct FrontPlan {
+  int nlevels = 0;
+  std::vector<int64_t> front_off, level_front;  // per node; per level [begin, end)
+  int64_t front_total = 0;
+  std::vector<double> svals;                    // S blocks concatenated
+  std::vector<int64_t> sblk_off;                // per (cell, offset): block offset or -1
+  std::vector<int64_t> sc_src, sc_dst;          // scatter items
+  std::vector<int> sc_nx, sc_ny, sc_nf, sc_mode, sc_rx, sc_ry, sc_level;
+  std::vector<int> ea_child, ea_row, ea_parent, ea_level;  // extend-add rows; ea_level per (level, pass)
+  std::vector<int> ea_map;                      // per child: its panel rows' positions in the parent front
+  std::vector<int64_t> ea_map_off;
+};
+// The ordering, supernodes and row structure of cholesky_factor (no numeric
+// values: Linv / L21 stay empty) and the device plan of its numeric phase.
+SupernodalFactor cholesky_symbolic(const BlockSPD& S, int leaf_cells, FrontPlan& plan);
+// Host execution of a plan (for checking the plan's indexing against
+// cholesky_factor): fills Linv and L21 as cholesky_factor does.
+void execute_plan_host(const FrontPlan& plan, const SupernodalFactor& F, std::vector<double>& Linv,
+                       std::vector<double>& L21);
 
 // The factor rearranged for level-scheduled dense matrix-vector sweeps (the
 // layout the GPU solve kernels read).  Per supernode s:

@@ -300,6 +300,22 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, int KS, 
 static double check(const char* name, const BlockSPD& S, int leaf) {
   auto t0 = std::chrono::steady_clock::now();
   SupernodalFactor F = cholesky_factor(S, leaf);
+  {  // the device plan, executed on the host, must reproduce the factor
+    FrontPlan plan;
+    SupernodalFactor G = cholesky_symbolic(S, leaf, plan);
+    std::vector<double> Li, L2;
+    execute_plan_host(plan, G, Li, L2);
+    double d = 0;
+    for (size_t i = 0; i < Li.size(); ++i) d = std::max(d, std::fabs(Li[i] - F.Linv[i]));
+    for (size_t i = 0; i < L2.size(); ++i) d = std::max(d, std::fabs(L2[i] - F.L21[i]));
+    std::printf("   front plan on host: max |diff| to cholesky_factor %.3e (fronts %.1f MB, %zu scatter items, %zu "
+                "extend-add rows)\n",
+                d, plan.front_total * 8e-6, plan.sc_src.size(), plan.ea_child.size());
+    if (d != 0.0 || Li.size() != F.Linv.size() || L2.size() != F.L21.size()) {
+      std::printf("front plan mismatch\n");
+      std::exit(2);
+    }
+  }
   double ts = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::mt19937_64 rng(20261202);
   std::uniform_real_distribution<double> U(-1, 1);
