@@ -98,7 +98,11 @@ const char* pecs_last_error(pecs_ctx* ctx);
 const char* pecs_global_error(void);
 int pecs_set_stream(pecs_ctx* ctx, void* hip_stream);
 int pecs_sizes(pecs_ctx* ctx, int64_t* n_semi, int64_t* n_elec, int64_t* n_poisson);
-/* Setup statistics: factor sizes (doubles), setup seconds, #supernode levels. */
+/* Setup statistics, up to 12 values: factor sizes in doubles (semiconductor,
+ * electrolyte, multiplier), setup seconds, factorisation seconds (symbolic +
+ * numeric + sweep layout + tuning), supernode levels (3), Schur system size,
+ * multiplier system size, seconds and dense flops of the numeric phase on the
+ * device (0 when PECS_HOST_FACTOR=1 factors on the host). */
 int pecs_stats(pecs_ctx* ctx, double* out, int n);
 
 /* Constant matrices as CSR (rows, nnz) in the reference layouts, rebuilt on the
@@ -108,6 +112,18 @@ int pecs_stats(pecs_ctx* ctx, double* out, int n);
  * `constrained` (may be NULL, PECS_POISSON only) flags Neumann D.n DoFs. */
 int pecs_export_matrix(const pecs_mesh* mesh, const pecs_params* params, int which, int64_t* n, int64_t* nnz,
                        int64_t* rowptr, int32_t* col, double* val, int8_t* constrained);
+
+/* VTU output of a state, as print_results (SolarCell.cpp:1826-1858) with the
+ * rescaling of PostProcessor (PostProcessor.cpp:9-27, 110-140):
+ * <dir>/Semiconductor-NNN.vtu ("Electrons Current", "Electrons Density",
+ * "Holes ..."), <dir>/Electrolyte-NNN.vtu ("Reductants ...", "Oxidants ...")
+ * and <dir>/Poisson-NNN.vtu ("Field", "Potential"), NNN = step in three
+ * digits.  Vectors as pecs_get_state returns them; char_* are the
+ * "characteristic length / density / time" of the parameter file.  Stateless:
+ * runs on the host. */
+int pecs_write_vtu(const pecs_mesh* mesh, const pecs_params* params, const double* u_n, const double* u_p,
+                   const double* u_r, const double* u_o, const double* poisson, int step, const char* dir,
+                   double char_length, double char_density, double char_time);
 
 /* Per-step right-hand sides.
  * assemble_semiconductor_rhs (SolarCell.cpp:1037-1071, worker 1073-1414):

@@ -25,7 +25,7 @@ ABI_FUNCTIONS = [
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config",
     "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
-    "pecs_last_residual_batch",
+    "pecs_last_residual_batch", "pecs_write_vtu",
 ]
 
 
@@ -192,6 +192,18 @@ def export_matrix(prm, which):
     return (A, cons.astype(bool)) if which == POISSON else A
 
 
+def write_vtu(prm, state, step, out_dir=".", char_length=1.0, char_density=1.0, char_time=1.0):
+    """print_results: Semiconductor-NNN.vtu, Electrolyte-NNN.vtu, Poisson-NNN.vtu
+    in out_dir from a state dict (keys n, p, r, o, P); host only."""
+    L = lib()
+    L.pecs_write_vtu.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(CParams)] + \
+        [ctypes.POINTER(ctypes.c_double)] * 5 + [ctypes.c_int, ctypes.c_char_p] + [ctypes.c_double] * 3
+    m, p = to_c(prm)
+    v = [np.ascontiguousarray(state[c], dtype=np.float64) for c in "nproP"]
+    _check(L.pecs_write_vtu(ctypes.byref(m), ctypes.byref(p), *[_ptr(a) for a in v], int(step),
+                            os.fsencode(out_dir), float(char_length), float(char_density), float(char_time)))
+
+
 def scale_parameters(phys):
     """Parameters::parse_and_scale_parameters through the library (dict in, dict out)."""
     c = CPhysical(**phys)
@@ -243,10 +255,11 @@ class SolarCellGPU:
         _check(rc, self.ctx)
 
     def stats(self):
-        out = np.zeros(10)
-        self._c(lib().pecs_stats(self.ctx, _ptr(out), 10))
+        out = np.zeros(12)
+        self._c(lib().pecs_stats(self.ctx, _ptr(out), 12))
         keys = ["factor_semi_doubles", "factor_elec_doubles", "factor_poisson_doubles", "setup_s", "factor_s",
-                "levels_semi", "levels_elec", "levels_poisson", "n_schur_semi", "n_multipliers"]
+                "levels_semi", "levels_elec", "levels_poisson", "n_schur_semi", "n_multipliers",
+                "device_numeric_s", "device_numeric_flops"]
         return dict(zip(keys, out.tolist()))
 
     def sweep_config(self, which):

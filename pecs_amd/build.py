@@ -19,8 +19,8 @@ HOST = os.path.join(HERE, "host")
 RUNNER = os.path.join(BUILD, "pecs_run")
 HOST_TEST = os.path.join(BUILD, "pecs_host_test")
 
-HOST_SRC = ["mesh.cpp", "ops.cpp", "cholesky.cpp"]
-HIP_SRC = ["kernels.hip", "abi.hip"]
+HOST_SRC = ["mesh.cpp", "ops.cpp", "cholesky.cpp", "vtu.cpp"]
+HIP_SRC = ["kernels.hip", "devfactor.hip", "abi.hip"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-mavx2", "-mfma"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -49,7 +49,8 @@ def build(verbose=False):
         _run([hipcc] + HIPFLAGS + ["-c", os.path.join(CSRC, s), "-o", o], verbose)
         hobjs.append(o)
     tmp = LIB + ".tmp"
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-o", tmp] + hobjs + objs + ["-lgomp"], verbose)
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-o", tmp] + hobjs + objs +
+         ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lgomp", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     _run(["g++"] + CXXFLAGS + [os.path.join(CSRC, "selftest.cpp")] + objs + ["-o", SELFTEST], verbose)
     # host classes in the reference roles + the command-line driver, linked to the library

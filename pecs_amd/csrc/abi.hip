@@ -14,10 +14,12 @@
 
 #include "../../include/pecs_abi.h"
 #include "cholesky.hpp"
+#include "devfactor.hpp"
 #include "fe.hpp"
 #include "kernels.hpp"
 #include "mesh.hpp"
 #include "ops.hpp"
+#include "vtu.hpp"
 
 using namespace pecs;
 
@@ -31,6 +33,7 @@ struct FactorDev {
   int nlevels = 0, n = 0;
   double nnz = 0.0;
   double tri_bytes = 0.0;      // algorithmic bytes of a forward + backward sweep
+  double device_seconds = 0.0, device_flops = 0.0;  // numeric factorisation on the device (0: on the host)
   std::vector<int> cell_base;  // host copy: grid cell -> permuted base
   FactorLevels lv() const {
     return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(), bw.data(), fm.data(),
@@ -336,19 +339,49 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
   rebuild_tiles(P);
 }
 
+// Factor S and upload the sweep layout.  The numeric factorisation runs on
+// the device (factor_on_device) after a host symbolic phase;
+// PECS_HOST_FACTOR=1 runs it on the host instead (cholesky_factor +
+// pack_factor), for comparison.
 FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
+  const char* hf = std::getenv("PECS_HOST_FACTOR");
+  const bool on_host = hf && std::string(hf) == "1";
   PackedFactor P;
-  {
+  FactorDev D;
+  double* dfwd = nullptr;  // device-factor output (sweep layout), owned here until handed over
+  double* dbwd = nullptr;
+  auto free_tmp = [&]() {
+    if (dfwd) (void)hipFree(dfwd);
+    if (dbwd) (void)hipFree(dbwd);
+    dfwd = dbwd = nullptr;
+  };
+  if (on_host) {
     SupernodalFactor F = cholesky_factor(S, leaf);
     P = pack_factor(F);
+  } else {
+    FrontPlan plan;
+    SupernodalFactor F = cholesky_symbolic(S, leaf, plan);
+    P = pack_layout(F);
+    HIPCHECK(hipMalloc(&dfwd, std::max<int64_t>(P.nvals, 1) * sizeof(double)));
+    if (hipMalloc(&dbwd, std::max<int64_t>(P.nvals, 1) * sizeof(double)) != hipSuccess) {
+      free_tmp();
+      throw std::runtime_error("make_factor: out of device memory for the factor");
+    }
+    try {
+      const DeviceFactorStats st = factor_on_device(plan, F, P, dfwd, dbwd, c->stream);
+      D.device_seconds = st.seconds;
+      D.device_flops = st.flops;
+    } catch (...) {
+      free_tmp();
+      throw;
+    }
   }
-  FactorDev D;
   D.n = P.n;
   D.nlevels = P.nlevels;
   D.cell_base = P.cell_base;
   D.goff = P.goff;
   D.xoff = P.xoff;
-  D.nnz = (double)P.fwd.size();
+  D.nnz = (double)P.nvals;
   D.tri_bytes = P.tri_bytes;
   const int nn = (int)P.p0.size();
   DevFactor& d = D.d;
@@ -363,20 +396,43 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   d.rows_off = upload(c, P.rows_off);
   d.z_off = upload(c, P.z_off);
   if (nrhs > 2) {  // MFMA sweeps: blocked copies only
-    pack_blocked(P);
+    if (on_host) {
+      pack_blocked(P);
+      d.fblk = upload(c, P.fblk);
+      d.bblk = upload(c, P.bblk);
+      std::vector<double>().swap(P.fblk);
+      std::vector<double>().swap(P.bblk);
+    } else {
+      blocked_layout(P);
+      try {
+        double* fb = dalloc<double>(c, P.fblk_off.back());
+        double* bb = dalloc<double>(c, P.bblk_off.back());
+        blocked_on_device(P, dfwd, dbwd, fb, bb, c->stream);
+        d.fblk = fb;
+        d.bblk = bb;
+      } catch (...) {
+        free_tmp();
+        throw;
+      }
+      free_tmp();
+    }
     d.fwd = nullptr;
     d.bwd = nullptr;
-    d.fblk = upload(c, P.fblk);
-    d.bblk = upload(c, P.bblk);
     d.fblk_off = upload(c, P.fblk_off);
     d.bblk_off = upload(c, P.bblk_off);
     d.fblk0 = upload(c, P.fblk0);
     d.bblk0 = upload(c, P.bblk0);
-    std::vector<double>().swap(P.fblk);
-    std::vector<double>().swap(P.bblk);
   } else {
-    d.fwd = upload(c, P.fwd);
-    d.bwd = upload(c, P.bwd);
+    if (on_host) {
+      d.fwd = upload(c, P.fwd);
+      d.bwd = upload(c, P.bwd);
+    } else {  // hand the device arrays over to the context
+      c->allocs.push_back(dfwd);
+      c->allocs.push_back(dbwd);
+      d.fwd = dfwd;
+      d.bwd = dbwd;
+      dfwd = dbwd = nullptr;
+    }
     d.fblk = d.bblk = nullptr;
     d.fblk_off = d.bblk_off = nullptr;
     d.fblk0 = d.bblk0 = nullptr;
@@ -874,11 +930,37 @@ int pecs_sizes(pecs_ctx* ctx, int64_t* ns, int64_t* ne, int64_t* np) {
 
 int pecs_stats(pecs_ctx* ctx, double* o, int n) {
   if (!ctx || !o) return -1;
-  const double v[10] = {ctx->Fs[0].nnz + ctx->Fs[1].nnz, ctx->Fe[0].nnz + ctx->Fe[1].nnz, ctx->FH.nnz,
+  const FactorDev* fs[5] = {&ctx->Fs[0], &ctx->Fs[1], &ctx->Fe[0], &ctx->Fe[1], &ctx->FH};
+  double dev_s = 0.0, dev_flops = 0.0;
+  for (const FactorDev* f : fs) {
+    dev_s += f->device_seconds;
+    dev_flops += f->device_flops;
+  }
+  const double v[12] = {ctx->Fs[0].nnz + ctx->Fs[1].nnz, ctx->Fe[0].nnz + ctx->Fe[1].nnz, ctx->FH.nnz,
                         ctx->setup_seconds, ctx->factor_seconds, (double)ctx->Fs[0].nlevels,
                         (double)ctx->Fe[0].nlevels, (double)ctx->FH.nlevels, (double)ctx->Fs[0].n,
-                        (double)ctx->FH.n};
-  for (int i = 0; i < n && i < 10; ++i) o[i] = v[i];
+                        (double)ctx->FH.n, dev_s, dev_flops};
+  for (int i = 0; i < n && i < 12; ++i) o[i] = v[i];
+  return 0;
+}
+
+int pecs_write_vtu(const pecs_mesh* mesh, const pecs_params* params, const double* u_n, const double* u_p,
+                   const double* u_r, const double* u_o, const double* poisson, int step, const char* dir,
+                   double char_length, double char_density, double char_time) {
+  try {
+    validate(mesh, params);
+    if (!u_n || !u_p || !u_r || !u_o || !poisson) throw std::runtime_error("pecs_write_vtu: null state vector");
+    if (!(char_length > 0 && char_density > 0 && char_time > 0))
+      throw std::runtime_error("pecs_write_vtu: characteristic scales must be positive");
+    SolarCellMesh m = make_solarcell_mesh(mesh->n_refine, to_geo(mesh));
+    RefTables T(params->degree);
+    PoissonLayout L = poisson_layout(m, params->degree - 1);
+    const double* u[4] = {u_n, u_p, u_r, u_o};
+    write_state_vtu(m, L, T, u, poisson, step, dir ? dir : ".", vtu_scales(char_length, char_density, char_time));
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
   return 0;
 }
 

@@ -580,7 +580,7 @@ void execute_plan_host(const FrontPlan& plan, const SupernodalFactor& F, std::ve
   (void)nn;
 }
 
-PackedFactor pack_factor(const SupernodalFactor& F) {
+PackedFactor pack_layout(const SupernodalFactor& F) {
   PackedFactor P;
   const int nn = (int)F.nodes.size();
   P.n = F.n;
@@ -608,38 +608,7 @@ PackedFactor pack_factor(const SupernodalFactor& F) {
     P.tri_bytes += 2.0 * 8.0 * (0.5 * n_ * (n_ + 1.0) + (double)m_ * n_);
   }
   P.zlen = zl;
-  P.fwd.assign(nf, 0.0);
-  P.bwd.assign(nf, 0.0);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int s = 0; s < nn; ++s) {
-    const int n_ = P.ns[s], m_ = P.ms[s];
-    const double* Li = &F.Linv[F.nodes[s].linv_off];
-    const double* L21 = &F.L21[F.nodes[s].l21_off];
-    double* f = &P.fwd[P.fwd_off[s]];
-    for (int r = 0; r < n_; ++r)
-      for (int j = 0; j <= r; ++j) f[(int64_t)r * (r + 1) / 2 + j] = Li[(int64_t)r * n_ + j];
-    // W21 = L21 Linv, 8 panel rows at a time so each Linv row is reused
-    double* W = f + (int64_t)n_ * (n_ + 1) / 2;
-    for (int i0 = 0; i0 < m_; i0 += 8) {
-      const int i1 = std::min(m_, i0 + 8);
-      for (int k = 0; k < n_; ++k) {
-        const double* Lk = Li + (int64_t)k * n_;
-        for (int i = i0; i < i1; ++i) {
-          const double a = L21[(int64_t)i * n_ + k];
-          if (a == 0.0) continue;
-          double* Wi = W + (int64_t)i * n_;
-#pragma omp simd
-          for (int j = 0; j <= k; ++j) Wi[j] += a * Lk[j];
-        }
-      }
-    }
-    double* bw = &P.bwd[P.bwd_off[s]];
-    for (int j = 0; j < n_; ++j) {
-      double* row = bw + (int64_t)j * (n_ + m_) - (int64_t)j * (j - 1) / 2;
-      for (int i = j; i < n_; ++i) row[i - j] = Li[(int64_t)i * n_ + j];
-      for (int i = 0; i < m_; ++i) row[n_ - j + i] = -W[(int64_t)i * n_ + j];
-    }
-  }
+  P.nvals = nf;
   // lanes per row: the smallest power of two >= an eighth of the mean row length
   auto lanes = [](double mean) {
     int w = 8;
@@ -698,7 +667,49 @@ PackedFactor pack_factor(const SupernodalFactor& F) {
   return P;
 }
 
-void pack_blocked(PackedFactor& P) {
+void pack_values(const SupernodalFactor& F, PackedFactor& P) {
+  const int nn = (int)P.p0.size();
+  P.fwd.assign(P.nvals, 0.0);
+  P.bwd.assign(P.nvals, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int s = 0; s < nn; ++s) {
+    const int n_ = P.ns[s], m_ = P.ms[s];
+    const double* Li = &F.Linv[F.nodes[s].linv_off];
+    const double* L21 = &F.L21[F.nodes[s].l21_off];
+    double* f = &P.fwd[P.fwd_off[s]];
+    for (int r = 0; r < n_; ++r)
+      for (int j = 0; j <= r; ++j) f[(int64_t)r * (r + 1) / 2 + j] = Li[(int64_t)r * n_ + j];
+    // W21 = L21 Linv, 8 panel rows at a time so each Linv row is reused
+    double* W = f + (int64_t)n_ * (n_ + 1) / 2;
+    for (int i0 = 0; i0 < m_; i0 += 8) {
+      const int i1 = std::min(m_, i0 + 8);
+      for (int k = 0; k < n_; ++k) {
+        const double* Lk = Li + (int64_t)k * n_;
+        for (int i = i0; i < i1; ++i) {
+          const double a = L21[(int64_t)i * n_ + k];
+          if (a == 0.0) continue;
+          double* Wi = W + (int64_t)i * n_;
+#pragma omp simd
+          for (int j = 0; j <= k; ++j) Wi[j] += a * Lk[j];
+        }
+      }
+    }
+    double* bw = &P.bwd[P.bwd_off[s]];
+    for (int j = 0; j < n_; ++j) {
+      double* row = bw + (int64_t)j * (n_ + m_) - (int64_t)j * (j - 1) / 2;
+      for (int i = j; i < n_; ++i) row[i - j] = Li[(int64_t)i * n_ + j];
+      for (int i = 0; i < m_; ++i) row[n_ - j + i] = -W[(int64_t)i * n_ + j];
+    }
+  }
+}
+
+PackedFactor pack_factor(const SupernodalFactor& F) {
+  PackedFactor P = pack_layout(F);
+  pack_values(F, P);
+  return P;
+}
+
+void blocked_layout(PackedFactor& P) {
   const int nn = (int)P.p0.size();
   for (int dir = 0; dir < 2; ++dir) {
     const bool fwd = dir == 0;
@@ -718,8 +729,18 @@ void pack_blocked(PackedFactor& P) {
     }
     b0[nn] = (int)boff.size();
     boff.push_back(total);
+  }
+}
+
+void pack_blocked(PackedFactor& P) {
+  const int nn = (int)P.p0.size();
+  blocked_layout(P);
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool fwd = dir == 0;
+    const std::vector<int>& b0 = fwd ? P.fblk0 : P.bblk0;
+    const std::vector<int64_t>& boff = fwd ? P.fblk_off : P.bblk_off;
     std::vector<double>& dst = fwd ? P.fblk : P.bblk;
-    dst.assign(total, 0.0);
+    dst.assign(boff.back(), 0.0);
     const std::vector<double>& src = fwd ? P.fwd : P.bwd;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int s = 0; s < nn; ++s) {

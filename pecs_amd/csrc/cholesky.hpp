@@ -100,6 +100,7 @@ struct PackedFactor {
   std::vector<int> frtiles, brtiles;  // split levels: row tiles (node, row0, nrows, 64 * first tile + nseg)
   std::vector<int> froff, broff;      // per level offsets into frtiles / brtiles
   std::vector<int> lnodes, lnoff;     // nodes of each level + offsets
+  int64_t nvals = 0;                  // entries of fwd (= of bwd)
   double tri_bytes = 0.0;             // bytes read by one forward + backward sweep
   // MFMA-blocked copies (see pack_blocked): per node, blocks of 16 rows; a
   // block holds its columns [clo, clo + 16 nsteps) in 16-column steps of four
@@ -109,6 +110,10 @@ struct PackedFactor {
   std::vector<int> fblk0, bblk0;            // per node: index of its first block
 };
 PackedFactor pack_factor(const SupernodalFactor& F);
+// pack_factor in two parts: every index array, offset and tile (no values;
+// needs only the symbolic factor), then fwd / bwd from Linv and L21.
+PackedFactor pack_layout(const SupernodalFactor& F);
+void pack_values(const SupernodalFactor& F, PackedFactor& P);
 
 // Build the MFMA-blocked copies from fwd / bwd and release the row-major
 // arrays.  Block b of node s covers node rows [16b, 16b + 16) (forward: the
@@ -118,6 +123,9 @@ PackedFactor pack_factor(const SupernodalFactor& F);
 // l = i + 16 k) is row 16b + i, column clo + 16 st + 4 k + q, zero outside the
 // row.  A wave streaming one block reads 512 contiguous bytes per load.
 void pack_blocked(PackedFactor& P);
+// The block offsets of pack_blocked alone (fblk0 / fblk_off / bblk0 /
+// bblk_off; the last offset is the total length).
+void blocked_layout(PackedFactor& P);
 inline int blk_clo(bool fwd, int b) { return fwd ? 0 : 16 * b; }
 inline int blk_chi(bool fwd, int b, int ns, int ms) {
   if (!fwd) return ns + ms;

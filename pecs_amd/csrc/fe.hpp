@@ -96,6 +96,7 @@ struct RefTables {
   std::vector<double> psix, psiy, divpsi;  // [nrt*nq]
   std::vector<double> phiP;                // [nphi*nq]
   std::vector<double> fpsix, fpsiy;        // [4*nrt*nfq]
+  std::vector<double> rtX, rtG;            // RT nodes: deg+2 equispaced (normal), deg+1 Gauss (tangential)
 
   explicit RefTables(int kk) { build(kk); }
 
@@ -148,31 +149,13 @@ struct RefTables {
     nrt = 2 * nrtx;
     nphi = (deg + 1) * (deg + 1);
     nlocP = nrt + nphi;
-    std::vector<double> X(deg + 2), G, GW;
-    for (int i = 0; i < deg + 2; ++i) X[i] = (double)i / (deg + 1);
-    gauss01(deg + 1, G, GW);
+    rtX.assign(deg + 2, 0.0);
+    for (int i = 0; i < deg + 2; ++i) rtX[i] = (double)i / (deg + 1);
+    std::vector<double> GW;
+    gauss01(deg + 1, rtG, GW);
     auto pnodes = equispaced(deg);
     psix.assign(nrt * nq, 0); psiy.assign(nrt * nq, 0); divpsi.assign(nrt * nq, 0);
     fpsix.assign(4 * nrt * nfq, 0); fpsiy.assign(4 * nrt * nfq, 0);
-    auto rt_eval = [&](int i, double xi, double eta, double& vx, double& vy, double& dv) {
-      vx = vy = dv = 0.0;
-      if (i < nrtx) {
-        int a = i % (deg + 2), b = i / (deg + 2);
-        double l, dl, gg, dg;
-        lagrange(X, a, xi, l, dl);
-        lagrange(G, b, eta, gg, dg);
-        vx = l * gg;
-        dv = dl * gg;
-      } else {
-        int ii = i - nrtx;
-        int a = ii % (deg + 1), b = ii / (deg + 1);
-        double gg, dg, l, dl;
-        lagrange(G, a, xi, gg, dg);
-        lagrange(X, b, eta, l, dl);
-        vy = gg * l;
-        dv = gg * dl;
-      }
-    };
     for (int i = 0; i < nrt; ++i) {
       for (int q = 0; q < nq; ++q) {
         double vx, vy, dv;
@@ -199,6 +182,27 @@ struct RefTables {
           lagrange(pnodes, b, qy[q], vy, dy);
           phiP[(a + (deg + 1) * b) * nq + q] = vx * vy;
         }
+  }
+
+  // Reference RT basis function i at (xi, eta): value (vx, vy) and divergence.
+  void rt_eval(int i, double xi, double eta, double& vx, double& vy, double& dv) const {
+    vx = vy = dv = 0.0;
+    if (i < nrtx) {
+      int a = i % (deg + 2), b = i / (deg + 2);
+      double l, dl, gg, dg;
+      lagrange(rtX, a, xi, l, dl);
+      lagrange(rtG, b, eta, gg, dg);
+      vx = l * gg;
+      dv = dl * gg;
+    } else {
+      int ii = i - nrtx;
+      int a = ii % (deg + 1), b = ii / (deg + 1);
+      double gg, dg, l, dl;
+      lagrange(rtG, a, xi, gg, dg);
+      lagrange(rtX, b, eta, l, dl);
+      vy = gg * l;
+      dv = gg * dl;
+    }
   }
 
   // Local RT index of face-node m of face f (the DoF carrying the flux there)

@@ -297,6 +297,212 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, int KS, 
     for (int h = 0; h < nrhs; ++h) x[h * n + p] = X[p * NRP + h];
 }
 
+// CPU emulation of factor_on_device / blocked_on_device: the kernels' index
+// arithmetic thread by thread (bounds-checked) and the dense calls in their
+// column-major Upper meaning, written as plain loops.
+#define DEV_CHECK(cond, what)                                                  \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      std::fprintf(stderr, "device factor emulation: out of bounds: %s\n", what); \
+      std::exit(3);                                                            \
+    }                                                                          \
+  } while (0)
+
+static void device_factor_emulate(const FrontPlan& plan, const SupernodalFactor& F, const PackedFactor& P,
+                                  std::vector<double>& fwd, std::vector<double>& bwd) {
+  const int nn = (int)F.nodes.size();
+  const int64_t FT = plan.front_total;
+  std::vector<double> fr(FT, 0.0);
+  fwd.assign(P.nvals, 0.0);
+  bwd.assign(P.nvals, 0.0);
+  auto FR = [&](int64_t i) -> double& {
+    DEV_CHECK(i >= 0 && i < FT, "front");
+    return fr[i];
+  };
+  auto FW = [&](int64_t i) -> double& {
+    DEV_CHECK(i >= 0 && i < P.nvals, "fwd");
+    return fwd[i];
+  };
+  auto BW = [&](int64_t i) -> double& {
+    DEV_CHECK(i >= 0 && i < P.nvals, "bwd");
+    return bwd[i];
+  };
+  for (int l = 0; l < F.nlevels; ++l) {
+    // k_front_scatter: 64 lanes per item
+    for (int it = plan.sc_level[l]; it < plan.sc_level[l + 1]; ++it) {
+      const int nx = plan.sc_nx[it], ny = plan.sc_ny[it], nf = plan.sc_nf[it];
+      for (int t = 0; t < 64; ++t)
+        for (int e = t; e < nx * ny; e += 64) {
+          const int a = e / ny, b = e - a * ny;
+          DEV_CHECK(plan.sc_src[it] + e < (int64_t)plan.svals.size(), "svals");
+          const double v = plan.svals[plan.sc_src[it] + e];
+          if (plan.sc_mode[it] == 0) {
+            if (plan.sc_rx[it] + a >= plan.sc_ry[it] + b) FR(plan.sc_dst[it] + (int64_t)a * nf + b) = v;
+          } else {
+            FR(plan.sc_dst[it] + (int64_t)b * nf + a) = v;
+          }
+        }
+    }
+    // k_extend_add, two passes
+    for (int pass = 0; pass < 2; ++pass)
+      for (int it = plan.ea_level[2 * l + pass]; it < plan.ea_level[2 * l + pass + 1]; ++it) {
+        const int c = plan.ea_child[it], r = plan.ea_row[it], s = plan.ea_parent[it];
+        const int nsc = P.ns[c], nfc = nsc + P.ms[c], nfp = P.ns[s] + P.ms[s];
+        DEV_CHECK(r < P.ms[c], "child row");
+        const int* m = &plan.ea_map[plan.ea_map_off[c]];
+        DEV_CHECK(plan.ea_map_off[c] + P.ms[c] <= plan.ea_map_off[c + 1], "map");
+        for (int t = 0; t < 64; ++t)
+          for (int q = t; q <= r; q += 64) {
+            DEV_CHECK(m[r] >= m[q] && m[r] < nfp, "map lower triangle");
+            FR(plan.front_off[s] + (int64_t)m[r] * nfp + m[q]) += FR(plan.front_off[c] + (int64_t)(nsc + r) * nfc + nsc + q);
+          }
+      }
+    // dense calls, column-major with leading dimension nf
+    for (int s : F.level_nodes[l]) {
+      const int ns = P.ns[s], ms = P.ms[s], nf = ns + ms;
+      if (ns == 0) continue;
+      const int64_t o = plan.front_off[s];
+      DEV_CHECK(o + (int64_t)nf * nf <= FT, "front extent");
+      auto A = [&](int i, int j) -> double& { return FR(o + i + (int64_t)j * nf); };
+      // potrf(Upper): A11 = U^T U
+      for (int j = 0; j < ns; ++j) {
+        double d = A(j, j);
+        for (int k = 0; k < j; ++k) d -= A(k, j) * A(k, j);
+        if (!(d > 0)) {
+          std::printf("emulated potrf: not positive definite\n");
+          std::exit(2);
+        }
+        A(j, j) = std::sqrt(d);
+        for (int i = j + 1; i < ns; ++i) {
+          double v = A(j, i);
+          for (int k = 0; k < j; ++k) v -= A(k, j) * A(k, i);
+          A(j, i) = v / A(j, j);
+        }
+      }
+      // trsm(Left, Upper, Transpose): U^T X = A12
+      for (int jc = 0; jc < ms; ++jc)
+        for (int i = 0; i < ns; ++i) {
+          double v = A(i, ns + jc);
+          for (int k = 0; k < i; ++k) v -= A(k, i) * A(k, ns + jc);
+          A(i, ns + jc) = v / A(i, i);
+        }
+      // syrk(Upper, Transpose, -1, 1): A22 -= A12^T A12
+      for (int j = 0; j < ms; ++j)
+        for (int i = 0; i <= j; ++i) {
+          double v = 0;
+          for (int k = 0; k < ns; ++k) v += A(k, ns + i) * A(k, ns + j);
+          A(ns + i, ns + j) -= v;
+        }
+      // trtri(Upper): A11 = U^-1 (column by column, back substitution)
+      std::vector<double> Ui((size_t)ns * ns, 0.0);
+      for (int j = 0; j < ns; ++j) {
+        Ui[j + (size_t)j * ns] = 1.0 / A(j, j);
+        for (int i = j - 1; i >= 0; --i) {
+          double v = 0;
+          for (int k = i + 1; k <= j; ++k) v += A(i, k) * Ui[k + (size_t)j * ns];
+          Ui[i + (size_t)j * ns] = -v / A(i, i);
+        }
+      }
+      for (int j = 0; j < ns; ++j)
+        for (int i = 0; i <= j; ++i) A(i, j) = Ui[i + (size_t)j * ns];
+      // trmm(Left, Upper, None): W = U^-1 A12 into fwd, ldc = ns
+      const int64_t wo = P.fwd_off[s] + (int64_t)ns * (ns + 1) / 2;
+      for (int jc = 0; jc < ms; ++jc)
+        for (int i = 0; i < ns; ++i) {
+          double v = 0;
+          for (int k = i; k < ns; ++k) v += A(i, k) * A(k, ns + jc);
+          FW(wo + i + (int64_t)jc * ns) = v;
+        }
+    }
+  }
+  // k_pack_rows: items (node, 16-row chunk), 256 threads, LDS tile t[16][17]
+  for (int s = 0; s < nn; ++s)
+    for (int j0 = 0; j0 < P.ns[s]; j0 += 16) {
+      const int ns = P.ns[s], ms = P.ms[s], nf = ns + ms;
+      const int64_t Fo = plan.front_off[s], fo = P.fwd_off[s], Wo = fo + (int64_t)ns * (ns + 1) / 2, bo = P.bwd_off[s];
+      const int jend = std::min(j0 + 16, ns);
+      for (int r = j0; r < jend; ++r)
+        for (int t = 0; t < 256; ++t)
+          for (int c = t; c <= r; c += 256) FW(fo + (int64_t)r * (r + 1) / 2 + c) = FR(Fo + (int64_t)r * nf + c);
+      double tile[16][17];
+      for (int i0 = j0; i0 < nf; i0 += 16) {
+        for (int t = 0; t < 256; ++t) {
+          const int a = t & 15, b = t >> 4, j = j0 + a, i = i0 + b;
+          double v = 0.0;
+          if (j < ns && i >= j && i < nf) v = i < ns ? FR(Fo + (int64_t)i * nf + j) : -FW(Wo + (int64_t)(i - ns) * ns + j);
+          tile[a][b] = v;
+        }
+        for (int t = 0; t < 256; ++t) {
+          const int a = t & 15, b = t >> 4, j = j0 + b, i = i0 + a;
+          if (j < ns && i >= j && i < nf) {
+            DEV_CHECK((int64_t)j * nf - (int64_t)j * (j - 1) / 2 + (i - j) < (int64_t)ns * (ns + 1) / 2 + (int64_t)ns * ms,
+                      "bwd row within node");
+            BW(bo + (int64_t)j * nf - (int64_t)j * (j - 1) / 2 + (i - j)) = tile[b][a];
+          }
+        }
+      }
+    }
+}
+
+// k_pack_blocked, thread by thread.
+static void blocked_emulate(const PackedFactor& P, const std::vector<double>& fwd, const std::vector<double>& bwd,
+                            std::vector<double>& fblk, std::vector<double>& bblk) {
+  const int nn = (int)P.p0.size();
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool fw = dir == 0;
+    const std::vector<int>& b0 = fw ? P.fblk0 : P.bblk0;
+    const std::vector<int64_t>& boff = fw ? P.fblk_off : P.bblk_off;
+    std::vector<double>& dst = fw ? fblk : bblk;
+    const std::vector<double>& src = fw ? fwd : bwd;
+    dst.assign(boff.back(), -1.0);  // every entry must be written
+    for (int s = 0; s < nn; ++s)
+      for (int gb = b0[s]; gb < b0[s + 1]; ++gb) {
+        const int b = gb - b0[s], ns = P.ns[s], ms = P.ms[s];
+        const int clo = fw ? 0 : 16 * b;
+        int chi;
+        if (fw) {
+          const int last = std::min(16 * b + 15, ns + ms - 1);
+          chi = last < ns ? last + 1 : ns;
+        } else {
+          chi = ns + ms;
+        }
+        const int nsteps = (chi - clo + 15) / 16;
+        DEV_CHECK(boff[gb] + (int64_t)nsteps * 256 == boff[gb + 1], "block extent");
+        const int rows = fw ? ns + ms : ns;
+        const int64_t so = fw ? P.fwd_off[s] : P.bwd_off[s];
+        for (int t = 0; t < 256; ++t)
+          for (int e = t; e < nsteps * 256; e += 256) {
+            const int st = e >> 8, q = (e >> 6) & 3, lane = e & 63, i = lane & 15, k = lane >> 4;
+            const int lr = 16 * b + i, col = clo + 16 * st + 4 * k + q;
+            double v = 0.0;
+            if (lr < rows) {
+              int64_t off;
+              int rs, re;
+              if (fw) {
+                rs = 0;
+                if (lr < ns) {
+                  off = (int64_t)lr * (lr + 1) / 2;
+                  re = lr + 1;
+                } else {
+                  off = (int64_t)ns * (ns + 1) / 2 + (int64_t)(lr - ns) * ns;
+                  re = ns;
+                }
+              } else {
+                off = (int64_t)lr * (ns + ms) - (int64_t)lr * (lr - 1) / 2 - lr;
+                rs = lr;
+                re = ns + ms;
+              }
+              if (col >= rs && col < re) {
+                DEV_CHECK(so + off + col >= 0 && so + off + col < (int64_t)src.size(), "blocked source");
+                v = src[so + off + col];
+              }
+            }
+            dst[boff[gb] + e] = v;
+          }
+      }
+  }
+}
+
 static double check(const char* name, const BlockSPD& S, int leaf) {
   auto t0 = std::chrono::steady_clock::now();
   SupernodalFactor F = cholesky_factor(S, leaf);
@@ -313,6 +519,31 @@ static double check(const char* name, const BlockSPD& S, int leaf) {
                 d, plan.front_total * 8e-6, plan.sc_src.size(), plan.ea_child.size());
     if (d != 0.0 || Li.size() != F.Linv.size() || L2.size() != F.L21.size()) {
       std::printf("front plan mismatch\n");
+      std::exit(2);
+    }
+    // the device numeric phase and its sweep-layout kernels, emulated
+    PackedFactor PL = pack_layout(G);
+    std::vector<double> ef, eb;
+    device_factor_emulate(plan, G, PL, ef, eb);
+    PackedFactor PR = pack_factor(F);
+    double df = 0, mx = 0;
+    for (int64_t i = 0; i < PL.nvals; ++i) {
+      df = std::max(df, std::max(std::fabs(ef[i] - PR.fwd[i]), std::fabs(eb[i] - PR.bwd[i])));
+      mx = std::max(mx, std::fabs(PR.fwd[i]));
+    }
+    PL.fwd = ef;
+    PL.bwd = eb;
+    PackedFactor PB2 = PL;
+    pack_blocked(PB2);
+    blocked_layout(PL);
+    std::vector<double> fb, bb;
+    blocked_emulate(PL, ef, eb, fb, bb);
+    const bool blk_same = fb == PB2.fblk && bb == PB2.bblk && PL.fblk_off == PB2.fblk_off && PL.bblk_off == PB2.bblk_off;
+    std::printf("   device factor emulation: max |diff| of the sweep layout to pack_factor %.3e (max |entry| %.3e); "
+                "blocked copies %s\n",
+                df, mx, blk_same ? "identical" : "DIFFER");
+    if (!(df <= 1e-10 * mx) || !blk_same) {
+      std::printf("device factor emulation mismatch\n");
       std::exit(2);
     }
   }

@@ -3,6 +3,7 @@
 //
 //   pecs_run [input_file.prm] [--degree K] [--steps N] [--steady-tol TOL]
 //            [--check-every M] [--out DIR] [--ignore-local-refinement] [--host-loop]
+//            [--no-vtu]
 //
 // Without --steps the step count is the reference's: `time < timeStamps[k]`
 // with `time += delta_t` in double arithmetic, replayed on the host so the
@@ -16,7 +17,11 @@
 // reference's message frame and exit status 1 (main.cpp:28-51).  --host-loop
 // runs the loop body through the per-call host-vector interface
 // (assemble_*_rhs / solve_full_system / solve_Poisson, one C-ABI call each)
-// instead of the device-resident pecs_step.
+// instead of the device-resident pecs_step.  As print_results does, the
+// initial state and the state at every output time are written as
+// Semiconductor-NNN.vtu / Electrolyte-NNN.vtu / Poisson-NNN.vtu in DIR
+// (NNN counting from 0, or from the number of outputs on a restart,
+// SolarCell.cpp:1991-2038); --no-vtu skips them.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -42,6 +47,7 @@ struct Cli {
   std::string out = ".";
   bool ignore_local = false;
   bool host_loop = false;
+  bool vtu = true;
 };
 
 Cli parse(int argc, char** argv) {
@@ -59,9 +65,10 @@ Cli parse(int argc, char** argv) {
     else if (a == "--out") c.out = next();
     else if (a == "--ignore-local-refinement") c.ignore_local = true;
     else if (a == "--host-loop") c.host_loop = true;
+    else if (a == "--no-vtu") c.vtu = false;
     else if (a == "-h" || a == "--help") {
       std::printf("usage: pecs_run [input_file.prm] [--degree K] [--steps N] [--steady-tol TOL] "
-                  "[--check-every M] [--out DIR] [--ignore-local-refinement] [--host-loop]\n");
+                  "[--check-every M] [--out DIR] [--ignore-local-refinement] [--host-loop] [--no-vtu]\n");
       std::exit(0);
     } else if (!a.empty() && a[0] == '-') throw std::runtime_error("unknown option " + a);
     else c.prm = a;
@@ -80,6 +87,9 @@ int run(const Cli& cli) {
   Parameters prm;
   prm.mesh = rs.mesh;
   prm.p = rs.params;
+  prm.char_length = rs.char_length;
+  prm.char_density = rs.char_density;
+  prm.char_time = rs.char_time;
   SolarCellProblem sim(prm);
   double stats[8] = {0};
   pecs_stats(sim.ctx(), stats, 8);
@@ -102,16 +112,28 @@ int run(const Cli& cli) {
     std::printf("running until t = %g\n", rs.t_end);
   }
   if (cli.steps >= 0) counts.assign(1, cli.steps);
+  unsigned time_step_number = rs.restart ? (unsigned)rs.time_stamps : 0u;
+  double print_s = 0;  // output time, kept out of the step rate
+  auto print = [&](bool pull) {
+    if (!cli.vtu) return;
+    const auto tp = std::chrono::steady_clock::now();
+    if (pull) sim.pull_state();
+    sim.print_results(time_step_number++, cli.out);
+    print_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
+  };
+  print(false);
 
   long done = 0;
   bool steady = false;
   double res = -1;
+  print_s = 0;
   const auto t0 = std::chrono::steady_clock::now();
   for (size_t k = 0; k < counts.size() && !steady; ++k) {
     long left = counts[k];
     if (cli.host_loop) {
       for (; left > 0; --left, ++done) sim.step_host();
       std::printf("output %zu: step %ld\n", k + 1, done);
+      print(false);
       continue;
     }
     while (left > 0 && !steady) {
@@ -125,8 +147,9 @@ int run(const Cli& cli) {
       steady = cli.steady_tol > 0 && res <= cli.steady_tol;
     }
     std::printf("output %zu: step %ld, steady residual %.3e\n", k + 1, done, res);
+    print(true);
   }
-  const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() - print_s;
   if (!cli.host_loop) sim.pull_state();
   sim.electron_hole_pair.print_dofs(cli.out);
   sim.redox_pair.print_dofs(cli.out);

@@ -139,6 +139,7 @@ struct RunSettings {
   pecs_params params{};
   pecs_scaled scaled{};
   double t_end = 0, t_end_2 = 0;
+  double char_length = 1, char_density = 1, char_time = 1;
   int time_stamps = 0, local_refinements = 0;
   bool restart = false;
 };
@@ -153,6 +154,9 @@ inline RunSettings settings_from(const ParameterFile& f, int degree) {
   ph.char_length = f.get_double("physical", "characteristic length");
   ph.char_time = f.get_double("physical", "characteristic time");
   ph.char_density = f.get_double("physical", "characteristic density");
+  r.char_length = ph.char_length;
+  r.char_time = ph.char_time;
+  r.char_density = ph.char_density;
   ph.electron_mobility = f.get_double("electrons", "mobility");
   ph.hole_mobility = f.get_double("holes", "mobility");
   ph.reductant_mobility = f.get_double("reductants", "mobility");

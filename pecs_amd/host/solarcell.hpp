@@ -59,6 +59,8 @@ struct Parameters {  // scaled values, Parameters.hpp:25-80 after scaling
   pecs_mesh mesh{0.5, 0.5, 1.0, 1.0, 4, 0, 0};
   pecs_params p{};
   int time_steps = 200;
+  // characteristic scales (Parameters.hpp:65-67), used only to rescale output
+  double char_length = 1.0, char_density = 1.0, char_time = 1.0;
   Parameters() {
     p.degree = 1;
     p.dt = 1e-3;
@@ -249,6 +251,19 @@ class SolarCellProblem {
                          electron_hole_pair.carrier_2.solution.data(), redox_pair.carrier_1.solution.data(),
                          redox_pair.carrier_2.solution.data(), Poisson_object.solution.data()),
           ctx_);
+  }
+
+  // print_results (SolarCell.cpp:1826-1858): Semiconductor-NNN.vtu,
+  // Electrolyte-NNN.vtu and Poisson-NNN.vtu in dir from the members' solution
+  // vectors (call pull_state first after device-resident steps).
+  void print_results(unsigned time_step_number, const std::string& dir = ".") {
+    timer.enter("Printing");
+    check(pecs_write_vtu(&prm_.mesh, &prm_.p, electron_hole_pair.carrier_1.solution.data(),
+                         electron_hole_pair.carrier_2.solution.data(), redox_pair.carrier_1.solution.data(),
+                         redox_pair.carrier_2.solution.data(), Poisson_object.solution.data(), (int)time_step_number,
+                         dir.c_str(), prm_.char_length, prm_.char_density, prm_.char_time),
+          nullptr);
+    timer.leave("Printing");
   }
 
   pecs_ctx* ctx() const { return ctx_; }

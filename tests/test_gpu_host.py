@@ -121,7 +121,11 @@ def test_cli_run_restart_and_host_loop(tmp_path):
     assert "output 2: step" in out
     steps = int(out.split("output 2: step")[1].split(",")[0])
     _say(f"[gpu] pecs_run host loop, {steps} steps")
-    _run([prm, "--degree", degree, "--out", host, "--host-loop"])
+    _run([prm, "--degree", degree, "--out", host, "--host-loop", "--no-vtu"])
+    # print_results: the initial state and each of the 2 output times
+    assert sorted(f.name for f in dev.glob("*.vtu")) == sorted(
+        f"{stem}-{i:03d}.vtu" for stem in ("Semiconductor", "Electrolyte", "Poisson") for i in range(3))
+    assert not list(host.glob("*.vtu"))
 
     p = _oracle_params(prm, degree)
     o = OracleSolarCell(p)
@@ -131,12 +135,24 @@ def test_cli_run_restart_and_host_loop(tmp_path):
         h = _read_dofs(host / (name + ".dofs"))
         assert _rel(d, st[c]) < 1e-10, (name, _rel(d, st[c]))
         assert _rel(h, d) < 1e-12, (name, _rel(h, d))
+    # the last VTU holds the final state: densities at the cell corners are
+    # the corner nodal values of the .dofs vectors (degree 1: all 4 nodes)
+    import xml.etree.ElementTree as ET
+    for stem, pair in (("Semiconductor", ("n", "p")), ("Electrolyte", ("r", "o"))):
+        piece = ET.parse(dev / f"{stem}-002.vtu").getroot().find("UnstructuredGrid/Piece")
+        arrays = {a.get("Name"): np.array(a.text.split(), float) for a in piece.find("PointData")}
+        for c in pair:
+            d = _read_dofs(dev / (NAMES[c] + ".dofs"))
+            nc = len(d) // 12
+            corners = d[8 * nc:].reshape(nc, 4)[:, [0, 1, 2, 3]].reshape(-1)
+            assert np.allclose(arrays[NAMES[c] + " Density"], corners, rtol=1e-14, atol=0), (stem, c)
 
     _say("[gpu] pecs_run restart from the written .dofs")
     prm.write_text(PRM.format(restart="true"))
     out = _run([prm, "--degree", degree, "--out", dev])
     steps2 = int(out.split("output 2: step")[1].split(",")[0])
     assert steps2 >= 2
+    assert (dev / "Poisson-004.vtu").exists() and not (dev / "Poisson-005.vtu").exists()
     st2 = o.run(steps2, st)
     for c, name in NAMES.items():
         d = _read_dofs(dev / (name + ".dofs"))
