@@ -23,6 +23,11 @@ workload (refinement 4, k = 1, dt = 1e-3).
 batched: an I-V sweep of 16 applied biases on the measured mesh stepped
 together (one read of each factor per sweep for all of them, FP64 MFMA):
 aggregate step throughput and time to steady state per bias point.
+
+sweep (--sweep N): an uneven-mesh sweep (refinements 5-8, conic geometries,
+drawn biases, generation and doping), one context per instance, stepped in
+waves by graph launches; time to steady state per bias point against the
+plain path (each instance alone, launch by launch).
 """
 import argparse
 import json
@@ -177,6 +182,105 @@ def batched_leg(args, workload):
     return out
 
 
+def sweep_leg(args):
+    """Uneven-mesh I-V sweep: args.sweep instances with refinement drawn from
+    {args.sweep_rmin..args.sweep_rmax}, conic half-angle U[0, 15] degrees
+    (r2 = r1 + H tan(angle), L = 1.5), applied bias U[-0.5, 0.5], generation
+    amplitude logU[0.1, 10], donor doping logU[0.5, 5] (seeded).  Each instance
+    is its own context (its own mesh, factors and tuned tile shapes); the
+    instances that fit in HBM at once form a wave, stepped together by one
+    graph launch per stream and step (pecs_sweep_*) until each reaches the
+    steady tolerance (it then leaves the sweep).  The plain path is each
+    instance stepped alone, launch by launch: its time per step is measured
+    per instance and multiplied by the instance's steps to steady state."""
+    import numpy as np
+    import torch
+    import pecs_amd
+    rng = np.random.default_rng(args.sweep_seed)
+    draws = []
+    for i in range(args.sweep):
+        r = int(rng.integers(args.sweep_rmin, args.sweep_rmax + 1))
+        ang = float(rng.uniform(0.0, 15.0))
+        p = pecs_amd.workload_params(r, args.degree, args.dt)
+        p.update(r1=0.5, r2=0.5 + float(np.tan(np.radians(ang))), L=1.5, phi_bi=0.4,
+                 phi_app=float(rng.uniform(-0.5, 0.5)), G0=float(10 ** rng.uniform(-1, 1)),
+                 ne=float(10 ** rng.uniform(np.log10(0.5), np.log10(5.0))))
+        draws.append((r, ang, p))
+    order = sorted(range(len(draws)), key=lambda i: -draws[i][0])
+    reserve = args.sweep_reserve_gib * (1 << 30)
+    total_setup = total_wall = total_plain = 0.0
+    steps = [None] * len(draws)
+    plain_ms = [None] * len(draws)
+    waves = []
+    pos = 0
+    while pos < len(order):
+        members, idx = [], []
+        t0 = time.perf_counter()
+        while pos < len(order):
+            free, _ = torch.cuda.mem_get_info()
+            if members and free < reserve:
+                break
+            i = order[pos]
+            members.append(pecs_amd.SolarCellGPU(draws[i][2]))
+            idx.append(i)
+            pos += 1
+            print(f"[bench] sweep: instance {i} (refinement {draws[i][0]}) set up, {free / (1 << 30):.0f} GiB "
+                  f"were free", file=sys.stderr, flush=True)
+        setup = time.perf_counter() - t0
+        total_setup += setup
+        # plain path: each member alone, launch by launch
+        for g, i in zip(members, idx):
+            g.initial_state()
+            g.use_graph(False)
+            g.step(2)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            g.step(args.sweep_plain_steps)
+            torch.cuda.synchronize()
+            plain_ms[i] = (time.perf_counter() - t1) / args.sweep_plain_steps * 1e3
+            g.use_graph(True)
+            g.initial_state()
+        sw = pecs_amd.SolarCellSweep(members, args.sweep_streams)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        active = np.ones(len(members), bool)
+        n = 0
+        while active.any() and n < args.steady_max_steps:
+            k = min(args.steady_check_every, args.steady_max_steps - n)
+            sw.step(k)
+            n += k
+            r = np.array([g.residual() for g in members])
+            newly = active & (r <= args.steady_tol)
+            for j in np.nonzero(newly)[0]:
+                steps[idx[j]] = n
+            if newly.any():
+                active &= ~newly
+                sw.set_active(active)
+            if n % 500 == 0:
+                print(f"[bench] sweep: step {n}, {int(active.sum())} of {len(members)} still stepping",
+                      file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t1
+        total_wall += wall
+        for j in np.nonzero(active)[0]:
+            steps[idx[j]] = -n
+        sw.close()
+        for g in members:
+            g.close()
+        waves.append({"instances": len(members), "refinements": [draws[i][0] for i in idx],
+                      "setup_s": round(setup, 2), "sweep_s": round(wall, 3), "steps": n})
+        print(f"[bench] sweep wave {len(waves)}: {waves[-1]}", file=sys.stderr, flush=True)
+    for i in range(len(draws)):
+        total_plain += abs(steps[i]) * plain_ms[i] * 1e-3
+    N = len(draws)
+    return {"instances": N, "refinements": [d[0] for d in draws], "half_angles_deg": [round(d[1], 2) for d in draws],
+            "streams": args.sweep_streams, "waves": waves, "setup_s": round(total_setup, 2),
+            "tol": args.steady_tol, "steps_to_steady": steps,
+            "reached": all(s is not None and s > 0 for s in steps),
+            "s_per_bias_point": total_wall / N, "plain_s_per_bias_point": total_plain / N,
+            "speedup_vs_plain": total_plain / total_wall if total_wall > 0 else None,
+            "plain_ms_per_step": [round(m, 4) for m in plain_ms]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +298,17 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="also run an I-V sweep of this many bias points (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="uneven-mesh sweep of this many instances (refinement, cone angle, bias, generation and "
+                         "doping drawn per instance); 0: skip")
+    ap.add_argument("--sweep-only", action="store_true", help="run only the uneven-mesh sweep")
+    ap.add_argument("--sweep-rmin", type=int, default=5)
+    ap.add_argument("--sweep-rmax", type=int, default=8)
+    ap.add_argument("--sweep-seed", type=int, default=20261202)
+    ap.add_argument("--sweep-streams", type=int, default=4)
+    ap.add_argument("--sweep-plain-steps", type=int, default=20)
+    ap.add_argument("--sweep-reserve-gib", type=float, default=64.0,
+                    help="stop adding instances to a wave when less device memory than this is free")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -215,6 +330,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n}), flush=True)
+        return
+    if args.sweep_only:
+        print(json.dumps({"sweep": sweep_leg(args)}), flush=True)
         return
     if args.batch_only:
         wl = (f"PECS half cell, refinement {args.refine} ({1 << args.refine}x{1 << args.refine} quads/region), "
@@ -246,7 +364,14 @@ def main():
     ph = (ctypes.c_double * 8)()
     nst = ctypes.c_int()
     L.pecs_phase_times(g.ctx, ph, 8, ctypes.byref(nst))
+    L.pecs_enable_timing(g.ctx, 0)
     phase = [ph[i] / max(nst.value, 1) for i in range(8)]
+    # the same number of steps replayed as captured graphs (no timing events)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t0) / args.steps * 1e3
     names = ["rhs_semi", "rhs_elec", "schur_transforms", "carrier_cholesky", "recoveries", "poisson_cond",
              "chol_poisson", "poisson_recover"]
     print("[bench] per-step phase ms: " + ", ".join(f"{n}={v:.3f}" for n, v in zip(names, phase)),
@@ -285,6 +410,7 @@ def main():
                    "dofs_per_step": U, "n_carrier_semi": g.n_semi, "n_carrier_elec": g.n_elec,
                    "n_poisson": g.n_poisson, "setup_s": round(setup_s, 2),
                    "steady_residual_last_step": res,
+                   "ms_per_step_graph_replay": ms_graph,
                    "factor_stats": {k: v for k, v in st.items() if k not in ("setup_s",)},
                    "phase_ms_per_step": {n: round(v, 4) for n, v in zip(names, phase)}},
         "roofline": {"kernel": "supernodal triangular sweeps (fwd+bwd) with the semiconductor and electrolyte "
@@ -315,6 +441,8 @@ def main():
               flush=True)
     if args.batch > 0:
         out["batched"] = batched_leg(args, workload)
+    if args.sweep > 0:
+        out["sweep"] = sweep_leg(args)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

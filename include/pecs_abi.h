@@ -163,6 +163,26 @@ int pecs_step(pecs_ctx* ctx, int nsteps);
 int pecs_last_residual(pecs_ctx* ctx, double* residual);       /* max over the instances */
 int pecs_last_residual_batch(pecs_ctx* ctx, double* residual);  /* one per instance */
 
+/* Uneven-mesh sweeps: contexts with different meshes (refinement, geometry)
+ * stepped together.  Each time step launches every active member's captured
+ * step graph; members are spread round-robin over nstreams (1..4) streams, so
+ * their steps overlap.  pecs_sweep_step
+ * updates every active member's steady residual (pecs_last_residual[_batch]);
+ * set_active takes one flag per member (0: left out of later steps, e.g. once
+ * steady).  The members must not be used elsewhere during pecs_sweep_step.
+ * Errors: pecs_global_error().  No reference counterpart (the reference runs
+ * one problem per process). */
+typedef struct pecs_sweep pecs_sweep;
+int pecs_sweep_create(pecs_ctx** ctxs, int n, int nstreams, pecs_sweep** out);
+int pecs_sweep_set_active(pecs_sweep* sweep, const int* active);
+int pecs_sweep_step(pecs_sweep* sweep, int nsteps);
+int pecs_sweep_destroy(pecs_sweep* sweep);
+
+/* pecs_step replays one time step captured as a HIP graph (captured at the
+ * first call; kernels and stream dependencies only).  enable = 0 launches the
+ * kernels one by one instead; the results are identical.  No reference
+ * counterpart (launch mechanics of this library). */
+int pecs_use_graph(pecs_ctx* ctx, int enable);
 /* Per-phase timing of pecs_step with hipEvents on the context stream
  * (the TimerOutput sections of SolarCell.cpp:1905-2087): phases 0 rhs semi,
  * 1 rhs elec, 2 Schur transforms of both regions, 3 carrier Cholesky solves of

@@ -25,7 +25,8 @@ ABI_FUNCTIONS = [
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config",
     "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
-    "pecs_last_residual_batch", "pecs_write_vtu",
+    "pecs_last_residual_batch", "pecs_write_vtu", "pecs_use_graph", "pecs_sweep_create", "pecs_sweep_set_active",
+    "pecs_sweep_step", "pecs_sweep_destroy",
 ]
 
 
@@ -330,7 +331,59 @@ class SolarCellGPU:
     def step(self, nsteps=1):
         self._c(lib().pecs_step(self.ctx, int(nsteps)))
 
+    def enable_timing(self, enable):
+        """Per-phase hipEvent timing of pecs_step (steps then launch kernel by kernel)."""
+        L = lib()
+        L.pecs_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self._c(L.pecs_enable_timing(self.ctx, int(bool(enable))))
+
+    def use_graph(self, enable):
+        """Replay the step as a captured HIP graph (default) or launch kernel by kernel."""
+        L = lib()
+        L.pecs_use_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self._c(L.pecs_use_graph(self.ctx, int(bool(enable))))
+
     def residual(self):
         r = ctypes.c_double()
         self._c(lib().pecs_last_residual(self.ctx, ctypes.byref(r)))
         return r.value
+
+
+class SolarCellSweep:
+    """Contexts with different meshes stepped together (pecs_sweep_*): every
+    active member's captured step graph per time step, spread over nstreams
+    streams so that the members' steps overlap."""
+
+    def __init__(self, members, nstreams=1):
+        L = lib()
+        L.pecs_sweep_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.pecs_sweep_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.pecs_sweep_set_active.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.pecs_sweep_destroy.argtypes = [ctypes.c_void_p]
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.ctx for m in self.members])
+        self.h = ctypes.c_void_p()
+        _check(L.pecs_sweep_create(arr, len(self.members), int(nstreams), ctypes.byref(self.h)))
+
+    def set_active(self, mask):
+        flags = (ctypes.c_int * len(self.members))(*[int(bool(a)) for a in mask])
+        _check(lib().pecs_sweep_set_active(self.h, flags))
+
+    def step(self, nsteps=1):
+        _check(lib().pecs_sweep_step(self.h, int(nsteps)))
+
+    def residuals(self):
+        """Steady residual of every member (max over its batch instances)."""
+        return np.array([m.residuals().max() for m in self.members])
+
+    def close(self):
+        if self.h:
+            lib().pecs_sweep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -4,6 +4,7 @@
 
 #include <array>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -48,6 +49,12 @@ struct pecs_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;            // second stream for the concurrent region solves
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // one time step captured as a graph (pecs_step replays it); cap is the
+  // stream it is captured on (capture needs a stream other than the null stream)
+  hipStream_t cap = nullptr;
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  bool use_graph = true;
   Params P;                  // parameters of instance 0 (matrix parameters are common to all)
   int B = 1;                 // instances sharing mesh, matrices and factors (a bias / illumination / doping batch)
   std::vector<Params> Pb;    // per instance
@@ -79,6 +86,7 @@ struct pecs_ctx {
   double* tH = nullptr;
   double* old[5] = {};
   double* acc = nullptr;  // 2 per instance
+  double* rpart = nullptr;  // residual partial sums: 5 vectors x RES_PARTS pairs per instance
   double* io[4] = {};     // staging for host-pointer calls (one instance)
   std::vector<double> last_res;
   double setup_seconds = 0.0, factor_seconds = 0.0;
@@ -91,6 +99,12 @@ struct pecs_ctx {
   int timed_steps = 0;
 
   ~pecs_ctx() {
+    if (step_exec) (void)hipGraphExecDestroy(step_exec);
+    if (step_graph) (void)hipGraphDestroy(step_graph);
+    if (cap) {
+      (void)hipStreamSynchronize(cap);
+      (void)hipStreamDestroy(cap);
+    }
     if (side) {
       (void)hipStreamSynchronize(side);
       (void)hipStreamDestroy(side);
@@ -355,6 +369,13 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
     if (dbwd) (void)hipFree(dbwd);
     dfwd = dbwd = nullptr;
   };
+  const char* tr = std::getenv("PECS_SETUP_TRACE");
+  const bool trace = tr && std::string(tr) == "1";
+  auto tnow = [] { return std::chrono::steady_clock::now(); };
+  auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+  };
+  const auto t0 = tnow();
   if (on_host) {
     SupernodalFactor F = cholesky_factor(S, leaf);
     P = pack_factor(F);
@@ -362,6 +383,7 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
     FrontPlan plan;
     SupernodalFactor F = cholesky_symbolic(S, leaf, plan);
     P = pack_layout(F);
+    if (trace) std::fprintf(stderr, "[setup] n=%d: symbolic + layout %.3f s\n", P.n, secs(t0, tnow()));
     HIPCHECK(hipMalloc(&dfwd, std::max<int64_t>(P.nvals, 1) * sizeof(double)));
     if (hipMalloc(&dbwd, std::max<int64_t>(P.nvals, 1) * sizeof(double)) != hipSuccess) {
       free_tmp();
@@ -449,7 +471,10 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   d.X = dalloc<double>(c, (size_t)nr * P.n);
   HIPCHECK(hipMemset(d.Z, 0, (size_t)nr * std::max<int64_t>(P.zlen, 1) * sizeof(double)));
   HIPCHECK(hipMemset(d.X, 0, (size_t)nr * P.n * sizeof(double)));
+  const auto tt = tnow();
+  if (trace) std::fprintf(stderr, "[setup] n=%d: numeric + upload %.3f s\n", P.n, secs(t0, tt));
   tune_factor(c, P, d, nrhs);
+  if (trace) std::fprintf(stderr, "[setup] n=%d: tuning %.3f s\n", P.n, secs(tt, tnow()));
   auto to4 = [](const std::vector<int>& v) {
     std::vector<int4> o(v.size() / 4);
     for (size_t t = 0; t < o.size(); ++t) o[t] = make_int4(v[4 * t], v[4 * t + 1], v[4 * t + 2], v[4 * t + 3]);
@@ -643,6 +668,51 @@ void step_once(pecs_ctx* c) {
   pm.mark();
 }
 
+// Capture step_once into c->step_exec (once per context).  The capture runs
+// on c->cap with the context's own stream swapped for it; the side-stream
+// fork/join of the region solves becomes part of the graph.  The step has no
+// memset or host work, only kernels and event dependencies.
+void ensure_step_graph(pecs_ctx* c) {
+  if (c->step_exec) return;
+  hipStream_t keep = c->stream;
+  const int timing = c->timing;
+  c->stream = c->cap;
+  c->timing = 0;
+  hipGraph_t g = nullptr;
+  try {
+    HIPCHECK(hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal));
+    try {
+      step_once(c);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->cap, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHECK(hipStreamEndCapture(c->cap, &g));
+    hipGraphExec_t e = nullptr;
+    HIPCHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    c->step_graph = g;
+    c->step_exec = e;
+  } catch (...) {
+    c->stream = keep;
+    c->timing = timing;
+    throw;
+  }
+  c->stream = keep;
+  c->timing = timing;
+}
+
+// One time step: the captured graph, or the launches themselves when graphs
+// are off or per-phase timing is on (the timing events are host-recorded).
+void run_step(pecs_ctx* c) {
+  if (c->use_graph && !c->timing) {
+    ensure_step_graph(c);
+    HIPCHECK(hipGraphLaunch(c->step_exec, c->stream));
+  } else {
+    step_once(c);
+  }
+}
+
 void collect_timing(pecs_ctx* c) {
   if (!c->timing || c->ev_used == 0) return;
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -728,6 +798,7 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
       throw std::runtime_error("no HIP device: the PECS path runs on the GPU only");
     HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
     c->P = to_params(params);
@@ -746,13 +817,28 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     LdgRegionOps oe = build_ldg_ops(c->mesh.elec, T, c->P);
     PoissonOps op = build_poisson_ops(c->mesh, c->L, T, c->P);
     auto tf = std::chrono::steady_clock::now();
+    const char* tr = std::getenv("PECS_SETUP_TRACE");
+    if (tr && std::string(tr) == "1")
+      std::fprintf(stderr, "[setup] mesh + local operators %.3f s\n",
+                   std::chrono::duration<double>(tf - t0).count());
     c->share_s = c->P.mu[0] == c->P.mu[1];
     c->share_e = c->P.mu[2] == c->P.mu[3];
-    c->Fs[0] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[0]), leaf, c->share_s ? 2 * B : B);
-    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur_matrix(c->mesh.semi, os, c->P.mu[1]), leaf, B);
-    c->Fe[0] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[2]), leaf, c->share_e ? 2 * B : B);
-    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur_matrix(c->mesh.elec, oe, c->P.mu[3]), leaf, B);
-    c->FH = make_factor(c.get(), multiplier_matrix(c->mesh, op, T), leaf, B);
+    auto timed = [&](const char* what, auto&& build) {
+      const auto ta = std::chrono::steady_clock::now();
+      BlockSPD S = build();
+      if (tr && std::string(tr) == "1")
+        std::fprintf(stderr, "[setup] %s matrix %.3f s\n", what,
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - ta).count());
+      return S;
+    };
+    auto schur = [&](const Region& R, const LdgRegionOps& o, double mu) {
+      return timed("Schur", [&] { return schur_matrix(R, o, mu); });
+    };
+    c->Fs[0] = make_factor(c.get(), schur(c->mesh.semi, os, c->P.mu[0]), leaf, c->share_s ? 2 * B : B);
+    if (!c->share_s) c->Fs[1] = make_factor(c.get(), schur(c->mesh.semi, os, c->P.mu[1]), leaf, B);
+    c->Fe[0] = make_factor(c.get(), schur(c->mesh.elec, oe, c->P.mu[2]), leaf, c->share_e ? 2 * B : B);
+    if (!c->share_e) c->Fe[1] = make_factor(c.get(), schur(c->mesh.elec, oe, c->P.mu[3]), leaf, B);
+    c->FH = make_factor(c.get(), timed("multiplier", [&] { return multiplier_matrix(c->mesh, op, T); }), leaf, B);
     c->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
     // reference tables
     DevTabs& tb = c->tabs;
@@ -844,6 +930,7 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     c->rH = dalloc<double>(c.get(), B * (size_t)c->FH.n);
     c->tH = dalloc<double>(c.get(), std::max(B, scratch_rhs(B)) * (size_t)c->FH.n);
     c->acc = dalloc<double>(c.get(), 2 * B);
+    c->rpart = dalloc<double>(c.get(), (size_t)10 * RES_PARTS * B);
     c->last_res.assign(B, -1.0);
     HIPCHECK(hipDeviceSynchronize());
     c->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1187,39 +1274,169 @@ int pecs_initial_state(pecs_ctx* ctx) {
   })
 }
 
+namespace {
+
+// The state before the last step of a pecs_step call (for the steady residual).
+// Copy kernels, not memcpys: they queue behind the step graphs on the same
+// compute queue.  (A device-to-device hipMemcpyAsync issued on a non-blocking
+// stream right after a graph launch was seen to read the state before the
+// graph had finished.)
+void save_old(pecs_ctx* c, hipStream_t s) {
+  const int64_t nsv[5] = {c->Ns, c->Ns, c->Ne, c->Ne, c->NP};
+  for (int j = 0; j < 4; ++j) HIPCHECK((hipError_t)launch_copy(c->u[j], c->old[j], c->B * nsv[j], s));
+  HIPCHECK((hipError_t)launch_copy(c->pv, c->old[4], c->B * nsv[4], s));
+}
+
+// ||x^{n+1} - x^n||^2 and ||x^n||^2 over the five state vectors, per
+// instance, into c->acc (2 per instance), then to h (host, 2 B doubles).
+void launch_residual(pecs_ctx* c, hipStream_t s, double* h) {
+  const int64_t nsv[5] = {c->Ns, c->Ns, c->Ne, c->Ne, c->NP};
+  const int B = c->B;
+  for (int b = 0; b < B; ++b) {
+    for (int j = 0; j < 5; ++j) {
+      const double* cur = j < 4 ? c->u[j] : c->pv;
+      HIPCHECK((hipError_t)launch_diff_norms(cur + b * nsv[j], c->old[j] + b * nsv[j], nsv[j],
+                                             c->rpart + (size_t)(5 * b + j) * 2 * RES_PARTS, s));
+    }
+  }
+  HIPCHECK((hipError_t)launch_sum_partials(c->rpart, 5, c->acc, B, s));
+  HIPCHECK(hipMemcpyAsync(h, c->acc, 2 * B * sizeof(double), hipMemcpyDeviceToHost, s));
+}
+
+void finish_residual(pecs_ctx* c, const double* h) {
+  for (int b = 0; b < c->B; ++b)
+    c->last_res[b] = (h[2 * b + 1] > 0) ? std::sqrt(h[2 * b] / h[2 * b + 1]) / c->P.dt : 0.0;
+}
+
+}  // namespace
+
 int pecs_step(pecs_ctx* ctx, int nsteps) {
   PECS_TRY(ctx, {
     if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
-    const int B = ctx->B;
-    const int64_t nsv[5] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne, ctx->NP};
+    const bool graph = ctx->use_graph && !ctx->timing;
     for (int i = 0; i < nsteps; ++i) {
       if (i == nsteps - 1) {
-        for (int j = 0; j < 4; ++j)
-          HIPCHECK(hipMemcpyAsync(ctx->old[j], ctx->u[j], B * nsv[j] * sizeof(double), hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-        HIPCHECK(hipMemcpyAsync(ctx->old[4], ctx->pv, B * nsv[4] * sizeof(double), hipMemcpyDeviceToDevice,
-                                ctx->stream));
+        // with graph replay the host waits around the last step (see pecs_sweep_step)
+        if (graph) HIPCHECK(hipStreamSynchronize(ctx->stream));
+        save_old(ctx, ctx->stream);
+        if (graph) HIPCHECK(hipStreamSynchronize(ctx->stream));
       }
-      step_once(ctx);
+      run_step(ctx);
       if (ctx->timing && ctx->ev_used + NPHASE + 1 > (int)ctx->events.size()) collect_timing(ctx);
     }
     collect_timing(ctx);
     if (nsteps > 0) {
-      HIPCHECK(hipMemsetAsync(ctx->acc, 0, 2 * B * sizeof(double), ctx->stream));
-      for (int b = 0; b < B; ++b) {
-        for (int j = 0; j < 4; ++j)
-          HIPCHECK((hipError_t)launch_diff_norms(ctx->u[j] + b * nsv[j], ctx->old[j] + b * nsv[j], nsv[j],
-                                                 ctx->acc + 2 * b, ctx->stream));
-        HIPCHECK((hipError_t)launch_diff_norms(ctx->pv + b * nsv[4], ctx->old[4] + b * nsv[4], nsv[4],
-                                               ctx->acc + 2 * b, ctx->stream));
-      }
-      std::vector<double> h(2 * B);
-      HIPCHECK(hipMemcpyAsync(h.data(), ctx->acc, 2 * B * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      if (graph) HIPCHECK(hipStreamSynchronize(ctx->stream));
+      std::vector<double> h(2 * ctx->B);
+      launch_residual(ctx, ctx->stream, h.data());
       HIPCHECK(hipStreamSynchronize(ctx->stream));
-      for (int b = 0; b < B; ++b)
-        ctx->last_res[b] = (h[2 * b + 1] > 0) ? std::sqrt(h[2 * b] / h[2 * b + 1]) / ctx->P.dt : 0.0;
+      finish_residual(ctx, h.data());
     }
   })
+}
+
+// ---------------------------------------------------------------------------
+// Sweeps over contexts with different meshes (uneven-mesh I-V / geometry
+// sweeps): a time step of all active members is one launch of each member's
+// captured step graph, the members spread round-robin over up to 4 streams so
+// that the small members' launch-latency-bound steps overlap each other and
+// the large members' bandwidth-bound ones.  (One parent graph holding the
+// members' graphs as child nodes measured slower than these launches.)
+
+struct pecs_sweep {
+  std::vector<pecs_ctx*> m;
+  std::vector<char> active;
+  std::vector<hipStream_t> st;
+  ~pecs_sweep() {
+    for (hipStream_t s : st) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+int pecs_sweep_create(pecs_ctx** ctxs, int n, int nstreams, pecs_sweep** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  try {
+    if (!ctxs || n < 1) throw std::runtime_error("pecs_sweep_create: need at least one context");
+    if (nstreams < 1 || nstreams > 4) throw std::runtime_error("pecs_sweep_create: nstreams must be 1..4");
+    for (int i = 0; i < n; ++i) {
+      if (!ctxs[i]) throw std::runtime_error("pecs_sweep_create: null context");
+      for (int j = 0; j < i; ++j)
+        if (ctxs[j] == ctxs[i]) throw std::runtime_error("pecs_sweep_create: a context appears twice");
+    }
+    auto w = std::make_unique<pecs_sweep>();
+    w->m.assign(ctxs, ctxs + n);
+    w->active.assign(n, 1);
+    w->st.assign(nstreams, nullptr);
+    for (hipStream_t& s : w->st) HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = w.release();
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pecs_sweep_set_active(pecs_sweep* w, const int* active) {
+  if (!w || !active) return -1;
+  for (size_t i = 0; i < w->m.size(); ++i) w->active[i] = active[i] ? 1 : 0;
+  return 0;
+}
+
+int pecs_sweep_step(pecs_sweep* w, int nsteps) {
+  if (!w) return -1;
+  try {
+    if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
+    const int ns = (int)w->st.size();
+    std::vector<int> q_of(w->m.size(), -1);
+    int slot = 0;
+    for (size_t i = 0; i < w->m.size(); ++i) {
+      if (!w->active[i]) continue;
+      // the member's own stream drained: its state is complete before the sweep reads it
+      HIPCHECK(hipStreamSynchronize(w->m[i]->stream));
+      ensure_step_graph(w->m[i]);
+      q_of[i] = slot++ % ns;
+    }
+    for (int t = 0; t < nsteps; ++t) {
+      // Around the last step the host waits for the streams: the copy of the
+      // state before it and the residual kernels after it must not overlap
+      // a step graph (on non-blocking streams, kernels queued right behind a
+      // graph launch were seen to overlap the graph's tail).
+      if (t == nsteps - 1) {
+        for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
+        for (size_t i = 0; i < w->m.size(); ++i)
+          if (q_of[i] >= 0) save_old(w->m[i], w->st[q_of[i]]);
+        for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
+      }
+      for (size_t i = 0; i < w->m.size(); ++i)
+        if (q_of[i] >= 0) HIPCHECK(hipGraphLaunch(w->m[i]->step_exec, w->st[q_of[i]]));
+    }
+    if (nsteps > 0) {
+      for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
+      std::vector<std::vector<double>> h(w->m.size());
+      for (size_t i = 0; i < w->m.size(); ++i)
+        if (q_of[i] >= 0) {
+          h[i].assign(2 * w->m[i]->B, 0.0);
+          launch_residual(w->m[i], w->st[q_of[i]], h[i].data());
+        }
+      for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
+      for (size_t i = 0; i < w->m.size(); ++i)
+        if (q_of[i] >= 0) finish_residual(w->m[i], h[i].data());
+    }
+  } catch (const std::exception& e) {
+    for (hipStream_t s : w->st) (void)hipStreamSynchronize(s);
+    g_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pecs_sweep_destroy(pecs_sweep* w) {
+  if (!w) return -1;
+  delete w;
+  return 0;
 }
 
 int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels) {
@@ -1232,6 +1449,10 @@ int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels) {
       for (int j = 0; j < 6; ++j) out[6 * l + j] = v[j];
     }
   })
+}
+
+int pecs_use_graph(pecs_ctx* ctx, int enable) {
+  PECS_TRY(ctx, { ctx->use_graph = enable != 0; })
 }
 
 int pecs_enable_timing(pecs_ctx* ctx, int enable) {

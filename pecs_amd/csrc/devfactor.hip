@@ -8,7 +8,7 @@
 // triangle, so the dense calls below work on Upper / column-major operands:
 //   potrf(Upper):            leading block = U11 (U11^T U11 = F11), U11 = L11^T
 //   trsm(Left, Upper, T):    panel (columns ns..nf of rows 0..ns) = L21^T
-//   syrk(Upper, T, -1, 1):   trailing block -= L21 L21^T
+//   gemm(T, N, -1, 1):       trailing block -= L21 L21^T (both triangles)
 //   trtri(Upper):            leading block = U11^-1 = Linv^T
 //   trmm(Left, Upper, N):    W21^T = Linv^T L21^T, column-major ns x ms
 //                            = W21 row-major ms x ns, straight into fwd.
@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -201,6 +203,10 @@ __global__ __launch_bounds__(256) void k_pack_blocked(int fwd, const int* __rest
   }
 }
 
+// Groups of at most kFewFronts fronts with ns >= kLargeFront use one
+// non-batched call per front.
+constexpr int kLargeFront = 512, kFewFronts = 8;
+
 struct Group {
   int level, ns, ms, count;
   size_t ptr0;  // first of 4 * count pointers: A11[count] | A12[count] | A22[count] | W[count]
@@ -263,12 +269,22 @@ DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor
   double** d_ptrs = pool.up(ptrs, st);
   rocblas_int* d_info = pool.alloc<rocblas_int>(ninfo);
   DFCHECK(hipMemsetAsync(d_info, 0, std::max<size_t>(ninfo, 1) * sizeof(rocblas_int), st));
+  // PECS_SETUP_TRACE=1: per-level times of the assembly kernels and the dense calls on stderr
+  const char* tr = std::getenv("PECS_SETUP_TRACE");
+  const bool trace = tr && std::string(tr) == "1";
+  std::vector<hipEvent_t> ev;
+  std::vector<std::pair<int, hipEvent_t>> opev;
+  if (trace) {
+    ev.resize(2 * F.nlevels + 2);
+    for (auto& e : ev) DFCHECK(hipEventCreate(&e));
+  }
   rocblas_handle h = nullptr;
   RBCHECK(rocblas_create_handle(&h));
   try {
     RBCHECK(rocblas_set_stream(h, st));
     const double one = 1.0, minus_one = -1.0;
     size_t gi = 0;
+    if (trace) DFCHECK(hipEventRecord(ev[0], st));
     for (int l = 0; l < F.nlevels; ++l) {
       const int s0 = plan.sc_level[l], s1 = plan.sc_level[l + 1];
       if (s1 > s0) k_front_scatter<<<s1 - s0, 64, 0, st>>>(sv, d_sc, s0, fr);
@@ -277,34 +293,85 @@ DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor
         if (e1 > e0) k_extend_add<<<e1 - e0, 64, 0, st>>>(d_ea, e0, d_ns, d_ms, d_front_off, d_map, d_map_off, fr);
       }
       DFCHECK(hipGetLastError());
+      if (trace) DFCHECK(hipEventRecord(ev[2 * l + 1], st));
       for (; gi < groups.size() && groups[gi].level == l; ++gi) {
         const Group& g = groups[gi];
+        auto mark = [&](int op) {  // trace: per-call events, op 0..4 = potrf trsm gemm trtri trmm, 5 = end
+          if (!trace) return;
+          hipEvent_t e;
+          DFCHECK(hipEventCreate(&e));
+          DFCHECK(hipEventRecord(e, st));
+          opev.push_back({op, e});
+        };
         const int ns = g.ns, ms = g.ms, nf = ns + ms, cnt = g.count;
         double* const* A11 = d_ptrs + g.ptr0;
         double* const* A12 = A11 + cnt;
         double* const* A22 = A12 + cnt;
         double* const* Wo = A22 + cnt;
         rocblas_int* info = d_info + g.info0;
+        if (ns >= kLargeFront && cnt <= kFewFronts) {
+          // a few large fronts: one non-batched (blocked, GEMM-rich) call per
+          // front runs faster than the batched routines
+          for (int f = 0; f < cnt; ++f) {
+            double* a11 = ptrs[g.ptr0 + f];
+            double* a12 = ptrs[g.ptr0 + cnt + f];
+            double* a22 = ptrs[g.ptr0 + 2 * cnt + f];
+            double* wo = ptrs[g.ptr0 + 3 * cnt + f];
+            mark(0);
+            RBCHECK(rocsolver_dpotrf(h, rocblas_fill_upper, ns, a11, nf, info + f));
+            if (ms > 0) {
+              mark(1);
+              RBCHECK(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                    rocblas_diagonal_non_unit, ns, ms, &one, a11, nf, a12, nf));
+              mark(2);
+              RBCHECK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ms, ms, ns, &minus_one,
+                                    a12, nf, a12, nf, &one, a22, nf));
+            }
+            mark(3);
+            RBCHECK(rocsolver_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, ns, a11, nf, info + cnt + f));
+            if (ms > 0) {
+              mark(4);
+              RBCHECK(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                    rocblas_diagonal_non_unit, ns, ms, &one, a11, nf, a12, nf, wo, ns));
+            }
+            mark(5);
+          }
+          continue;
+        }
+        mark(0);
         RBCHECK(rocsolver_dpotrf_batched(h, rocblas_fill_upper, ns, A11, nf, info, cnt));
         if (ms > 0) {
+          mark(1);
           RBCHECK(rocblas_dtrsm_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
                                         rocblas_diagonal_non_unit, ns, ms, &one, (const double* const*)A11, nf, A12,
                                         nf, cnt));
-          RBCHECK(rocblas_dsyrk_batched(h, rocblas_fill_upper, rocblas_operation_transpose, ms, ns, &minus_one,
-                                        (const double* const*)A12, nf, &one, A22, nf, cnt));
+          mark(2);
+          // trailing update A22 -= A12^T A12 as a full GEMM: twice the flops of
+          // SYRK, but rocBLAS's GEMM kernels run it several times faster; the
+          // extra (strictly lower, column-major) triangle lands in the part of
+          // the front that nothing reads
+          RBCHECK(rocblas_dgemm_batched(h, rocblas_operation_transpose, rocblas_operation_none, ms, ms, ns,
+                                        &minus_one, (const double* const*)A12, nf, (const double* const*)A12, nf,
+                                        &one, A22, nf, cnt));
         }
+        mark(3);
         RBCHECK(rocsolver_dtrtri_batched(h, rocblas_fill_upper, rocblas_diagonal_non_unit, ns, A11, nf, info + cnt,
                                          cnt));
-        if (ms > 0)
+        if (ms > 0) {
+          mark(4);
           RBCHECK(rocblas_dtrmm_batched(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                                         rocblas_diagonal_non_unit, ns, ms, &one, (const double* const*)A11, nf,
                                         (const double* const*)A12, nf, Wo, ns, cnt));
+        }
+        mark(5);
       }
+      if (trace) DFCHECK(hipEventRecord(ev[2 * l + 2], st));
     }
     if (!pk.empty())
       k_pack_rows<<<(unsigned)pk.size(), 256, 0, st>>>(d_pk, d_ns, d_ms, d_front_off, d_fwd_off, d_bwd_off, fr, fwd,
                                                        bwd);
     DFCHECK(hipGetLastError());
+    if (trace) DFCHECK(hipEventRecord(ev[2 * F.nlevels + 1], st));
     std::vector<rocblas_int> info(ninfo);
     if (ninfo) DFCHECK(hipMemcpyAsync(info.data(), d_info, ninfo * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
     DFCHECK(hipStreamSynchronize(st));
@@ -314,9 +381,41 @@ DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor
           throw std::runtime_error("device factorisation: front of shape (" + std::to_string(g.ns) + ", " +
                                    std::to_string(g.ms) + ") at level " + std::to_string(g.level) +
                                    (i < g.count ? " is not positive definite" : " has a singular factor"));
+    if (trace) {
+      float a = 0, d = 0, ms = 0;
+      size_t g2 = 0;
+      for (int l = 0; l < F.nlevels; ++l) {
+        DFCHECK(hipEventElapsedTime(&a, ev[2 * l], ev[2 * l + 1]));
+        DFCHECK(hipEventElapsedTime(&d, ev[2 * l + 1], ev[2 * l + 2]));
+        int ng = 0, mxn = 0, mxm = 0;
+        for (; g2 < groups.size() && groups[g2].level == l; ++g2, ++ng) {
+          mxn = std::max(mxn, groups[g2].ns);
+          mxm = std::max(mxm, groups[g2].ms);
+        }
+        std::fprintf(stderr, "[factor] level %2d: %5zu fronts, %3d shapes (max ns %d, ms %d): assembly %.2f ms, dense %.2f ms\n",
+                     l, F.level_nodes[l].size(), ng, mxn, mxm, a, d);
+      }
+      double opms[5] = {0, 0, 0, 0, 0};
+      for (size_t i = 0; i + 1 < opev.size(); ++i) {
+        if (opev[i].first == 5) continue;
+        float t = 0;
+        DFCHECK(hipEventElapsedTime(&t, opev[i].second, opev[i + 1].second));
+        opms[opev[i].first] += t;
+      }
+      std::fprintf(stderr, "[factor] dense calls: potrf %.1f ms, trsm %.1f ms, gemm %.1f ms, trtri %.1f ms, trmm %.1f ms\n",
+                   opms[0], opms[1], opms[2], opms[3], opms[4]);
+      for (auto& pe : opev) (void)hipEventDestroy(pe.second);
+      opev.clear();
+      DFCHECK(hipEventElapsedTime(&ms, ev[2 * F.nlevels], ev[2 * F.nlevels + 1]));
+      std::fprintf(stderr, "[factor] sweep layout %.2f ms; fronts %.2f GB; %.3g dense flops\n", ms,
+                   stats.front_bytes * 1e-9, stats.flops);
+      for (auto& e : ev) (void)hipEventDestroy(e);
+    }
   } catch (...) {
     (void)hipStreamSynchronize(st);
     rocblas_destroy_handle(h);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& pe : opev) (void)hipEventDestroy(pe.second);
     throw;
   }
   RBCHECK(rocblas_destroy_handle(h));

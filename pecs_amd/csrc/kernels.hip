@@ -450,17 +450,24 @@ struct SweepShape {
 
 // forward, part 1: t_p = b_p - sum of the updates z owed to row p, into X.
 // Right-hand side h = blockIdx.y; vectors are h-major (b[h n + p], Z[h zlen + z]).
+// L lanes per row (L = 1, or 8 on levels with few rows: their rows collect
+// many descendants' contributions); the lanes' partial sums are combined by
+// shuffles in a fixed order.
+template <int L>
 __global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int* __restrict__ lrows, int nrow,
                                                     const double* __restrict__ b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrow) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / L, lane = t % L;
+  if (i >= nrow) return;  // nrow * L threads: whole L-groups exit together
   const int h = blockIdx.y;
   const int p = lrows[i];
   const double* __restrict__ Zh = F.Z + (int64_t)h * F.zlen;
-  double a = b[(int64_t)h * F.n + p];
   const int64_t c1 = F.cptr[p + 1];
-  for (int64_t c = F.cptr[p]; c < c1; ++c) a -= Zh[F.czidx[c]];
-  F.X[(int64_t)h * F.n + p] = a;
+  double z = 0.0;
+  for (int64_t c = F.cptr[p] + lane; c < c1; c += L) z += Zh[F.czidx[c]];
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) z += __shfl_xor(z, o, L);
+  if (lane == 0) F.X[(int64_t)h * F.n + p] = b[(int64_t)h * F.n + p] - z;
 }
 
 // backward, part 1: Z[q] = x[rows[q]] for the panel rows of the level's nodes
@@ -981,8 +988,18 @@ __global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTa
   for (int l = 0; l < NPHI; ++l) b[id[NRT + l]] = fphi[l];
 }
 
+__global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+// Steady-residual norms without atomics (the sums are bitwise reproducible):
+// block g of k_diff_norms writes its partial sums of |a - b|^2 and |b|^2 to
+// part[2 g], part[2 g + 1] (RES_BLOCKS blocks per vector); k_sum_partials then
+// adds each instance's partials in a fixed order.
+constexpr int RES_BLOCKS = RES_PARTS;
+
 __global__ __launch_bounds__(256) void k_diff_norms(const double* __restrict__ a, const double* __restrict__ b,
-                                                    int64_t n, double* acc) {
+                                                    int64_t n, double* __restrict__ part) {
   __shared__ double s0[256], s1[256];
   double d = 0.0, o = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1001,8 +1018,35 @@ __global__ __launch_bounds__(256) void k_diff_norms(const double* __restrict__ a
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomicAdd(&acc[0], s0[0]);
-    atomicAdd(&acc[1], s1[0]);
+    part[2 * blockIdx.x] = s0[0];
+    part[2 * blockIdx.x + 1] = s1[0];
+  }
+}
+
+// One block per instance: acc[2 i], acc[2 i + 1] = sums of the instance's
+// nparts partial pairs at part + 2 nparts i.
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int nparts,
+                                                      double* __restrict__ acc) {
+  __shared__ double s0[256], s1[256];
+  const double* p = part + 2 * (int64_t)nparts * blockIdx.x;
+  double d = 0.0, o = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    d += p[2 * i];
+    o += p[2 * i + 1];
+  }
+  s0[threadIdx.x] = d;
+  s1[threadIdx.x] = o;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    acc[2 * blockIdx.x] = s0[0];
+    acc[2 * blockIdx.x + 1] = s1[0];
   }
 }
 
@@ -1146,7 +1190,8 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
   if (nrhs > 2) return (int)hipErrorInvalidValue;
   for (int l = 0; l < L.nlevels; ++l) {
     const int ng = L.goff[l + 1] - L.goff[l];
-    if (ng > 0) k_fwd_gather<<<dim3(nblk(ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
+    if (ng >= 16384) k_fwd_gather<1><<<dim3(nblk(ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
+    else if (ng > 0) k_fwd_gather<8><<<dim3(nblk(8 * (int64_t)ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
     const int nt = L.foff[l + 1] - L.foff[l];
     const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s);
     if (e) return e;
@@ -1158,8 +1203,9 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
     const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s);
     if (e) return e;
   }
-  hipError_t e = hipMemcpyAsync(v, F.X, (size_t)nrhs * F.n * sizeof(double), hipMemcpyDeviceToDevice, s);
-  if (e != hipSuccess) return (int)e;
+  // a kernel, not a memcpy node, so that the solve can be captured in a graph
+  const int64_t nx = (int64_t)nrhs * F.n;
+  k_copy<<<(unsigned)std::min<int64_t>(nblk(nx, 256), 4096), 256, 0, s>>>(F.X, v, nx);
   return (int)hipGetLastError();
 }
 
@@ -1197,11 +1243,19 @@ int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, co
   return (int)hipGetLastError();
 }
 
-int launch_diff_norms(const double* a, const double* b, int64_t n, double* acc, hipStream_t s) {
-  int nb = nblk(n, 256);
-  if (nb > 1024) nb = 1024;
-  if (nb < 1) nb = 1;
-  k_diff_norms<<<nb, 256, 0, s>>>(a, b, n, acc);
+int launch_diff_norms(const double* a, const double* b, int64_t n, double* part, hipStream_t s) {
+  k_diff_norms<<<RES_BLOCKS, 256, 0, s>>>(a, b, n, part);
+  return (int)hipGetLastError();
+}
+
+int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0;
+  k_copy<<<(unsigned)std::min<int64_t>(nblk(n, 256), 4096), 256, 0, s>>>(src, dst, n);
+  return (int)hipGetLastError();
+}
+
+int launch_sum_partials(const double* part, int nvec, double* acc, int ninst, hipStream_t s) {
+  k_sum_partials<<<ninst, 256, 0, s>>>(part, nvec * RES_BLOCKS, acc);
   return (int)hipGetLastError();
 }
 

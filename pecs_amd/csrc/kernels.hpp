@@ -151,6 +151,13 @@ int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const 
 int launch_poisson_rhs_assemble(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                                 const double* up, const double* ur, const double* uo, int ncs, int nce, double* b,
                                 hipStream_t s);
-int launch_diff_norms(const double* a, const double* b, int64_t n, double* acc, hipStream_t s);
+// Steady residual: per vector, RES_PARTS partial pairs (|a-b|^2, |b|^2) into
+// part[0 .. 2 RES_PARTS); then per instance the sum of nvec vectors' partials
+// (stored consecutively, instance after instance) into acc[2 i], acc[2 i + 1].
+constexpr int RES_PARTS = 1024;
+int launch_diff_norms(const double* a, const double* b, int64_t n, double* part, hipStream_t s);
+int launch_sum_partials(const double* part, int nvec, double* acc, int ninst, hipStream_t s);
+// dst[0:n] = src[0:n] as a kernel (stays on the stream's compute queue)
+int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace pecs

@@ -46,11 +46,14 @@ bool dense_inverse(int n, const double* A, double* Ainv) {
   return true;
 }
 
+// FNV-1a over 64-bit words (equal records give equal hashes; the table
+// lookup still compares whole records).
 static uint64_t fnv1a(const double* p, size_t n) {
   uint64_t h = 1469598103934665603ull;
-  const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
-  for (size_t i = 0; i < n * sizeof(double); ++i) {
-    h ^= b[i];
+  for (size_t i = 0; i < n; ++i) {
+    uint64_t w;
+    std::memcpy(&w, p + i, sizeof w);
+    h ^= w;
     h *= 1099511628211ull;
   }
   return h;
@@ -64,9 +67,12 @@ static void dedup(const std::vector<double>& rec, int ncell, int stride, std::ve
   type.assign(ncell, -1);
   table.clear();
   ntype = 0;
+  std::vector<uint64_t> hash(ncell);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < ncell; ++c) hash[c] = fnv1a(&rec[(size_t)c * stride], stride);
   for (int c = 0; c < ncell; ++c) {
     const double* r = &rec[(size_t)c * stride];
-    uint64_t h = fnv1a(r, stride);
+    const uint64_t h = hash[c];
     auto& cand = seen[h];
     int found = -1;
     for (int t : cand)
@@ -91,6 +97,8 @@ LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P)
   O.cell.resize(R.ncell);
   std::vector<double> hdiam(R.ncell);
   for (int c = 0; c < R.ncell; ++c) hdiam[c] = cell_diameter(&R.V[8 * c]);
+  int singular = 0;  // the cells are independent: one OpenMP loop, errors raised after it
+#pragma omp parallel for schedule(static) reduction(| : singular)
   for (int c = 0; c < R.ncell; ++c) {
     const double* v = &R.V[8 * c];
     const double h = hdiam[c];
@@ -149,10 +157,12 @@ LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P)
       }
     }
     for (int t = 0; t < nb * nb; ++t) L.Dself[t] += (P.theta / P.dt) * L.Mk[t];
-    if (!dense_inverse(nb, L.Mk.data(), L.Minv.data())) throw std::runtime_error("singular cell mass matrix");
+    if (!dense_inverse(nb, L.Mk.data(), L.Minv.data())) singular = 1;
   }
+  if (singular) throw std::runtime_error("singular cell mass matrix");
   O.type_stride = nb * nb + 5 * 2 * nb * nb;
   std::vector<double> rec((size_t)R.ncell * O.type_stride);
+#pragma omp parallel for schedule(static)
   for (int c = 0; c < R.ncell; ++c) {
     double* r = &rec[(size_t)c * O.type_stride];
     const LdgCellOps& L = O.cell[c];
@@ -192,8 +202,24 @@ BlockSPD schur_matrix(const Region& R, const LdgRegionOps& ops, double mu) {
     if (b.empty()) b.assign(nb * nb, 0.0);
     return b.data();
   };
-  std::vector<double> Pm(2 * nb * nb);
-  for (int M = 0; M < R.ncell; ++M) {
+  // Cells of one colour (i mod 3, j mod 3) have disjoint closed neighbourhoods,
+  // so the blocks they add to are disjoint: each colour runs in parallel.
+  std::vector<std::vector<int>> colour(9);
+  for (int M = 0; M < R.ncell; ++M) colour[(R.ci[M] % 3) + 3 * (R.cj[M] % 3)].push_back(M);
+  for (int X = 0; X < R.ncell; ++X)  // allocate every block first (no allocation inside the parallel loops)
+    for (int f = -1; f < 4; ++f) {
+      const int Y = f < 0 ? X : R.nbr[4 * X + f];
+      if (Y < 0) continue;
+      for (int g = -1; g < 4; ++g) {
+        const int Z = g < 0 ? Y : R.nbr[4 * Y + g];
+        if (Z >= 0) block(X, Z);
+      }
+    }
+  for (const std::vector<int>& cells_of_colour : colour)
+#pragma omp parallel for schedule(dynamic, 64)
+  for (size_t mi = 0; mi < cells_of_colour.size(); ++mi) {
+    const int M = cells_of_colour[mi];
+    std::vector<double> Pm(2 * nb * nb);
     const LdgCellOps& L = ops.cell[M];
     int cells[5];
     const double* Bs[5];
@@ -306,8 +332,10 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
   O.Ainv.assign((size_t)nc * nl * nl, 0);
   O.A.assign((size_t)nc * nl * nl, 0);
   O.ubias.assign((size_t)nc * 3 * nrt, 0);
-  std::vector<double> psx(nrt), psy(nrt);
+  int singular = 0;  // the cells are independent: one OpenMP loop, errors raised after it
+#pragma omp parallel for schedule(static) reduction(| : singular)
   for (int K = 0; K < nc; ++K) {
+    std::vector<double> psx(nrt), psy(nrt);
     const double* v = &m.pV[8 * K];
     const double eps = (m.pmat[K] == SEMI_ID || m.pmat[K] == SEMI_BL_ID) ? P.eps_s : P.eps_e;
     double* A = &O.A[(size_t)K * nl * nl];
@@ -329,7 +357,7 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
         }
       }
     }
-    if (!dense_inverse(nl, A, &O.Ainv[(size_t)K * nl * nl])) throw std::runtime_error("singular local mixed matrix");
+    if (!dense_inverse(nl, A, &O.Ainv[(size_t)K * nl * nl])) singular = 1;
     // constant Dirichlet / Schottky data of semiconductor cells (SolarCell.cpp:593-681)
     if (m.pmat[K] == SEMI_ID || m.pmat[K] == SEMI_BL_ID) {
       for (int f = 0; f < 4; ++f) {
@@ -363,6 +391,7 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
       }
     }
   }
+  if (singular) throw std::runtime_error("singular local mixed matrix");
   // multipliers: interior and Neumann faces
   const int kf = T.deg + 1;
   auto is_mult = [&](int K, int f) {

@@ -386,9 +386,9 @@ static void device_factor_emulate(const FrontPlan& plan, const SupernodalFactor&
           for (int k = 0; k < i; ++k) v -= A(k, i) * A(k, ns + jc);
           A(i, ns + jc) = v / A(i, i);
         }
-      // syrk(Upper, Transpose, -1, 1): A22 -= A12^T A12
+      // gemm(Transpose, None, -1, 1): A22 -= A12^T A12 (both triangles)
       for (int j = 0; j < ms; ++j)
-        for (int i = 0; i <= j; ++i) {
+        for (int i = 0; i < ms; ++i) {
           double v = 0;
           for (int k = 0; k < ns; ++k) v += A(k, ns + i) * A(k, ns + j);
           A(ns + i, ns + j) -= v;
