@@ -281,6 +281,53 @@ def sweep_leg(args):
             "plain_ms_per_step": [round(m, 4) for m in plain_ms]}
 
 
+def warm_start_leg(args):
+    """Learned warm start: steady states of args.ws_train_groups x 16 draws
+    (cone angle per group; applied bias, generation amplitude and doping per
+    instance) at refinement args.ws_refine train the network (FP32, on the
+    GPU); on args.ws_test_groups x 16 held-out draws the time to steady state
+    from the network's prediction (BF16 inference) is compared with the cold
+    start from the equilibrium state (both batched per group and stepped
+    together by pecs_sweep)."""
+    import numpy as np
+    from pecs_amd import warmstart as ws
+    rng = np.random.default_rng(args.ws_seed)
+    log = lambda m: print(f"[bench] warm start: {m}", file=sys.stderr, flush=True)
+    k, dt, r = args.degree, args.dt, args.ws_refine
+    train_groups = ws.draw_groups(rng, args.ws_train_groups, 16)
+    test_groups = ws.draw_groups(rng, args.ws_test_groups, 16)
+    log(f"generating {16 * len(train_groups)} training steady states (refinement {r})")
+    Xtr, Ytr, steps_tr, wall_tr, setup_tr, _ = ws.generate(train_groups, r, k, dt, args.steady_tol,
+                                                        args.steady_check_every, args.steady_max_steps, log)
+    log(f"training set done: {wall_tr:.1f} s stepping, {setup_tr:.1f} s setup")
+    t0 = time.perf_counter()
+    net, norm, hist = ws.train(Xtr, Ytr, epochs=args.ws_epochs, log=log)
+    t_train = time.perf_counter() - t0
+    log(f"cold start of {16 * len(test_groups)} held-out draws")
+    Xte, Yte, steps_cold, wall_cold, _, res_cold = ws.generate(test_groups, r, k, dt, args.steady_tol,
+                                                     args.steady_check_every, args.steady_max_steps, log)
+    log("warm start of the held-out draws")
+    ev = ws.evaluate(net, norm, test_groups, r, k, dt, args.steady_tol, args.steady_check_every,
+                     args.steady_max_steps, log=log)
+    p32 = norm.back(net.predict(Xte, bf16=False))
+    err = np.linalg.norm(ev["pred"] - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
+    err32 = np.linalg.norm(p32 - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
+    n_te = len(Yte)
+    net.close()
+    return {"refinement": r, "train_draws": len(Ytr), "test_draws": n_te, "outputs": int(Ytr.shape[1]),
+            "network": "4 -> 3 x 256 ReLU -> outputs; FP32 Adam training, BF16 inference",
+            "train_s": round(t_train, 2), "train_loss_first_last": [hist[0], hist[-1]],
+            "prediction_rel_error_bf16": {"median": float(np.median(err)), "max": float(err.max())},
+            "prediction_rel_error_fp32": {"median": float(np.median(err32)), "max": float(err32.max())},
+            "first_block_residual_median": {"warm": float(np.median(ev["first_block_residual"])),
+                                            "cold": float(np.median(res_cold))},
+            "fallbacks": ev["fallbacks"], "tol": args.steady_tol,
+            "cold_steps_mean": float(steps_cold.mean()), "warm_steps_mean": float(ev["steps"].mean()),
+            "cold_s_per_bias_point": wall_cold / n_te, "warm_s_per_bias_point": ev["wall"] / n_te,
+            "warm_reached": bool(np.all(ev["steps"] > 0)),
+            "speedup_warm_vs_cold": wall_cold / ev["wall"] if ev["wall"] > 0 else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,6 +356,14 @@ def main():
     ap.add_argument("--sweep-plain-steps", type=int, default=20)
     ap.add_argument("--sweep-reserve-gib", type=float, default=64.0,
                     help="stop adding instances to a wave when less device memory than this is free")
+    ap.add_argument("--warm-start", action="store_true", help="run only the learned warm-start leg")
+    ap.add_argument("--ws-refine", type=int, default=5)
+    ap.add_argument("--ws-train-groups", type=int, default=64, help="16 draws per group (one cone angle)")
+    ap.add_argument("--ws-test-groups", type=int, default=8)
+    ap.add_argument("--ws-epochs", type=int, default=300)
+    ap.add_argument("--ws-seed", type=int, default=20261202)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernel by kernel instead of replaying captured step graphs (counter collection)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -331,6 +386,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n}), flush=True)
         return
+    if args.warm_start:
+        print(json.dumps({"warm_start": warm_start_leg(args)}), flush=True)
+        return
     if args.sweep_only:
         print(json.dumps({"sweep": sweep_leg(args)}), flush=True)
         return
@@ -343,6 +401,8 @@ def main():
     t0 = time.perf_counter()
     g = pecs_amd.SolarCellGPU(p)
     setup_s = time.perf_counter() - t0
+    if args.no_graph:
+        g.use_graph(False)
     st = g.stats()
     print(f"[bench] setup {setup_s:.1f}s (factor {st['factor_s']:.1f}s) sizes semi={g.n_semi} "
           f"elec={g.n_elec} poisson={g.n_poisson}", file=sys.stderr, flush=True)
@@ -367,11 +427,13 @@ def main():
     L.pecs_enable_timing(g.ctx, 0)
     phase = [ph[i] / max(nst.value, 1) for i in range(8)]
     # the same number of steps replayed as captured graphs (no timing events)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    torch.cuda.synchronize()
-    ms_graph = (time.perf_counter() - t0) / args.steps * 1e3
+    ms_graph = None
+    if not args.no_graph:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.step(args.steps)
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - t0) / args.steps * 1e3
     names = ["rhs_semi", "rhs_elec", "schur_transforms", "carrier_cholesky", "recoveries", "poisson_cond",
              "chol_poisson", "poisson_recover"]
     print("[bench] per-step phase ms: " + ", ".join(f"{n}={v:.3f}" for n, v in zip(names, phase)),

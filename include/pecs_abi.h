@@ -178,6 +178,25 @@ int pecs_sweep_set_active(pecs_sweep* sweep, const int* active);
 int pecs_sweep_step(pecs_sweep* sweep, int nsteps);
 int pecs_sweep_destroy(pecs_sweep* sweep);
 
+/* Learned warm start (no reference counterpart): a fully connected network,
+ * nin inputs -> nlayers hidden layers of nhidden ReLU units -> nout linear
+ * outputs, trained in FP32 on the GPU (MSE loss, Adam) and evaluated with
+ * BF16 weights and activations and FP32 accumulation (bf16 = 1) or in FP32.
+ * Parameters are one FP32 array: W_1 (nin x nhidden, row-major), b_1, W_2,
+ * b_2, ...  x: batch x nin, y: batch x nout, host arrays, row-major.
+ * pecs_mlp_gradients returns the loss and the gradient of every parameter
+ * (same layout) without updating them.  Errors: pecs_mlp_last_error(). */
+typedef struct pecs_mlp pecs_mlp;
+int pecs_mlp_create(int nin, int nhidden, int nlayers, int nout, unsigned seed, pecs_mlp** out);
+int pecs_mlp_destroy(pecs_mlp* net);
+int64_t pecs_mlp_num_params(pecs_mlp* net);
+int pecs_mlp_get_params(pecs_mlp* net, float* params);
+int pecs_mlp_set_params(pecs_mlp* net, const float* params);
+int pecs_mlp_gradients(pecs_mlp* net, const float* x, const float* y, int batch, float* grads, float* loss);
+int pecs_mlp_train_step(pecs_mlp* net, const float* x, const float* y, int batch, float lr, float* loss);
+int pecs_mlp_predict(pecs_mlp* net, const float* x, int batch, float* y, int bf16);
+const char* pecs_mlp_last_error(void);
+
 /* pecs_step replays one time step captured as a HIP graph (captured at the
  * first call; kernels and stream dependencies only).  enable = 0 launches the
  * kernels one by one instead; the results are identical.  No reference

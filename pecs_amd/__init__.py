@@ -26,7 +26,9 @@ ABI_FUNCTIONS = [
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config",
     "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
     "pecs_last_residual_batch", "pecs_write_vtu", "pecs_use_graph", "pecs_sweep_create", "pecs_sweep_set_active",
-    "pecs_sweep_step", "pecs_sweep_destroy",
+    "pecs_sweep_step", "pecs_sweep_destroy", "pecs_mlp_create", "pecs_mlp_destroy", "pecs_mlp_num_params",
+    "pecs_mlp_get_params", "pecs_mlp_set_params", "pecs_mlp_gradients", "pecs_mlp_train_step", "pecs_mlp_predict",
+    "pecs_mlp_last_error",
 ]
 
 
@@ -380,6 +382,82 @@ class SolarCellSweep:
     def close(self):
         if self.h:
             lib().pecs_sweep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WarmStartNet:
+    """The warm-start network (pecs_mlp_*): nin -> nlayers x nhidden (ReLU) ->
+    nout, trained in FP32 on the GPU, evaluated in BF16 (or FP32)."""
+
+    def __init__(self, nin, nhidden, nlayers, nout, seed=0):
+        L = lib()
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.pecs_mlp_create.argtypes = [ctypes.c_int] * 4 + [ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]
+        L.pecs_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.pecs_mlp_num_params.argtypes = [ctypes.c_void_p]
+        L.pecs_mlp_num_params.restype = ctypes.c_int64
+        L.pecs_mlp_get_params.argtypes = [ctypes.c_void_p, fp]
+        L.pecs_mlp_set_params.argtypes = [ctypes.c_void_p, fp]
+        L.pecs_mlp_gradients.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_int, fp, fp]
+        L.pecs_mlp_train_step.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_int, ctypes.c_float, fp]
+        L.pecs_mlp_predict.argtypes = [ctypes.c_void_p, fp, ctypes.c_int, fp, ctypes.c_int]
+        L.pecs_mlp_last_error.restype = ctypes.c_char_p
+        self.nin, self.nout = nin, nout
+        self.h = ctypes.c_void_p()
+        self._c(L.pecs_mlp_create(nin, nhidden, nlayers, nout, seed, ctypes.byref(self.h)))
+        self.nparams = int(L.pecs_mlp_num_params(self.h))
+
+    @staticmethod
+    def _c(rc):
+        if rc != 0:
+            msg = lib().pecs_mlp_last_error()
+            raise RuntimeError("pecs_mlp: " + (msg.decode() if msg else f"error {rc}"))
+
+    @staticmethod
+    def _f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def params(self):
+        out = np.zeros(self.nparams, np.float32)
+        self._c(lib().pecs_mlp_get_params(self.h, self._f(out)[1]))
+        return out
+
+    def set_params(self, p):
+        a, ptr = self._f(p)
+        assert a.size == self.nparams
+        self._c(lib().pecs_mlp_set_params(self.h, ptr))
+
+    def gradients(self, x, y):
+        xa, xp = self._f(x)
+        ya, yp = self._f(y)
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        self._c(lib().pecs_mlp_gradients(self.h, xp, yp, xa.shape[0], self._f(g)[1], ctypes.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, x, y, lr):
+        xa, xp = self._f(x)
+        ya, yp = self._f(y)
+        loss = ctypes.c_float()
+        self._c(lib().pecs_mlp_train_step(self.h, xp, yp, xa.shape[0], float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def predict(self, x, bf16=True):
+        xa, xp = self._f(x)
+        y = np.zeros((xa.shape[0], self.nout), np.float32)
+        self._c(lib().pecs_mlp_predict(self.h, xp, xa.shape[0], self._f(y)[1], int(bool(bf16))))
+        return y
+
+    def close(self):
+        if self.h:
+            lib().pecs_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

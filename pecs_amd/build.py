@@ -20,7 +20,7 @@ RUNNER = os.path.join(BUILD, "pecs_run")
 HOST_TEST = os.path.join(BUILD, "pecs_host_test")
 
 HOST_SRC = ["mesh.cpp", "ops.cpp", "cholesky.cpp", "vtu.cpp"]
-HIP_SRC = ["kernels.hip", "devfactor.hip", "abi.hip"]
+HIP_SRC = ["kernels.hip", "devfactor.hip", "mlp.hip", "abi.hip"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-mavx2", "-mfma"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -52,8 +52,8 @@ struct pecs_ctx {
   // one time step captured as a graph (pecs_step replays it); cap is the
   // stream it is captured on (capture needs a stream other than the null stream)
   hipStream_t cap = nullptr;
-  hipGraph_t step_graph = nullptr;
-  hipGraphExec_t step_exec = nullptr;
+  hipGraph_t step_graph = nullptr, last_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr, last_exec = nullptr;
   bool use_graph = true;
   Params P;                  // parameters of instance 0 (matrix parameters are common to all)
   int B = 1;                 // instances sharing mesh, matrices and factors (a bias / illumination / doping batch)
@@ -101,6 +101,8 @@ struct pecs_ctx {
   ~pecs_ctx() {
     if (step_exec) (void)hipGraphExecDestroy(step_exec);
     if (step_graph) (void)hipGraphDestroy(step_graph);
+    if (last_exec) (void)hipGraphExecDestroy(last_exec);
+    if (last_graph) (void)hipGraphDestroy(last_graph);
     if (cap) {
       (void)hipStreamSynchronize(cap);
       (void)hipStreamDestroy(cap);
@@ -254,6 +256,8 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
     HIPCHECK(hipMemsetAsync(d.Z, 0, (size_t)nrs * std::max<int64_t>(P.zlen, 1) * sizeof(double), s));
     std::vector<int> tl, rtl;
     std::vector<int4> t4, r4;
+    const char* tr = std::getenv("PECS_SETUP_TRACE");
+    const bool trace = tr && std::string(tr) == "1";
     P.fks.assign(P.nlevels, 0);
     P.bks.assign(P.nlevels, 0);
     for (int l = 0; l < P.nlevels; ++l) {
@@ -331,6 +335,9 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
             bk = ks;
           }
         }
+        if (trace)
+          std::fprintf(stderr, "[setup] tune n=%d level %d %s: W %d MAXR %d rows %d segment %d (%.3f ms)\n", P.n, l,
+                       fwd ? "fwd" : "bwd", bw, bm, br, bk, best);
         if (fwd) {
           P.fw[l] = bw;
           P.fm[l] = bm;
@@ -668,12 +675,49 @@ void step_once(pecs_ctx* c) {
   pm.mark();
 }
 
-// Capture step_once into c->step_exec (once per context).  The capture runs
-// on c->cap with the context's own stream swapped for it; the side-stream
-// fork/join of the region solves becomes part of the graph.  The step has no
-// memset or host work, only kernels and event dependencies.
-void ensure_step_graph(pecs_ctx* c) {
-  if (c->step_exec) return;
+// The state before the last step of a pecs_step call (for the steady residual).
+// Copy kernels, not memcpys, so that they become kernel nodes of the
+// last-step graph.  (Work queued on a non-blocking stream right behind a
+// graph launch was seen to overlap the graph's tail; inside the graph the
+// copy, the step and the residual are ordered by the graph's edges.)
+void save_old(pecs_ctx* c, hipStream_t s) {
+  const int64_t nsv[5] = {c->Ns, c->Ns, c->Ne, c->Ne, c->NP};
+  for (int j = 0; j < 4; ++j) HIPCHECK((hipError_t)launch_copy(c->u[j], c->old[j], c->B * nsv[j], s));
+  HIPCHECK((hipError_t)launch_copy(c->pv, c->old[4], c->B * nsv[4], s));
+}
+
+// ||x^{n+1} - x^n||^2 and ||x^n||^2 over the five state vectors, per
+// instance, into c->acc (2 per instance).
+void launch_residual(pecs_ctx* c, hipStream_t s) {
+  const int64_t nsv[5] = {c->Ns, c->Ns, c->Ne, c->Ne, c->NP};
+  const int B = c->B;
+  for (int b = 0; b < B; ++b) {
+    for (int j = 0; j < 5; ++j) {
+      const double* cur = j < 4 ? c->u[j] : c->pv;
+      HIPCHECK((hipError_t)launch_diff_norms(cur + b * nsv[j], c->old[j] + b * nsv[j], nsv[j],
+                                             c->rpart + (size_t)(5 * b + j) * 2 * RES_PARTS, s));
+    }
+  }
+  HIPCHECK((hipError_t)launch_sum_partials(c->rpart, 5, c->acc, B, s));
+}
+
+// c->acc to the host (after the stream has drained), then the residuals.
+void finish_residual(pecs_ctx* c) {
+  std::vector<double> h(2 * c->B);
+  HIPCHECK(hipMemcpy(h.data(), c->acc, 2 * c->B * sizeof(double), hipMemcpyDeviceToHost));
+  for (int b = 0; b < c->B; ++b)
+    c->last_res[b] = (h[2 * b + 1] > 0) ? std::sqrt(h[2 * b] / h[2 * b + 1]) / c->P.dt : 0.0;
+}
+
+
+// Capture into a graph (once per context), on c->cap with the context's own
+// stream swapped for it; the side-stream fork/join of the region solves
+// becomes part of the graph.  Two graphs: one time step, and the last step
+// of a pecs_step call with the copy of the state before it and the steady-
+// residual kernels after it, so that these are ordered by the graph's own
+// edges.  Neither holds a memset or host work: kernels and event
+// dependencies only.
+void capture(pecs_ctx* c, bool last, hipGraph_t& graph, hipGraphExec_t& exec) {
   hipStream_t keep = c->stream;
   const int timing = c->timing;
   c->stream = c->cap;
@@ -682,7 +726,9 @@ void ensure_step_graph(pecs_ctx* c) {
   try {
     HIPCHECK(hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal));
     try {
+      if (last) save_old(c, c->cap);
       step_once(c);
+      if (last) launch_residual(c, c->cap);
     } catch (...) {
       (void)hipStreamEndCapture(c->cap, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -691,8 +737,8 @@ void ensure_step_graph(pecs_ctx* c) {
     HIPCHECK(hipStreamEndCapture(c->cap, &g));
     hipGraphExec_t e = nullptr;
     HIPCHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
-    c->step_graph = g;
-    c->step_exec = e;
+    graph = g;
+    exec = e;
   } catch (...) {
     c->stream = keep;
     c->timing = timing;
@@ -702,14 +748,22 @@ void ensure_step_graph(pecs_ctx* c) {
   c->timing = timing;
 }
 
-// One time step: the captured graph, or the launches themselves when graphs
-// are off or per-phase timing is on (the timing events are host-recorded).
-void run_step(pecs_ctx* c) {
+void ensure_step_graph(pecs_ctx* c) {
+  if (!c->step_exec) capture(c, false, c->step_graph, c->step_exec);
+  if (!c->last_exec) capture(c, true, c->last_graph, c->last_exec);
+}
+
+// One time step, launched on stream s: the captured graph (the last-step
+// variant when last), or the launches themselves when graphs are off or
+// per-phase timing is on (the timing events are host-recorded).
+void run_step(pecs_ctx* c, bool last) {
   if (c->use_graph && !c->timing) {
     ensure_step_graph(c);
-    HIPCHECK(hipGraphLaunch(c->step_exec, c->stream));
+    HIPCHECK(hipGraphLaunch(last ? c->last_exec : c->step_exec, c->stream));
   } else {
+    if (last) save_old(c, c->stream);
     step_once(c);
+    if (last) launch_residual(c, c->stream);
   }
 }
 
@@ -1274,63 +1328,18 @@ int pecs_initial_state(pecs_ctx* ctx) {
   })
 }
 
-namespace {
-
-// The state before the last step of a pecs_step call (for the steady residual).
-// Copy kernels, not memcpys: they queue behind the step graphs on the same
-// compute queue.  (A device-to-device hipMemcpyAsync issued on a non-blocking
-// stream right after a graph launch was seen to read the state before the
-// graph had finished.)
-void save_old(pecs_ctx* c, hipStream_t s) {
-  const int64_t nsv[5] = {c->Ns, c->Ns, c->Ne, c->Ne, c->NP};
-  for (int j = 0; j < 4; ++j) HIPCHECK((hipError_t)launch_copy(c->u[j], c->old[j], c->B * nsv[j], s));
-  HIPCHECK((hipError_t)launch_copy(c->pv, c->old[4], c->B * nsv[4], s));
-}
-
-// ||x^{n+1} - x^n||^2 and ||x^n||^2 over the five state vectors, per
-// instance, into c->acc (2 per instance), then to h (host, 2 B doubles).
-void launch_residual(pecs_ctx* c, hipStream_t s, double* h) {
-  const int64_t nsv[5] = {c->Ns, c->Ns, c->Ne, c->Ne, c->NP};
-  const int B = c->B;
-  for (int b = 0; b < B; ++b) {
-    for (int j = 0; j < 5; ++j) {
-      const double* cur = j < 4 ? c->u[j] : c->pv;
-      HIPCHECK((hipError_t)launch_diff_norms(cur + b * nsv[j], c->old[j] + b * nsv[j], nsv[j],
-                                             c->rpart + (size_t)(5 * b + j) * 2 * RES_PARTS, s));
-    }
-  }
-  HIPCHECK((hipError_t)launch_sum_partials(c->rpart, 5, c->acc, B, s));
-  HIPCHECK(hipMemcpyAsync(h, c->acc, 2 * B * sizeof(double), hipMemcpyDeviceToHost, s));
-}
-
-void finish_residual(pecs_ctx* c, const double* h) {
-  for (int b = 0; b < c->B; ++b)
-    c->last_res[b] = (h[2 * b + 1] > 0) ? std::sqrt(h[2 * b] / h[2 * b + 1]) / c->P.dt : 0.0;
-}
-
-}  // namespace
 
 int pecs_step(pecs_ctx* ctx, int nsteps) {
   PECS_TRY(ctx, {
     if (nsteps < 0) throw std::runtime_error("nsteps must be >= 0");
-    const bool graph = ctx->use_graph && !ctx->timing;
     for (int i = 0; i < nsteps; ++i) {
-      if (i == nsteps - 1) {
-        // with graph replay the host waits around the last step (see pecs_sweep_step)
-        if (graph) HIPCHECK(hipStreamSynchronize(ctx->stream));
-        save_old(ctx, ctx->stream);
-        if (graph) HIPCHECK(hipStreamSynchronize(ctx->stream));
-      }
-      run_step(ctx);
+      run_step(ctx, i == nsteps - 1);
       if (ctx->timing && ctx->ev_used + NPHASE + 1 > (int)ctx->events.size()) collect_timing(ctx);
     }
     collect_timing(ctx);
     if (nsteps > 0) {
-      if (graph) HIPCHECK(hipStreamSynchronize(ctx->stream));
-      std::vector<double> h(2 * ctx->B);
-      launch_residual(ctx, ctx->stream, h.data());
       HIPCHECK(hipStreamSynchronize(ctx->stream));
-      finish_residual(ctx, h.data());
+      finish_residual(ctx);
     }
   })
 }
@@ -1399,31 +1408,14 @@ int pecs_sweep_step(pecs_sweep* w, int nsteps) {
       ensure_step_graph(w->m[i]);
       q_of[i] = slot++ % ns;
     }
-    for (int t = 0; t < nsteps; ++t) {
-      // Around the last step the host waits for the streams: the copy of the
-      // state before it and the residual kernels after it must not overlap
-      // a step graph (on non-blocking streams, kernels queued right behind a
-      // graph launch were seen to overlap the graph's tail).
-      if (t == nsteps - 1) {
-        for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
-        for (size_t i = 0; i < w->m.size(); ++i)
-          if (q_of[i] >= 0) save_old(w->m[i], w->st[q_of[i]]);
-        for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
-      }
+    for (int t = 0; t < nsteps; ++t)
       for (size_t i = 0; i < w->m.size(); ++i)
-        if (q_of[i] >= 0) HIPCHECK(hipGraphLaunch(w->m[i]->step_exec, w->st[q_of[i]]));
-    }
+        if (q_of[i] >= 0)
+          HIPCHECK(hipGraphLaunch(t == nsteps - 1 ? w->m[i]->last_exec : w->m[i]->step_exec, w->st[q_of[i]]));
     if (nsteps > 0) {
       for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
-      std::vector<std::vector<double>> h(w->m.size());
       for (size_t i = 0; i < w->m.size(); ++i)
-        if (q_of[i] >= 0) {
-          h[i].assign(2 * w->m[i]->B, 0.0);
-          launch_residual(w->m[i], w->st[q_of[i]], h[i].data());
-        }
-      for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
-      for (size_t i = 0; i < w->m.size(); ++i)
-        if (q_of[i] >= 0) finish_residual(w->m[i], h[i].data());
+        if (q_of[i] >= 0) finish_residual(w->m[i]);
     }
   } catch (const std::exception& e) {
     for (hipStream_t s : w->st) (void)hipStreamSynchronize(s);
