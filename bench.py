@@ -301,27 +301,29 @@ def warm_start_leg(args):
                                                         args.steady_check_every, args.steady_max_steps, log)
     log(f"training set done: {wall_tr:.1f} s stepping, {setup_tr:.1f} s setup")
     t0 = time.perf_counter()
-    net, norm, hist = ws.train(Xtr, Ytr, epochs=args.ws_epochs, log=log)
+    net, norm, hist = ws.train(Xtr, Ytr, hidden=args.ws_hidden, epochs=args.ws_epochs, log=log)
     t_train = time.perf_counter() - t0
     log(f"cold start of {16 * len(test_groups)} held-out draws")
     Xte, Yte, steps_cold, wall_cold, _, res_cold = ws.generate(test_groups, r, k, dt, args.steady_tol,
                                                      args.steady_check_every, args.steady_max_steps, log)
     log("warm start of the held-out draws")
     ev = ws.evaluate(net, norm, test_groups, r, k, dt, args.steady_tol, args.steady_check_every,
-                     args.steady_max_steps, log=log)
+                     args.steady_max_steps, fallback_residual=args.ws_fallback, log=log)
     p32 = norm.back(net.predict(Xte, bf16=False))
     err = np.linalg.norm(ev["pred"] - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
     err32 = np.linalg.norm(p32 - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
     n_te = len(Yte)
     net.close()
     return {"refinement": r, "train_draws": len(Ytr), "test_draws": n_te, "outputs": int(Ytr.shape[1]),
-            "network": "4 -> 3 x 256 ReLU -> outputs; FP32 Adam training, BF16 inference",
+            "network": f"4 -> 3 x {args.ws_hidden} ReLU -> outputs; FP32 Adam training, BF16 inference",
+            "epochs": args.ws_epochs, "fallback_residual": args.ws_fallback,
             "train_s": round(t_train, 2), "train_loss_first_last": [hist[0], hist[-1]],
             "prediction_rel_error_bf16": {"median": float(np.median(err)), "max": float(err.max())},
             "prediction_rel_error_fp32": {"median": float(np.median(err32)), "max": float(err32.max())},
             "first_block_residual_median": {"warm": float(np.median(ev["first_block_residual"])),
                                             "cold": float(np.median(res_cold))},
             "fallbacks": ev["fallbacks"], "tol": args.steady_tol,
+            "warm_steps_median": float(np.median(ev["steps"])), "cold_steps_median": float(np.median(steps_cold)),
             "cold_steps_mean": float(steps_cold.mean()), "warm_steps_mean": float(ev["steps"].mean()),
             "cold_s_per_bias_point": wall_cold / n_te, "warm_s_per_bias_point": ev["wall"] / n_te,
             "warm_reached": bool(np.all(ev["steps"] > 0)),
@@ -360,8 +362,11 @@ def main():
     ap.add_argument("--ws-refine", type=int, default=5)
     ap.add_argument("--ws-train-groups", type=int, default=64, help="16 draws per group (one cone angle)")
     ap.add_argument("--ws-test-groups", type=int, default=8)
-    ap.add_argument("--ws-epochs", type=int, default=300)
+    ap.add_argument("--ws-epochs", type=int, default=600)
     ap.add_argument("--ws-seed", type=int, default=20261202)
+    ap.add_argument("--ws-fallback", type=float, default=10.0,
+                    help="restart a warm-started instance cold if its residual after the first block is above this")
+    ap.add_argument("--ws-hidden", type=int, default=512)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying captured step graphs (counter collection)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,

@@ -197,10 +197,11 @@ int pecs_mlp_train_step(pecs_mlp* net, const float* x, const float* y, int batch
 int pecs_mlp_predict(pecs_mlp* net, const float* x, int batch, float* y, int bf16);
 const char* pecs_mlp_last_error(void);
 
-/* pecs_step replays one time step captured as a HIP graph (captured at the
- * first call; kernels and stream dependencies only).  enable = 0 launches the
- * kernels one by one instead; the results are identical.  No reference
- * counterpart (launch mechanics of this library). */
+/* pecs_step replays time steps captured as HIP graphs at its first call: one
+ * step, and the last step of a call together with the copy of the state
+ * before it and the steady-residual kernels (kernels and stream dependencies
+ * only).  enable = 0 launches the kernels one by one instead; the results are
+ * identical.  No reference counterpart (launch mechanics of this library). */
 int pecs_use_graph(pecs_ctx* ctx, int enable);
 /* Per-phase timing of pecs_step with hipEvents on the context stream
  * (the TimerOutput sections of SolarCell.cpp:1905-2087): phases 0 rhs semi,
