@@ -309,6 +309,23 @@ def warm_start_leg(args):
     log("warm start of the held-out draws")
     ev = ws.evaluate(net, norm, test_groups, r, k, dt, args.steady_tol, args.steady_check_every,
                      args.steady_max_steps, fallback_residual=args.ws_fallback, log=log)
+    # the plain path: one instance alone, launch by launch (4 held-out draws)
+    import torch
+    from pecs_amd import SolarCellGPU
+    plain = []
+    ang, inst = test_groups[0]
+    for d in inst[:4]:
+        g = SolarCellGPU(ws.instance_params(r, k, dt, ang, *d))
+        g.use_graph(False)
+        g.initial_state()
+        g.step(5)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g.step(100)
+        torch.cuda.synchronize()
+        plain.append((time.perf_counter() - t1) / 100)
+        g.close()
+    plain_s = float(np.mean(plain)) * float(steps_cold.mean())
     p32 = norm.back(net.predict(Xte, bf16=False))
     err = np.linalg.norm(ev["pred"] - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
     err32 = np.linalg.norm(p32 - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
@@ -327,7 +344,9 @@ def warm_start_leg(args):
             "cold_steps_mean": float(steps_cold.mean()), "warm_steps_mean": float(ev["steps"].mean()),
             "cold_s_per_bias_point": wall_cold / n_te, "warm_s_per_bias_point": ev["wall"] / n_te,
             "warm_reached": bool(np.all(ev["steps"] > 0)),
-            "speedup_warm_vs_cold": wall_cold / ev["wall"] if ev["wall"] > 0 else None}
+            "speedup_warm_vs_cold": wall_cold / ev["wall"] if ev["wall"] > 0 else None,
+            "plain_ms_per_step": 1e3 * float(np.mean(plain)), "plain_s_per_bias_point": plain_s,
+            "speedup_batched_warm_vs_plain": plain_s / (ev["wall"] / n_te) if ev["wall"] > 0 else None}
 
 
 def main():

@@ -7,7 +7,7 @@
 //                   the other region evaluated at QGauss(k+2) points.
 //   k_schur_rhs     g = f_v + B^T W (cell + 4 face neighbours), with
 //                   W = mu A^{-1} f_p recomputed per cell from the RHS.
-//   k_fwd_gather / k_sweep / k_bwd_gather  supernodal triangular solves
+//   k_fwd_gather / k_sweep  supernodal triangular solves
 //                   with S = L L^T, level by level on the elimination tree:
 //                   row tiles of dense [Linv; L21 Linv] panels, vector chunks
 //                   staged in LDS, W lanes per row (deterministic, no atomics).
@@ -470,15 +470,6 @@ __global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int* __re
   if (lane == 0) F.X[(int64_t)h * F.n + p] = b[(int64_t)h * F.n + p] - z;
 }
 
-// backward, part 1: Z[q] = x[rows[q]] for the panel rows of the level's nodes
-__global__ __launch_bounds__(256) void k_bwd_gather(DevFactor F, const int64_t* __restrict__ idx, int nidx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nidx) return;
-  const int h = blockIdx.y;
-  const int64_t q = idx[i];
-  F.Z[(int64_t)h * F.zlen + q] = F.X[(int64_t)h * F.n + F.rows[q]];
-}
-
 template <int NRHS, int W, int MAXR_, bool FWD>
 __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restrict__ tiles,
                                                const double* __restrict__ b0, const double* __restrict__ b1,
@@ -545,10 +536,10 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
       } else if (pos < ns) {
         vec[0][i] = Y0[p0 + pos];
         if (NRHS > 1) vec[NRHS - 1][i] = Y1[p0 + pos];
-      } else {  // x at the panel rows, pre-gathered into Z by k_bwd_gather
-        const int64_t q = F.rows_off[s] + (pos - ns);
-        vec[0][i] = F.Z[q];
-        if (NRHS > 1) vec[NRHS - 1][i] = F.Z[F.zlen + q];
+      } else {  // x at the panel rows: rows of ancestors, solved by earlier launches
+        const int64_t p = F.rows[F.rows_off[s] + (pos - ns)];
+        vec[0][i] = F.X[p];
+        if (NRHS > 1) vec[NRHS - 1][i] = F.X[F.n + p];
       }
     }
     __syncthreads();
@@ -1197,8 +1188,7 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
     if (e) return e;
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
-    const int nx = L.xoff[l + 1] - L.xoff[l];
-    if (nx > 0) k_bwd_gather<<<dim3(nblk(nx, 256), nrhs), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
+    // the backward sweep reads the panel rows' x straight from X (no gather)
     const int nt = L.boff[l + 1] - L.boff[l];
     const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s);
     if (e) return e;
