@@ -662,6 +662,10 @@ PackedFactor pack_layout(const SupernodalFactor& F) {
   rebuild_tiles(P);
   P.rows = F.rows;
   P.cptr = F.contrib_ptr;
+  // the solves read the leaf level's right-hand side without a gather
+  for (int q = P.goff[0]; q < P.goff[1]; ++q)
+    if (P.cptr[P.lrows[q] + 1] != P.cptr[P.lrows[q]])
+      throw std::runtime_error("pack_layout: a leaf-level row is owed contributions");
   P.czidx.resize(F.contrib_node.size());
   for (size_t t = 0; t < P.czidx.size(); ++t) P.czidx[t] = P.z_off[F.contrib_node[t]] + F.contrib_row[t];
   return P;

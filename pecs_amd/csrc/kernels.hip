@@ -530,9 +530,10 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
     __syncthreads();
     for (int i = threadIdx.x; i < cl; i += 256) {
       const int pos = c0 + i;
-      if (FWD) {
-        vec[0][i] = F.X[p0 + pos];
-        if (NRHS > 1) vec[NRHS - 1][i] = F.X[F.n + p0 + pos];
+      if (FWD) {  // t from X, or straight from the right-hand side b0 (leaf level: nothing to gather)
+        const double* __restrict__ xs = b0 ? b0 : F.X;
+        vec[0][i] = xs[p0 + pos];
+        if (NRHS > 1) vec[NRHS - 1][i] = xs[F.n + p0 + pos];
       } else if (pos < ns) {
         vec[0][i] = Y0[p0 + pos];
         if (NRHS > 1) vec[NRHS - 1][i] = Y1[p0 + pos];
@@ -1130,7 +1131,7 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
 int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : 32; }
 
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 double* t, hipStream_t s, int ks, const int4* rtiles, int nrt) {
+                 double* t, hipStream_t s, int ks, const int4* rtiles, int nrt, const double* src) {
   if (nt <= 0) return 0;
   if (w == 0) {
     if (nrhs > 32) return (int)hipErrorInvalidValue;
@@ -1139,7 +1140,7 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
                : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s);
   }
   if (nrhs > 2 || ks > 0) return (int)hipErrorInvalidValue;
-  const double* v0 = nullptr;
+  const double* v0 = fwd ? src : nullptr;
   double* t1 = nrhs == 2 ? t + F.n : t;
   if (fwd) {
     return nrhs == 2 ? sweep_dispatch<2, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s)
@@ -1180,11 +1181,14 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
   }
   if (nrhs > 2) return (int)hipErrorInvalidValue;
   for (int l = 0; l < L.nlevels; ++l) {
+    // level 0 (leaves) owes nothing to its rows: its sweep reads the right-hand side itself
     const int ng = L.goff[l + 1] - L.goff[l];
-    if (ng >= 16384) k_fwd_gather<1><<<dim3(nblk(ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
-    else if (ng > 0) k_fwd_gather<8><<<dim3(nblk(8 * (int64_t)ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
+    if (l > 0 && ng >= 16384) k_fwd_gather<1><<<dim3(nblk(ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
+    else if (l > 0 && ng > 0)
+      k_fwd_gather<8><<<dim3(nblk(8 * (int64_t)ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
     const int nt = L.foff[l + 1] - L.foff[l];
-    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s);
+    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s, 0, nullptr, 0,
+                               l == 0 ? v : nullptr);
     if (e) return e;
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {

@@ -131,8 +131,11 @@ int launch_schur_recover(int k, const DevRegion& R, const double* b1, const doub
 // right-hand sides; ks > 0 splits its tiles into column segments of ks
 // columns whose partial sums (F.Part) are added by a reduction over rtiles.
 // Vectors are h-major: right-hand side h at t[h * n].
+// src (scalar forward sweeps only): read t from src instead of X (the leaf
+// level, whose rows are owed no contributions).
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 double* t, hipStream_t s, int ks = 0, const int4* rtiles = nullptr, int nrt = 0);
+                 double* t, hipStream_t s, int ks = 0, const int4* rtiles = nullptr, int nrt = 0,
+                 const double* src = nullptr);
 // Padded right-hand-side count of the MFMA sweep (16 or 32).
 int mfma_rhs_pad(int nrhs);
 // x = S^{-1} v for nrhs h-major right-hand sides in the factor's permuted
