@@ -169,7 +169,8 @@ int pecs_last_residual_batch(pecs_ctx* ctx, double* residual);  /* one per insta
  * their steps overlap.  pecs_sweep_step
  * updates every active member's steady residual (pecs_last_residual[_batch]);
  * set_active takes one flag per member (0: left out of later steps, e.g. once
- * steady).  The members must not be used elsewhere during pecs_sweep_step.
+ * steady).  The members must not be used elsewhere during pecs_sweep_step and
+ * must outlive the sweep (destroy the sweep first).
  * Errors: pecs_global_error().  No reference counterpart (the reference runs
  * one problem per process). */
 typedef struct pecs_sweep pecs_sweep;
