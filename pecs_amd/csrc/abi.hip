@@ -2,6 +2,7 @@
 // Declarations and reference citations: include/pecs_abi.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cstdio>
@@ -9,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -514,6 +516,29 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   return D;
 }
 
+// Cells 0..n-1 grouped by a key of nkey ints (ascending cell order within a
+// group), each group padded with -1 to a whole number of 64-cell waves.
+template <class KeyFn>
+std::vector<int> wave_groups(int n, int nkey, KeyFn key) {
+  std::vector<int> keys((size_t)n * nkey);
+  for (int i = 0; i < n; ++i) key(i, &keys[(size_t)i * nkey]);
+  const auto kb = [&](int i) { return keys.begin() + (size_t)i * nkey; };
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return std::lexicographical_compare(kb(a), kb(a) + nkey, kb(b), kb(b) + nkey);
+  });
+  std::vector<int> out;
+  for (int i = 0; i < n;) {
+    int j = i;
+    while (j < n && std::equal(kb(order[i]), kb(order[i]) + nkey, kb(order[j]))) ++j;
+    out.insert(out.end(), order.begin() + i, order.begin() + j);
+    while (out.size() % 64) out.push_back(-1);
+    i = j;
+  }
+  return out;
+}
+
 DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, const std::vector<int>& sigma,
                           const FactorDev& F) {
   DevRegion d{};
@@ -532,6 +557,31 @@ DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, c
   std::vector<int> pb(R.ncell);
   for (int i = 0; i < R.ncell; ++i) pb[i] = F.cell_base[R.ci[i] + R.N * R.cj[i]];
   d.perm_base = upload(c, pb);
+  // Wave-uniform cell lists for the Schur kernels, kept when the padding to
+  // whole waves adds at most max(ncell, 2048) cells (meshes with few
+  // operator types; on conic meshes every cell has its own type and the
+  // lists are not used).
+  // PECS_SCHUR_UNIFORM=0 turns them off.
+  const char* su = std::getenv("PECS_SCHUR_UNIFORM");
+  if (!(su && su[0] == '0')) {
+    std::vector<int> own = wave_groups(R.ncell, 1, [&](int M, int* key) { key[0] = O.type[M]; });
+    std::vector<int> nbh = wave_groups(R.ncell, 5, [&](int X, int* key) {
+      key[0] = O.type[X];
+      for (int f = 0; f < 4; ++f) {
+        const int M = R.nbr[4 * X + f];
+        key[1 + f] = M < 0 ? -1 : O.type[M];
+      }
+    });
+    const size_t lim = std::max<size_t>(2 * (size_t)R.ncell, (size_t)R.ncell + 2048);
+    if (own.size() <= lim) {
+      d.ucell_own = upload(c, own);
+      d.nu_own = (int)own.size();
+    }
+    if (nbh.size() <= lim) {
+      d.ucell_nbh = upload(c, nbh);
+      d.nu_nbh = (int)nbh.size();
+    }
+  }
   return d;
 }
 
@@ -923,6 +973,14 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     P.type = upload(c.get(), op.type);
     P.table = upload(c.get(), op.table);
     P.stride = op.type_stride;
+    {  // wave-uniform cell list for the recovery (as for the Schur kernels)
+      const char* su = std::getenv("PECS_SCHUR_UNIFORM");
+      std::vector<int> own = wave_groups(m.ncellP, 1, [&](int K, int* key) { key[0] = op.type[K]; });
+      if (!(su && su[0] == '0') && own.size() <= std::max<size_t>(2 * (size_t)m.ncellP, (size_t)m.ncellP + 2048)) {
+        P.ucell = upload(c.get(), own);
+        P.nucell = (int)own.size();
+      }
+    }
     P.pbid = upload(c.get(), m.pbid);
     P.pI = upload(c.get(), m.pI);
     P.pJ = upload(c.get(), m.pJ);

@@ -261,10 +261,10 @@ __global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, De
 // the cell-local inverse mass block times the cell's two current components.
 template <int K>
 __device__ __forceinline__ void schur_w(const DevRegion& R, const double* __restrict__ b, double mu, int M,
-                                        double* w) {
+                                        int ty, double* w) {
   constexpr int NB = Sz<K>::NB;
   const size_t nc = R.ncell;
-  const double* Minv = R.table + (size_t)R.type[M] * R.stride;
+  const double* Minv = R.table + (size_t)ty * R.stride;
   double fx[NB], fy[NB];
 #pragma unroll
   for (int a = 0; a < NB; ++a) {
@@ -285,16 +285,33 @@ __device__ __forceinline__ void schur_w(const DevRegion& R, const double* __rest
   }
 }
 
+// Cell of thread t.  UNI: cells come from a list grouped so that every
+// wave's cells share the operator types the kernel reads (groups padded to
+// whole waves with -1), and the table offsets are made wave-uniform: the
+// table entries are then fetched once per wave through the scalar cache and
+// used as scalar operands, instead of one vector load per lane and entry.
+template <bool UNI>
+__device__ __forceinline__ int schur_cell(const int* __restrict__ list, int n, int t) {
+  if (UNI) return t < n ? list[t] : -1;
+  return t < n ? t : -1;
+}
+
+template <bool UNI>
+__device__ __forceinline__ int wave_type(const DevRegion& R, int M) {
+  const int t = R.type[M];
+  return UNI ? __builtin_amdgcn_readfirstlane(t) : t;
+}
+
 // g = f_v + B^T W of cell X: its own block and its 4 face neighbours' blocks,
 // each neighbour's W recomputed from its right-hand side (no W in memory)
-template <int K, int NRHS>
+template <int K, int NRHS, bool UNI>
 __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __restrict__ b1,
                                                   const double* __restrict__ b2, double mu1, double mu2,
                                                   double* __restrict__ g1, double* __restrict__ g2, int64_t sb,
                                                   int64_t sg) {
   constexpr int NB = Sz<K>::NB;
-  const int X = blockIdx.x * blockDim.x + threadIdx.x;
-  if (X >= R.ncell) return;
+  const int X = schur_cell<UNI>(R.ucell_nbh, UNI ? R.nu_nbh : R.ncell, blockIdx.x * blockDim.x + threadIdx.x);
+  if (X < 0) return;
   {
     const int64_t i = blockIdx.y;
     b1 += i * sb;
@@ -312,11 +329,12 @@ __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __r
   for (int f = -1; f < 4; ++f) {
     const int M = f < 0 ? X : R.nbr[4 * (size_t)X + f];
     if (M < 0) continue;
-    const double* B = R.table + (size_t)R.type[M] * R.stride + NB * NB +
+    const int ty = wave_type<UNI>(R, M);
+    const double* B = R.table + (size_t)ty * R.stride + NB * NB +
                       (f < 0 ? 0 : 2 * NB * NB + (f ^ 1) * 2 * NB * NB);
     double w1[2 * NB], w2[2 * NB];
-    schur_w<K>(R, b1, mu1, M, w1);
-    if (NRHS > 1) schur_w<K>(R, b2, mu2, M, w2);
+    schur_w<K>(R, b1, mu1, M, ty, w1);
+    if (NRHS > 1) schur_w<K>(R, b2, mu2, M, ty, w2);
     for (int i = 0; i < 2 * NB; ++i) {
 #pragma unroll
       for (int a = 0; a < NB; ++a) {
@@ -334,15 +352,15 @@ __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __r
 }
 
 // q = W - A^{-1} B u; writes [Jx | Jy | u] (W recomputed from the RHS b)
-template <int K, int NRHS>
+template <int K, int NRHS, bool UNI>
 __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double* __restrict__ b1,
                                                       const double* __restrict__ b2, const double* __restrict__ x1,
                                                       const double* __restrict__ x2, double mu1, double mu2,
                                                       double* __restrict__ o1, double* __restrict__ o2, int64_t sb,
                                                       int64_t sx, int64_t so) {
   constexpr int NB = Sz<K>::NB;
-  const int M = blockIdx.x * blockDim.x + threadIdx.x;
-  if (M >= R.ncell) return;
+  const int M = schur_cell<UNI>(R.ucell_own, UNI ? R.nu_own : R.ncell, blockIdx.x * blockDim.x + threadIdx.x);
+  if (M < 0) return;
   {
     const int64_t i = blockIdx.y;
     b1 += i * sb;
@@ -353,7 +371,8 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
     o2 += i * so;
   }
   const size_t nc = R.ncell;
-  const double* tb = R.table + (size_t)R.type[M] * R.stride;
+  const int ty = wave_type<UNI>(R, M);
+  const double* tb = R.table + (size_t)ty * R.stride;
   double s1[2 * NB], s2[2 * NB], u1[NB], u2[NB];
   const int pb = R.perm_base[M];
 #pragma unroll
@@ -394,8 +413,8 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
     }
   }
   double w1[2 * NB], w2[2 * NB];
-  schur_w<K>(R, b1, mu1, M, w1);
-  if (NRHS > 1) schur_w<K>(R, b2, mu2, M, w2);
+  schur_w<K>(R, b1, mu1, M, ty, w1);
+  if (NRHS > 1) schur_w<K>(R, b2, mu2, M, ty, w2);
   const double* Minv = tb;
 #pragma unroll
   for (int h = 0; h < 2; ++h)
@@ -640,9 +659,13 @@ __global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restri
 // ks > 0: the tile is column segment tl.w (columns [tl.w ks, (tl.w + 1) ks))
 // of its row tile and writes its partial sums to Part (slot blockIdx.x, rows
 // relative to row0, 64 MB rows per slot); k_sweep_reduce adds the segments.
+// Single-block tiles (MB = 1) ask for 4 waves per SIMD: the forward sweep
+// then fits its accumulators in the VGPRs (111, no spills) instead of taking
+// 130 registers with AGPRs, which allowed only 3 waves; the backward sweep
+// goes from 4 to 5 waves the same way.
 template <int NRP, int MB, bool FWD>
-__global__ __launch_bounds__(256) void k_sweep_mfma(DevFactor F, const int4* __restrict__ tiles,
-                                                    double* __restrict__ Y, int ks, double* __restrict__ Part) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MB == 1 ? 4 : 1))) void k_sweep_mfma(
+    DevFactor F, const int4* __restrict__ tiles, double* __restrict__ Y, int ks, double* __restrict__ Part) {
   constexpr int NH = NRP / 16;
   const int4 tl = tiles[blockIdx.x];
   const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
@@ -915,34 +938,38 @@ __global__ __launch_bounds__(256) void k_poisson_face(DevPoisson P, const double
   r[P.mult_perm[m]] = s;
 }
 
-template <int K>
+// UNI: cells from P.ucell, whose waves share the cell type, so the table
+// offsets are wave-uniform (scalar-cache reads).  Faces without multipliers
+// contribute exact zeros (s - 0 = s), so the sum equals the one over the
+// faces that have them.
+template <int K, bool UNI>
 __global__ __launch_bounds__(64) void k_poisson_recover(DevPoisson P, const double* __restrict__ lam,
                                                         const double* __restrict__ y, double* __restrict__ out,
                                                         int64_t sl, int64_t sy, int64_t so) {
   constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
-  const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (Kc >= P.ncell) return;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Kc = UNI ? (t0 < P.nucell ? P.ucell[t0] : -1) : (t0 < P.ncell ? t0 : -1);
+  if (Kc < 0) return;
   lam += blockIdx.y * sl;
   y += blockIdx.y * sy;
   out += blockIdx.y * so;
   double e[4 * K];
-  int ej[4 * K];
-  int ne = 0;
+#pragma unroll
   for (int f = 0; f < 4; ++f) {
     const int base = P.mult_face[4 * (size_t)Kc + f];
-    if (base < 0) continue;
     const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
-    for (int m = 0; m < K; ++m) {
-      e[ne] = sf * P.gw[m] * lam[P.mult_perm[base + m]];
-      ej[ne] = rt_face<K>(f, m);
-      ++ne;
-    }
+#pragma unroll
+    for (int m = 0; m < K; ++m) e[f * K + m] = base < 0 ? 0.0 : sf * P.gw[m] * lam[P.mult_perm[base + m]];
   }
-  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  const int ty = UNI ? __builtin_amdgcn_readfirstlane(P.type[Kc]) : P.type[Kc];
+  const double* tb = P.table + (size_t)ty * P.stride;
   const int* id = P.idx + (size_t)Kc * NL;
   for (int i = 0; i < NL; ++i) {
     double s = y[(size_t)Kc * NL + i];
-    for (int t = 0; t < ne; ++t) s -= tb[i * NL + ej[t]] * e[t];
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int m = 0; m < K; ++m) s -= tb[i * NL + rt_face<K>(f, m)] * e[f * K + m];
     if (i < NRT) {
       int m;
       const int fc = face_of_local<K>(i, m);
@@ -1072,11 +1099,20 @@ int launch_rhs_carrier(int k, bool semi, const DevRegion& R, const DevRegion& O,
 
 int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, double mu1, double mu2,
                      double* g1, double* g2, int nrhs, hipStream_t s, int nb, int64_t sb, int64_t sg) {
+  if (R.ucell_nbh) {
+    const dim3 g(nblk(R.nu_nbh, 64), nb);
+    if (nrhs == 2) {
+      PECS_DISPATCH_K(k, (k_schur_rhs<KK, 2, true><<<g, 64, 0, s>>>(R, b1, b2, mu1, mu2, g1, g2, sb, sg)));
+    } else {
+      PECS_DISPATCH_K(k, (k_schur_rhs<KK, 1, true><<<g, 64, 0, s>>>(R, b1, b1, mu1, mu1, g1, g1, sb, sg)));
+    }
+    return (int)hipGetLastError();
+  }
   const dim3 g(nblk(R.ncell, 64), nb);
   if (nrhs == 2) {
-    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, mu1, mu2, g1, g2, sb, sg)));
+    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 2, false><<<g, 64, 0, s>>>(R, b1, b2, mu1, mu2, g1, g2, sb, sg)));
   } else {
-    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, mu1, mu1, g1, g1, sb, sg)));
+    PECS_DISPATCH_K(k, (k_schur_rhs<KK, 1, false><<<g, 64, 0, s>>>(R, b1, b1, mu1, mu1, g1, g1, sb, sg)));
   }
   return (int)hipGetLastError();
 }
@@ -1084,15 +1120,21 @@ int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* 
 int launch_schur_recover(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
                          const double* xs2, double mu1, double mu2, double* out1, double* out2, int nrhs,
                          hipStream_t s, int nb, int64_t sb, int64_t sx, int64_t so) {
-  const dim3 g(nblk(R.ncell, 64), nb);
-  if (nrhs == 2) {
-    PECS_DISPATCH_K(k, (k_schur_recover<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, xs1, xs2, mu1, mu2, out1, out2, sb, sx,
-                                                                so)));
-  } else {
-    PECS_DISPATCH_K(k, (k_schur_recover<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, xs1, xs1, mu1, mu1, out1, out1, sb, sx,
-                                                                so)));
+#define PECS_RECOVER(UU, NCELL)                                                                                 \
+  {                                                                                                             \
+    const dim3 g(nblk(NCELL, 64), nb);                                                                          \
+    if (nrhs == 2) {                                                                                            \
+      PECS_DISPATCH_K(k, (k_schur_recover<KK, 2, UU><<<g, 64, 0, s>>>(R, b1, b2, xs1, xs2, mu1, mu2, out1, out2, \
+                                                                      sb, sx, so)));                            \
+    } else {                                                                                                    \
+      PECS_DISPATCH_K(k, (k_schur_recover<KK, 1, UU><<<g, 64, 0, s>>>(R, b1, b1, xs1, xs1, mu1, mu1, out1, out1, \
+                                                                      sb, sx, so)));                            \
+    }                                                                                                           \
+    return (int)hipGetLastError();                                                                              \
   }
-  return (int)hipGetLastError();
+  if (R.ucell_own) PECS_RECOVER(true, R.nu_own)
+  PECS_RECOVER(false, R.ncell)
+#undef PECS_RECOVER
 }
 
 template <int NRHS, bool FWD>
@@ -1225,7 +1267,13 @@ int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStre
 
 int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
                            hipStream_t s, int nb, int64_t sl, int64_t sy, int64_t so) {
-  PECS_DISPATCH_K(k, (k_poisson_recover<KK><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl, sy, so)));
+  if (P.ucell) {
+    PECS_DISPATCH_K(k, (k_poisson_recover<KK, true><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl,
+                                                                                                 sy, so)));
+  } else {
+    PECS_DISPATCH_K(k, (k_poisson_recover<KK, false><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl,
+                                                                                                  sy, so)));
+  }
   return (int)hipGetLastError();
 }
 

@@ -22,6 +22,13 @@ struct DevRegion {
   int stride;
   const int* sigma_nb;   // interface partner cell in the other region, or -1
   const int* perm_base;  // permuted index of the cell's first density DoF in the Schur system
+  // Cells grouped so that each group of 64 (one wave) shares the operator
+  // types the Schur kernels read, groups padded with -1 (null: cells in
+  // order, per-lane types): by the cell's own type (recovery), and by the
+  // types of the cell and its 4 face neighbours (right-hand side).
+  const int* ucell_own;
+  const int* ucell_nbh;
+  int nu_own, nu_nbh;
 };
 
 struct DevPoisson {
@@ -41,6 +48,8 @@ struct DevPoisson {
   const int* mloc;       // 2 per multiplier: local RT DoF in that cell
   const double* mcoef;   // 2 per multiplier: sign * Gauss weight
   const double* gw;      // face Gauss weights (k)
+  const int* ucell;      // cells grouped by type in whole waves, padded with -1 (null: cells in order)
+  int nucell;
 };
 
 struct DevPhys {
