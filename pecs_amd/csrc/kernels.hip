@@ -503,25 +503,38 @@ struct SweepShape {
 };
 
 // forward, part 1: t_p = b_p - sum of the updates z owed to row p, into X.
-// Right-hand side h = blockIdx.y; vectors are h-major (b[h n + p], Z[h zlen + z]).
-// L lanes per row (L = 1, or 8 on levels with few rows: their rows collect
-// many descendants' contributions); the lanes' partial sums are combined by
+// Vectors are h-major (b[h n + p]); with two right-hand sides the updates are
+// pair-interleaved (Z[2 z + h]), so one 16-byte load fetches both.  L lanes
+// per row (L = 1, or 8 on levels with few rows: their rows collect many
+// descendants' contributions); the lanes' partial sums are combined by
 // shuffles in a fixed order.
-template <int L>
+template <int L, int NRHS>
 __global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int* __restrict__ lrows, int nrow,
                                                     const double* __restrict__ b) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = t / L, lane = t % L;
   if (i >= nrow) return;  // nrow * L threads: whole L-groups exit together
-  const int h = blockIdx.y;
   const int p = lrows[i];
-  const double* __restrict__ Zh = F.Z + (int64_t)h * F.zlen;
   const int64_t c1 = F.cptr[p + 1];
-  double z = 0.0;
-  for (int64_t c = F.cptr[p] + lane; c < c1; c += L) z += Zh[F.czidx[c]];
+  double z0 = 0.0, z1 = 0.0;
+  for (int64_t c = F.cptr[p] + lane; c < c1; c += L) {
+    if (NRHS == 2) {
+      const double2 z = *reinterpret_cast<const double2*>(F.Z + 2 * F.czidx[c]);
+      z0 += z.x;
+      z1 += z.y;
+    } else {
+      z0 += F.Z[F.czidx[c]];
+    }
+  }
 #pragma unroll
-  for (int o = L / 2; o > 0; o >>= 1) z += __shfl_xor(z, o, L);
-  if (lane == 0) F.X[(int64_t)h * F.n + p] = b[(int64_t)h * F.n + p] - z;
+  for (int o = L / 2; o > 0; o >>= 1) {
+    z0 += __shfl_xor(z0, o, L);
+    if (NRHS == 2) z1 += __shfl_xor(z1, o, L);
+  }
+  if (lane == 0) {
+    F.X[p] = b[p] - z0;
+    if (NRHS == 2) F.X[(int64_t)F.n + p] = b[(int64_t)F.n + p] - z1;
+  }
 }
 
 template <int NRHS, int W, int MAXR_, bool FWD>
@@ -632,8 +645,11 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
         if (NRHS > 1) Y1[p0 + lr] = r1;
       } else {
         const int64_t z = F.z_off[s] + (lr - ns);
-        F.Z[z] = r0;
-        if (NRHS > 1) F.Z[F.zlen + z] = r1;
+        if (NRHS > 1) {
+          *reinterpret_cast<double2*>(F.Z + 2 * z) = double2{r0, r1};
+        } else {
+          F.Z[z] = r0;
+        }
       }
     } else {
       F.X[p0 + lr] = r0;
@@ -655,30 +671,40 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
 // Accumulator lane l, register r holds row l/16 + 4r, right-hand side l%16.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// t_p = b_p - sum of the updates z owed to row p, interleaved (h fastest)
+// t_p = b_p - sum of the updates z owed to row p, interleaved (h fastest).
+// A thread takes two adjacent right-hand sides (16-byte loads and stores).
 template <int NRP>
 __global__ __launch_bounds__(256) void k_fwd_gather_il(DevFactor F, const int* __restrict__ lrows, int nrow,
                                                        const double* __restrict__ b, int nrhs) {
+  constexpr int NP = NRP / 2;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nrow * NRP) return;
-  const int i = (int)(e / NRP), h = (int)(e % NRP);
+  if (e >= (int64_t)nrow * NP) return;
+  const int i = (int)(e / NP), h = 2 * (int)(e % NP);
   const int p = lrows[i];
-  double a = 0.0;
+  double a0 = 0.0, a1 = 0.0;
   if (h < nrhs) {
-    a = b[(int64_t)h * F.n + p];
+    a0 = b[(int64_t)h * F.n + p];
+    if (h + 1 < nrhs) a1 = b[(int64_t)(h + 1) * F.n + p];
     const int64_t c1 = F.cptr[p + 1];
-    for (int64_t c = F.cptr[p]; c < c1; ++c) a -= F.Z[F.czidx[c] * NRP + h];
+    for (int64_t c = F.cptr[p]; c < c1; ++c) {
+      const double2 z = *reinterpret_cast<const double2*>(F.Z + F.czidx[c] * NRP + h);
+      a0 -= z.x;
+      a1 -= z.y;
+    }
+    if (h + 1 >= nrhs) a1 = 0.0;
   }
-  F.X[(int64_t)p * NRP + h] = a;
+  *reinterpret_cast<double2*>(F.X + (int64_t)p * NRP + h) = double2{a0, a1};
 }
 
 template <int NRP>
 __global__ __launch_bounds__(256) void k_bwd_gather_il(DevFactor F, const int64_t* __restrict__ idx, int nidx) {
+  constexpr int NP = NRP / 2;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nidx * NRP) return;
-  const int i = (int)(e / NRP), h = (int)(e % NRP);
+  if (e >= (int64_t)nidx * NP) return;
+  const int i = (int)(e / NP), h = 2 * (int)(e % NP);
   const int64_t q = idx[i];
-  F.Z[q * NRP + h] = F.X[(int64_t)F.rows[q] * NRP + h];
+  *reinterpret_cast<double2*>(F.Z + q * NRP + h) =
+      *reinterpret_cast<const double2*>(F.X + (int64_t)F.rows[q] * NRP + h);
 }
 
 // x (h-major, nrhs right-hand sides) from the interleaved result
@@ -1230,7 +1256,8 @@ template <int NRP>
 static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s) {
   for (int l = 0; l < L.nlevels; ++l) {
     const int ng = L.goff[l + 1] - L.goff[l];
-    if (ng > 0) k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v, nrhs);
+    if (ng > 0)
+      k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP / 2, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v, nrhs);
     const int nt = L.foff[l + 1] - L.foff[l];
     if (L.fw[l] != 0) return (int)hipErrorInvalidValue;
     const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, 0, L.fm[l], true, nrhs, t, s, L.fks[l],
@@ -1239,7 +1266,7 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
     const int nx = L.xoff[l + 1] - L.xoff[l];
-    if (nx > 0) k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
+    if (nx > 0) k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP / 2, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     const int nt = L.boff[l + 1] - L.boff[l];
     if (L.bw[l] != 0) return (int)hipErrorInvalidValue;
     const int e = launch_sweep(F, F.btiles + L.boff[l], nt, 0, L.bm[l], false, nrhs, t, s, L.bks[l],
@@ -1259,9 +1286,16 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
   for (int l = 0; l < L.nlevels; ++l) {
     // level 0 (leaves) owes nothing to its rows: its sweep reads the right-hand side itself
     const int ng = L.goff[l + 1] - L.goff[l];
-    if (l > 0 && ng >= 16384) k_fwd_gather<1><<<dim3(nblk(ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
-    else if (l > 0 && ng > 0)
-      k_fwd_gather<8><<<dim3(nblk(8 * (int64_t)ng, 256), nrhs), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v);
+    if (l > 0 && ng > 0) {
+      const int* lr = F.lrows + L.goff[l];
+      if (ng >= 16384) {
+        if (nrhs == 2) k_fwd_gather<1, 2><<<nblk(ng, 256), 256, 0, s>>>(F, lr, ng, v);
+        else k_fwd_gather<1, 1><<<nblk(ng, 256), 256, 0, s>>>(F, lr, ng, v);
+      } else {
+        if (nrhs == 2) k_fwd_gather<8, 2><<<nblk(8 * (int64_t)ng, 256), 256, 0, s>>>(F, lr, ng, v);
+        else k_fwd_gather<8, 1><<<nblk(8 * (int64_t)ng, 256), 256, 0, s>>>(F, lr, ng, v);
+      }
+    }
     const int nt = L.foff[l + 1] - L.foff[l];
     const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s, 0, nullptr, 0,
                                l == 0 ? v : nullptr);

@@ -97,7 +97,8 @@ struct DevFactor {
   const int4* frtiles;      // split row tiles of the MFMA sweeps, grouped by level
   const int4* brtiles;
   double* Part;             // partial sums of split tiles (MFMA sweeps)
-  double* Z;                // 2 * zlen scratch: forward updates, then backward gathered x
+  double* Z;                // nr * zlen scratch: forward updates (two right-hand sides pair-interleaved,
+                            // Z[2 z + h]; MFMA sweeps NRP-interleaved), then backward gathered x
   double* X;                // 2 * n scratch: forward t, then backward output
 };
 
