@@ -904,7 +904,8 @@ __device__ __forceinline__ double poisson_bias(const double* tb, const DevPhys& 
   return ph.phi_bi * U[i] - ph.phi_app * U[NRT + i] - ph.phi_sch * U[2 * NRT + i];
 }
 
-template <int K>
+// UNI: cells from P.ucell (waves share the cell type: scalar table reads)
+template <int K, bool UNI>
 __global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, const DevPhys* __restrict__ phs,
                                                      const double* __restrict__ un, const double* __restrict__ up,
                                                      const double* __restrict__ ur, const double* __restrict__ uo,
@@ -921,10 +922,12 @@ __global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, co
   }
   using S = Sz<K>;
   constexpr int NRT = S::NRT, NL = S::NL, NPHI = S::NPHI;
-  const int Kc = blockIdx.x * blockDim.x + threadIdx.x;
-  if (Kc >= P.ncell) return;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Kc = UNI ? (t0 < P.nucell ? P.ucell[t0] : -1) : (t0 < P.ncell ? t0 : -1);
+  if (Kc < 0) return;
   double f[NL];
-  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  const int ty = UNI ? __builtin_amdgcn_readfirstlane(P.type[Kc]) : P.type[Kc];
+  const double* tb = P.table + (size_t)ty * P.stride;
 #pragma unroll
   for (int i = 0; i < NRT; ++i) f[i] = poisson_bias<K>(tb, ph, i);
   poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, f + NRT);
@@ -1316,8 +1319,13 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
 int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                         const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
                         hipStream_t s, int nb, int64_t ss, int64_t se, int64_t sy) {
-  PECS_DISPATCH_K(k, (k_poisson_cell<KK><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(P, T, phs, un, up, ur, uo, ncs,
-                                                                                    nce, y, ss, se, sy)));
+  if (P.ucell) {
+    PECS_DISPATCH_K(k, (k_poisson_cell<KK, true><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(
+                           P, T, phs, un, up, ur, uo, ncs, nce, y, ss, se, sy)));
+  } else {
+    PECS_DISPATCH_K(k, (k_poisson_cell<KK, false><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(
+                           P, T, phs, un, up, ur, uo, ncs, nce, y, ss, se, sy)));
+  }
   return (int)hipGetLastError();
 }
 
