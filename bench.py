@@ -39,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
 def cpu_baseline(seconds_budget=20.0):
@@ -116,6 +117,23 @@ def gpu_time_to_steady(g, tol, check_every, max_steps):
     return {"s": time.perf_counter() - t0, "steps": n, "residual": res, "reached": res <= tol}
 
 
+def assembly_rate(args, g, phase):
+    """Carrier right-hand-side assembly of one step (k_rhs_carrier, both
+    regions): algorithmic bytes = old densities of both carriers, the cell's
+    RT field DoFs and 4 vertices read, both carriers' full right-hand sides
+    written, per cell; over the sum of the two regions' phase times.  The
+    kernel evaluates the geometry, the field, exp (generation) and the drift
+    terms in FP64 at every quadrature point, so it sits below the HBM bound."""
+    nb = (args.degree + 1) ** 2
+    nrt = 2 * args.degree * (args.degree + 1)
+    ncell = (g.n_semi + g.n_elec) // (3 * nb)
+    nbytes = 8.0 * ncell * (2 * nb + nrt + 8 + 2 * 3 * nb)
+    ms = phase[0] + phase[1]
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"kernel": "k_rhs_carrier (both regions)", "algorithmic_bytes": nbytes, "avg_ms": ms,
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
 def batched_leg(args, workload):
     """I-V sweep on one mesh: B bias points (applied bias evenly over
     [-0.5, 0.5] scaled volts) stepped together, every factor read once per
@@ -156,9 +174,22 @@ def batched_leg(args, workload):
     phase = {n: round(ph[i] / max(nst.value, 1), 4) for i, n in enumerate(names)}
     U = 2 * g.n_semi + 2 * g.n_elec + g.n_poisson
     ms_step = el / args.batch_steps * 1e3
+    # the batched carrier solves on FP64 MFMA: 2 flops per stored factor entry
+    # (both copies) per right-hand side, 2B right-hand sides per factor
+    fb = ctypes.c_double()
+    L.pecs_factor_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    flops = 0.0
+    for which in (0, 1):
+        L.pecs_factor_bytes(g.ctx, which, ctypes.byref(fb))
+        flops += 2.0 * (fb.value / 8.0) * 2 * B
+    cms = phase["carrier_cholesky"]
+    tf = flops / (cms * 1e-3) / 1e12 if cms > 0 else 0.0
     out = {"batch": B, "workload": workload + f", {B} applied biases in [-0.5, 0.5]", "setup_s": round(setup_s, 2),
            "ms_per_step_all_instances": ms_step, "ms_per_step_per_instance": ms_step / B,
            "dof_updates_per_s": U * B * args.batch_steps / el, "phase_ms_per_step": phase,
+           "carrier_solves_mfma": {"kernel": "k_sweep_mfma (FP64 MFMA 16x16x4) + gathers, both regions, one step",
+                                   "flops": flops, "avg_ms": cms, "achieved": tf, "peak": FP64_PEAK_TFS,
+                                   "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS},
            "sweep_tiles_semi": [list(t) for t in g.sweep_config(0)]}
     if not args.no_steady:
         g.initial_state()
@@ -506,6 +537,7 @@ def main():
                      "traffic_source": "profiles/carrier_solves_r8k2_hbm_traffic.md (rocprofv3 --pmc)",
                      "algorithmic_bytes_per_launch": chol_bytes, "avg_ms": chol_ms},
     }
+    out["assembly"] = assembly_rate(args, g, phase)
     out["config"]["sweep_tiles_semi"] = [list(t) for t in g.sweep_config(0)]
     if not args.no_steady:
         tts = gpu_time_to_steady(g, args.steady_tol, args.steady_check_every, args.steady_max_steps)
