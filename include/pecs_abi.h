@@ -223,6 +223,13 @@ int pecs_factor_bytes(pecs_ctx* ctx, int which, double* bytes);
  * solve has no tunable shape). */
 int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels);
 
+/* Which cell-local kernels take wave-uniform cell lists (cells grouped so
+ * that every 64-cell wave shares the operator types it reads): bit 0/1
+ * semiconductor Schur recovery / right-hand side, bit 2/3 the same for the
+ * electrolyte, bit 4 the Poisson condensation and recovery.  No counterpart
+ * in the reference (a kernel launch shape). */
+int pecs_uniform_lists(pecs_ctx* ctx, int* flags);
+
 /* Device-pointer variants (no copies, no synchronisation). */
 int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,
                                const double* u_r, const double* u_o, double* rhs_n, double* rhs_p);

@@ -1489,6 +1489,13 @@ int pecs_sweep_destroy(pecs_sweep* w) {
   return 0;
 }
 
+int pecs_uniform_lists(pecs_ctx* ctx, int* flags) {
+  if (!ctx || !flags) return -1;
+  *flags = (ctx->Rs.ucell_own ? 1 : 0) | (ctx->Rs.ucell_nbh ? 2 : 0) | (ctx->Re.ucell_own ? 4 : 0) |
+           (ctx->Re.ucell_nbh ? 8 : 0) | (ctx->Pd.ucell ? 16 : 0);
+  return 0;
+}
+
 int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels) {
   PECS_TRY(ctx, {
     const FactorDev* F = which == 0 ? &ctx->Fs[0] : which == 1 ? &ctx->Fe[0] : which == 2 ? &ctx->FH : nullptr;
