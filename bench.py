@@ -134,6 +134,55 @@ def assembly_rate(args, g, phase):
             "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
 
 
+def mixed_leg(args, workload):
+    """Mixed precision on a batch of bias points (pecs_set_mixed_precision:
+    FP32 contractions with the cell-type tables in the Schur and Poisson cell
+    kernels; FP64 state, assembly and triangular solves): the same steps from
+    the same initial state with the FP64 path and the mixed path, timed as
+    replayed graphs; per instance the relative L2 difference of each state
+    vector against the FP64 path (which follows the oracle to 1e-10)."""
+    import numpy as np
+    import torch
+    import pecs_amd
+    B = max(args.batch, 1)
+    prms = []
+    for v in np.linspace(-0.5, 0.5, B):
+        p = pecs_amd.workload_params(args.refine, args.degree, args.dt)
+        p.update(phi_app=float(v))
+        prms.append(p)
+    g = pecs_amd.SolarCellGPU(prms if B > 1 else prms[0])
+    n = args.mixed_steps
+
+    def run():
+        g.initial_state()
+        g.step(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.step(n)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / n * 1e3
+        return ms, [g.get_state(i) for i in range(B)]
+
+    ms64, s64 = run()
+    g.set_mixed_precision(True)
+    ms32, s32 = run()
+    g.close()
+    names = {"n": "electrons", "p": "holes", "r": "reductants", "o": "oxidants", "P": "poisson"}
+    per_inst = []
+    for a, b in zip(s32, s64):
+        per_inst.append({names[c]: float(np.linalg.norm(a[c] - b[c]) / max(np.linalg.norm(b[c]), 1e-300))
+                         for c in "nproP"})
+    worst = {names[c]: max(e[names[c]] for e in per_inst) for c in "nproP"}
+    print(f"[bench] mixed precision: {ms32:.3f} ms/step against {ms64:.3f} (FP64); worst errors {worst}",
+          file=sys.stderr, flush=True)
+    return {"batch": B, "workload": workload + f", {B} applied biases in [-0.5, 0.5]", "steps": n + 3,
+            "contractions": "FP32 with FP32 accumulation (Schur transform and recovery, Poisson condensation and "
+                            "recovery); state, assembly and triangular solves FP64",
+            "ms_per_step_fp64": ms64, "ms_per_step_mixed": ms32, "speedup": ms64 / ms32 if ms32 > 0 else None,
+            "rel_l2_error_vs_fp64_max_over_instances": worst,
+            "rel_l2_error_vs_fp64_per_instance_max_over_fields": [max(e.values()) for e in per_inst]}
+
+
 def batched_leg(args, workload):
     """I-V sweep on one mesh: B bias points (applied bias evenly over
     [-0.5, 0.5] scaled volts) stepped together, every factor read once per
@@ -408,6 +457,9 @@ def main():
     ap.add_argument("--sweep-plain-steps", type=int, default=20)
     ap.add_argument("--sweep-reserve-gib", type=float, default=64.0,
                     help="stop adding instances to a wave when less device memory than this is free")
+    ap.add_argument("--mixed", action="store_true",
+                    help="run only the mixed-precision leg (FP32 cell-local contractions) on a batch of --batch")
+    ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--warm-start", action="store_true", help="run only the learned warm-start leg")
     ap.add_argument("--ws-refine", type=int, default=5)
     ap.add_argument("--ws-train-groups", type=int, default=64, help="16 draws per group (one cone angle)")
@@ -443,6 +495,11 @@ def main():
         return
     if args.warm_start:
         print(json.dumps({"warm_start": warm_start_leg(args)}), flush=True)
+        return
+    if args.mixed:
+        wl = (f"PECS half cell, refinement {args.refine} ({1 << args.refine}x{1 << args.refine} quads/region), "
+              f"LDG k={args.degree}, RT{args.degree - 1}xDGQ{args.degree - 1} Poisson, dt={args.dt}")
+        print(json.dumps({"mixed_precision": mixed_leg(args, wl)}), flush=True)
         return
     if args.sweep_only:
         print(json.dumps({"sweep": sweep_leg(args)}), flush=True)

@@ -230,6 +230,15 @@ int pecs_sweep_config(pecs_ctx* ctx, int which, int* out, int n, int* nlevels);
  * in the reference (a kernel launch shape). */
 int pecs_uniform_lists(pecs_ctx* ctx, int* flags);
 
+/* Mixed precision (enable != 0): the cell-local contractions with the
+ * cell-type tables (Schur transform and recovery, Poisson condensation and
+ * recovery) run in FP32 with FP32 accumulation on FP32 copies of the tables;
+ * the state, the right-hand-side assembly and the triangular solves stay
+ * FP64.  Needs the wave-uniform cell lists (all five flags of
+ * pecs_uniform_lists; else an error).  Takes effect from the next step.  No
+ * counterpart in the reference (FP64 throughout); it replaces nothing. */
+int pecs_set_mixed_precision(pecs_ctx* ctx, int enable);
+
 /* Device-pointer variants (no copies, no synchronisation). */
 int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,
                                const double* u_r, const double* u_o, double* rhs_n, double* rhs_p);

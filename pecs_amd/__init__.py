@@ -24,6 +24,7 @@ ABI_FUNCTIONS = [
     "pecs_initial_state", "pecs_step", "pecs_last_residual", "pecs_rhs_semiconductor_dev",
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config", "pecs_uniform_lists",
+    "pecs_set_mixed_precision",
     "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
     "pecs_last_residual_batch", "pecs_write_vtu", "pecs_use_graph", "pecs_sweep_create", "pecs_sweep_set_active",
     "pecs_sweep_step", "pecs_sweep_destroy", "pecs_mlp_create", "pecs_mlp_destroy", "pecs_mlp_num_params",
@@ -275,6 +276,13 @@ class SolarCellGPU:
                                         ctypes.POINTER(ctypes.c_int)]
         self._c(L.pecs_sweep_config(self.ctx, which, out, 6 * 64, ctypes.byref(nl)))
         return [tuple(out[6 * l:6 * l + 6]) for l in range(nl.value)]
+
+    def set_mixed_precision(self, enable=True):
+        """FP32 cell-local contractions (FP64 state, assembly and solves);
+        needs the wave-uniform cell lists (planar meshes)."""
+        L = lib()
+        L.pecs_set_mixed_precision.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self._c(L.pecs_set_mixed_precision(self.ctx, 1 if enable else 0))
 
     def uniform_lists(self):
         """Bit flags of the cell-local kernels that take wave-uniform cell

@@ -69,6 +69,11 @@ struct pecs_ctx {
   DevTabs tabs{};
   DevRegion Rs{}, Re{};
   DevPoisson Pd{};
+  // mixed precision: FP32 copies of the semiconductor, electrolyte and
+  // Poisson cell-type tables (kept on the host when the wave-uniform lists
+  // are in use; uploaded by pecs_set_mixed_precision)
+  std::vector<float> tabf_host[3];
+  float* tabf_dev[3] = {};
   DevPhys ph{};  // instance 0
   FactorDev Fs[2], Fe[2], FH;
   bool share_s = false, share_e = false;
@@ -581,6 +586,8 @@ DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, c
       d.ucell_nbh = upload(c, nbh);
       d.nu_nbh = (int)nbh.size();
     }
+    if (d.ucell_own && d.ucell_nbh)
+      c->tabf_host[&R == &c->mesh.semi ? 0 : 1].assign(O.table.begin(), O.table.end());
   }
   return d;
 }
@@ -798,6 +805,17 @@ void capture(pecs_ctx* c, bool last, hipGraph_t& graph, hipGraphExec_t& exec) {
   c->timing = timing;
 }
 
+// Forget the captured step graphs (they are captured again at the next step),
+// after a change of the kernels' arguments.  The stream must be idle.
+void drop_graphs(pecs_ctx* c) {
+  if (c->step_exec) HIPCHECK(hipGraphExecDestroy(c->step_exec));
+  if (c->step_graph) HIPCHECK(hipGraphDestroy(c->step_graph));
+  if (c->last_exec) HIPCHECK(hipGraphExecDestroy(c->last_exec));
+  if (c->last_graph) HIPCHECK(hipGraphDestroy(c->last_graph));
+  c->step_exec = c->last_exec = nullptr;
+  c->step_graph = c->last_graph = nullptr;
+}
+
 void ensure_step_graph(pecs_ctx* c) {
   if (!c->step_exec) capture(c, false, c->step_graph, c->step_exec);
   if (!c->last_exec) capture(c, true, c->last_graph, c->last_exec);
@@ -979,6 +997,7 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
       if (!(su && su[0] == '0') && own.size() <= std::max<size_t>(2 * (size_t)m.ncellP, (size_t)m.ncellP + 2048)) {
         P.ucell = upload(c.get(), own);
         P.nucell = (int)own.size();
+        c->tabf_host[2].assign(op.table.begin(), op.table.end());
       }
     }
     P.pbid = upload(c.get(), m.pbid);
@@ -1487,6 +1506,25 @@ int pecs_sweep_destroy(pecs_sweep* w) {
   if (!w) return -1;
   delete w;
   return 0;
+}
+
+int pecs_set_mixed_precision(pecs_ctx* ctx, int enable) {
+  PECS_TRY(ctx, {
+    if (enable) {
+      if (!(ctx->Rs.ucell_own && ctx->Rs.ucell_nbh && ctx->Re.ucell_own && ctx->Re.ucell_nbh && ctx->Pd.ucell) ||
+          ctx->tabf_host[0].empty() || ctx->tabf_host[1].empty() || ctx->tabf_host[2].empty())
+        throw std::runtime_error(
+            "mixed precision needs the wave-uniform cell lists (a planar mesh with few cell types)");
+      for (int i = 0; i < 3; ++i)
+        if (!ctx->tabf_dev[i]) ctx->tabf_dev[i] = upload(ctx, ctx->tabf_host[i]);
+    }
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->Rs.table_f = enable ? ctx->tabf_dev[0] : nullptr;
+    ctx->Re.table_f = enable ? ctx->tabf_dev[1] : nullptr;
+    ctx->Pd.table_f = enable ? ctx->tabf_dev[2] : nullptr;
+    // the captured step graphs hold the old kernel arguments: capture again
+    drop_graphs(ctx);
+  })
 }
 
 int pecs_uniform_lists(pecs_ctx* ctx, int* flags) {

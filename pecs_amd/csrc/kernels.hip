@@ -473,6 +473,138 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
 }
 
 // ------------------------------------------------------------------------
+// Mixed precision (FP32 element contractions, FP64 vectors and solves)
+// ------------------------------------------------------------------------
+// The cell-local contractions with the cell-type tables run in FP32 on an
+// FP32 copy of the table, with FP32 accumulation; inputs are read and
+// results written in FP64, and the terms that are not contractions (the
+// density part f_v of the Schur right-hand side, the difference f - B u
+// before the recovery's inverse mass) stay FP64.  Wave-uniform cell lists
+// only (planar meshes): the table entries are scalar operands.
+template <int K>
+__device__ __forceinline__ void schur_rhs_one_f32(const DevRegion& R, const double* __restrict__ b, double mu,
+                                                  double* __restrict__ g, int X) {
+  constexpr int NB = Sz<K>::NB;
+  const size_t nc = R.ncell;
+  const float muf = (float)mu;
+  float acc[NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) acc[a] = 0.0f;
+  for (int f = -1; f < 4; ++f) {
+    const int M = f < 0 ? X : R.nbr[4 * (size_t)X + f];
+    if (M < 0) continue;
+    const int ty = wave_type<true>(R, M);
+    const float* Minv = R.table_f + (size_t)ty * R.stride;
+    const float* B = Minv + NB * NB + (f < 0 ? 0 : 2 * NB * NB + (f ^ 1) * 2 * NB * NB);
+    float fx[NB], fy[NB], w[2 * NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      fx[a] = (float)b[(size_t)M * NB + a];
+      fy[a] = (float)b[nc * NB + (size_t)M * NB + a];
+    }
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        sx += Minv[a * NB + t] * fx[t];
+        sy += Minv[a * NB + t] * fy[t];
+      }
+      w[a] = muf * sx;
+      w[NB + a] = muf * sy;
+    }
+    for (int i = 0; i < 2 * NB; ++i) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a) acc[a] += B[i * NB + a] * w[i];
+    }
+  }
+  const int pb = R.perm_base[X];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) g[pb + a] = b[2 * nc * NB + (size_t)X * NB + a] + (double)acc[a];
+}
+
+template <int K, int NRHS>
+__global__ __launch_bounds__(64) void k_schur_rhs_f32(DevRegion R, const double* __restrict__ b1,
+                                                      const double* __restrict__ b2, double mu1, double mu2,
+                                                      double* __restrict__ g1, double* __restrict__ g2, int64_t sb,
+                                                      int64_t sg) {
+  const int X = schur_cell<true>(R.ucell_nbh, R.nu_nbh, blockIdx.x * blockDim.x + threadIdx.x);
+  if (X < 0) return;
+  const int64_t i = blockIdx.y;
+  schur_rhs_one_f32<K>(R, b1 + i * sb, mu1, g1 + i * sg, X);
+  if (NRHS > 1) schur_rhs_one_f32<K>(R, b2 + i * sb, mu2, g2 + i * sg, X);
+}
+
+template <int K>
+__device__ __forceinline__ void schur_recover_one_f32(const DevRegion& R, const float* tb,
+                                                      const double* __restrict__ b, const double* __restrict__ x,
+                                                      double mu, double* __restrict__ o, int M) {
+  constexpr int NB = Sz<K>::NB;
+  const size_t nc = R.ncell;
+  float s[2 * NB], u[NB];
+  double ud[NB];
+  const int pb = R.perm_base[M];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    ud[a] = x[pb + a];
+    u[a] = (float)ud[a];
+  }
+  for (int i = 0; i < 2 * NB; ++i) {
+    float t = 0.0f;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) t += tb[NB * NB + i * NB + a] * u[a];
+    s[i] = t;
+  }
+  for (int f = 0; f < 4; ++f) {
+    const int X = R.nbr[4 * (size_t)M + f];
+    if (X < 0) continue;
+    const int px = R.perm_base[X];
+    float v[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) v[a] = (float)x[px + a];
+    const float* B = tb + NB * NB + 2 * NB * NB + f * 2 * NB * NB;
+    for (int i = 0; i < 2 * NB; ++i) {
+      float t = 0.0f;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) t += B[i * NB + a] * v[a];
+      s[i] += t;
+    }
+  }
+  // q = mu A^{-1} (f_q - B u): the difference in FP64, the inverse mass in FP32
+  const float muf = (float)mu;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float d[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) d[t] = (float)(b[(size_t)h * nc * NB + (size_t)M * NB + t] - (double)s[h * NB + t]);
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      float t1 = 0.0f;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) t1 += tb[a * NB + t] * d[t];
+      o[(size_t)h * nc * NB + (size_t)M * NB + a] = (double)(muf * t1);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NB; ++a) o[2 * nc * NB + (size_t)M * NB + a] = ud[a];
+}
+
+template <int K, int NRHS>
+__global__ __launch_bounds__(64) void k_schur_recover_f32(DevRegion R, const double* __restrict__ b1,
+                                                          const double* __restrict__ b2,
+                                                          const double* __restrict__ x1,
+                                                          const double* __restrict__ x2, double mu1, double mu2,
+                                                          double* __restrict__ o1, double* __restrict__ o2,
+                                                          int64_t sb, int64_t sx, int64_t so) {
+  const int M = schur_cell<true>(R.ucell_own, R.nu_own, blockIdx.x * blockDim.x + threadIdx.x);
+  if (M < 0) return;
+  const int64_t i = blockIdx.y;
+  const float* tb = R.table_f + (size_t)wave_type<true>(R, M) * R.stride;
+  schur_recover_one_f32<K>(R, tb, b1 + i * sb, x1 + i * sx, mu1, o1 + i * so, M);
+  if (NRHS > 1) schur_recover_one_f32<K>(R, tb, b2 + i * sb, x2 + i * sx, mu2, o2 + i * so, M);
+}
+
+// ------------------------------------------------------------------------
 // Supernodal triangular solves (S = L L^T), level by level
 // ------------------------------------------------------------------------
 // One workgroup per tile of rows of one supernode.  The vector the rows are
@@ -1046,6 +1178,90 @@ __global__ __launch_bounds__(64) void k_poisson_recover(DevPoisson P, const doub
   }
 }
 
+// Mixed precision: the cell condensation y = A^{-1} f and the recovery's
+// correction A^{-1} E^T lambda contract in FP32 with the FP32 table (the
+// right-hand side f, y and the results stay FP64); wave-uniform lists only.
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_cell_f32(DevPoisson P, DevTabs T, const DevPhys* __restrict__ phs,
+                                                         const double* __restrict__ un,
+                                                         const double* __restrict__ up,
+                                                         const double* __restrict__ ur,
+                                                         const double* __restrict__ uo, int ncs, int nce,
+                                                         double* __restrict__ y, int64_t ss, int64_t se,
+                                                         int64_t sy) {
+  const DevPhys ph = phs[blockIdx.y];
+  {
+    const int64_t i = blockIdx.y;
+    un += i * ss;
+    up += i * ss;
+    ur += i * se;
+    uo += i * se;
+    y += i * sy;
+  }
+  using S = Sz<K>;
+  constexpr int NRT = S::NRT, NL = S::NL;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Kc = t0 < P.nucell ? P.ucell[t0] : -1;
+  if (Kc < 0) return;
+  const int ty = __builtin_amdgcn_readfirstlane(P.type[Kc]);
+  const double* tb = P.table + (size_t)ty * P.stride;
+  const float* tf = P.table_f + (size_t)ty * P.stride;
+  double f[NL];
+#pragma unroll
+  for (int i = 0; i < NRT; ++i) f[i] = poisson_bias<K>(tb, ph, i);
+  poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, f + NRT);
+  float ff[NL];
+#pragma unroll
+  for (int j = 0; j < NL; ++j) ff[j] = (float)f[j];
+  for (int i = 0; i < NL; ++i) {
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) s += tf[i * NL + j] * ff[j];
+    y[(size_t)Kc * NL + i] = (double)s;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_recover_f32(DevPoisson P, const double* __restrict__ lam,
+                                                            const double* __restrict__ y, double* __restrict__ out,
+                                                            int64_t sl, int64_t sy, int64_t so) {
+  constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int Kc = t0 < P.nucell ? P.ucell[t0] : -1;
+  if (Kc < 0) return;
+  lam += blockIdx.y * sl;
+  y += blockIdx.y * sy;
+  out += blockIdx.y * so;
+  float e[4 * K];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const int base = P.mult_face[4 * (size_t)Kc + f];
+    const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+#pragma unroll
+    for (int m = 0; m < K; ++m)
+      e[f * K + m] = base < 0 ? 0.0f : (float)(sf * P.gw[m] * lam[P.mult_perm[base + m]]);
+  }
+  const float* tf = P.table_f + (size_t)__builtin_amdgcn_readfirstlane(P.type[Kc]) * P.stride;
+  const int* id = P.idx + (size_t)Kc * NL;
+  for (int i = 0; i < NL; ++i) {
+    float c = 0.0f;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int m = 0; m < K; ++m) c += tf[i * NL + rt_face<K>(f, m)] * e[f * K + m];
+    double s = y[(size_t)Kc * NL + i] - (double)c;
+    if (i < NRT) {
+      int m;
+      const int fc = face_of_local<K>(i, m);
+      if (fc >= 0) {
+        if (!owns_face<K>(P, Kc, fc)) continue;
+        if (P.pbid[4 * (size_t)Kc + fc] == 2) s = 0.0;  // Neumann D.n = 0 (Poisson.cpp:41-47)
+      }
+    }
+    out[id[i]] = s;
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTabs T, const DevPhys* __restrict__ phs,
                                                              const double* __restrict__ un,
@@ -1173,6 +1389,15 @@ int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* 
     }                                                                                                       \
     return (int)hipGetLastError();                                                                          \
   }
+  if (R.table_f && R.ucell_nbh) {  // mixed precision
+    const dim3 g(nblk(R.nu_nbh, 64), nb);
+    if (nrhs == 2) {
+      PECS_DISPATCH_K(k, (k_schur_rhs_f32<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, mu1, mu2, g1, g2, sb, sg)));
+    } else {
+      PECS_DISPATCH_K(k, (k_schur_rhs_f32<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, mu1, mu1, g1, g1, sb, sg)));
+    }
+    return (int)hipGetLastError();
+  }
   // a pass per right-hand side once a batch fills the chip (4 or more instances)
   if (R.ucell_nbh && nb >= 4) PECS_SCHUR_RHS(true, true, R.nu_nbh)
   if (R.ucell_nbh) PECS_SCHUR_RHS(true, false, R.nu_nbh)
@@ -1194,6 +1419,17 @@ int launch_schur_recover(int k, const DevRegion& R, const double* b1, const doub
                                                                       sb, sx, so)));                            \
     }                                                                                                           \
     return (int)hipGetLastError();                                                                              \
+  }
+  if (R.table_f && R.ucell_own) {  // mixed precision
+    const dim3 g(nblk(R.nu_own, 64), nb);
+    if (nrhs == 2) {
+      PECS_DISPATCH_K(k, (k_schur_recover_f32<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, xs1, xs2, mu1, mu2, out1, out2, sb,
+                                                                      sx, so)));
+    } else {
+      PECS_DISPATCH_K(k, (k_schur_recover_f32<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, xs1, xs1, mu1, mu1, out1, out1, sb,
+                                                                      sx, so)));
+    }
+    return (int)hipGetLastError();
   }
   if (R.ucell_own) PECS_RECOVER(true, R.nu_own)
   PECS_RECOVER(false, R.ncell)
@@ -1319,6 +1555,11 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
 int launch_poisson_cell(int k, const DevPoisson& P, const DevTabs& T, const DevPhys* phs, const double* un,
                         const double* up, const double* ur, const double* uo, int ncs, int nce, double* y,
                         hipStream_t s, int nb, int64_t ss, int64_t se, int64_t sy) {
+  if (P.table_f && P.ucell) {  // mixed precision
+    PECS_DISPATCH_K(k, (k_poisson_cell_f32<KK><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(
+                           P, T, phs, un, up, ur, uo, ncs, nce, y, ss, se, sy)));
+    return (int)hipGetLastError();
+  }
   if (P.ucell) {
     PECS_DISPATCH_K(k, (k_poisson_cell<KK, true><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(
                            P, T, phs, un, up, ur, uo, ncs, nce, y, ss, se, sy)));
@@ -1343,6 +1584,11 @@ int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStre
 
 int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
                            hipStream_t s, int nb, int64_t sl, int64_t sy, int64_t so) {
+  if (P.table_f && P.ucell) {  // mixed precision
+    PECS_DISPATCH_K(k, (k_poisson_recover_f32<KK><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl, sy,
+                                                                                              so)));
+    return (int)hipGetLastError();
+  }
   if (P.ucell) {
     PECS_DISPATCH_K(k, (k_poisson_recover<KK, true><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl,
                                                                                                  sy, so)));

@@ -29,6 +29,9 @@ struct DevRegion {
   const int* ucell_own;
   const int* ucell_nbh;
   int nu_own, nu_nbh;
+  // mixed precision (null: FP64): FP32 copy of the table; with the
+  // wave-uniform lists the Schur kernels then contract in FP32
+  const float* table_f;
 };
 
 struct DevPoisson {
@@ -50,6 +53,7 @@ struct DevPoisson {
   const double* gw;      // face Gauss weights (k)
   const int* ucell;      // cells grouped by type in whole waves, padded with -1 (null: cells in order)
   int nucell;
+  const float* table_f;  // mixed precision (null: FP64): FP32 copy of the table (Ainv used)
 };
 
 struct DevPhys {
