@@ -14,6 +14,11 @@
 //   k_schur_recover q = W - A^{-1} B u; writes [Jx | Jy | u].
 //   k_poisson_*     hybridised mixed Poisson: cell condensation, face
 //                   assembly, multiplier solve, cell recovery.
+// Batches of instances sharing the factors: k_sweep_mfma (FP64 MFMA) with
+// RHS-interleaved vectors and their gathers.  The Schur and Poisson cell
+// kernels take wave-uniform cell lists where the mesh has few cell types
+// (table entries as scalar operands), and FP32 variants (*_f32) serve the
+// mixed-precision mode.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
