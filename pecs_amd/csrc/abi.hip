@@ -544,8 +544,8 @@ std::vector<int> wave_groups(int n, int nkey, KeyFn key) {
   return out;
 }
 
-DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, const std::vector<int>& sigma,
-                          const FactorDev& F) {
+DevRegion make_region_dev(pecs_ctx* c, int reg, const Region& R, const LdgRegionOps& O,
+                          const std::vector<int>& sigma, const FactorDev& F) {
   DevRegion d{};
   d.N = R.N;
   d.ncell = R.ncell;
@@ -587,7 +587,7 @@ DevRegion make_region_dev(pecs_ctx* c, const Region& R, const LdgRegionOps& O, c
       d.nu_nbh = (int)nbh.size();
     }
     if (d.ucell_own && d.ucell_nbh)
-      c->tabf_host[&R == &c->mesh.semi ? 0 : 1].assign(O.table.begin(), O.table.end());
+      c->tabf_host[reg].assign(O.table.begin(), O.table.end());
   }
   return d;
 }
@@ -974,8 +974,8 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     tb.psix = upload(c.get(), T.psix);
     tb.psiy = upload(c.get(), T.psiy);
     // regions
-    c->Rs = make_region_dev(c.get(), c->mesh.semi, os, c->mesh.semi_sigma_nb, c->Fs[0]);
-    c->Re = make_region_dev(c.get(), c->mesh.elec, oe, c->mesh.elec_sigma_nb, c->Fe[0]);
+    c->Rs = make_region_dev(c.get(), 0, c->mesh.semi, os, c->mesh.semi_sigma_nb, c->Fs[0]);
+    c->Re = make_region_dev(c.get(), 1, c->mesh.elec, oe, c->mesh.elec_sigma_nb, c->Fe[0]);
     // Poisson
     DevPoisson& P = c->Pd;
     const SolarCellMesh& m = c->mesh;
