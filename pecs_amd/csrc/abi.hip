@@ -285,7 +285,7 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
         if (nrhs > 2) {
           for (int m : {1, 2, 4})
             for (int rows : {64 * m, 32 * m})
-              for (int ks : {0, 4096, 2048, 1024, 512, 256}) cand.push_back({0, m, rows, ks});
+              for (int ks : {0, 4096, 2048, 1024, 512, 256, 128}) cand.push_back({0, m, rows, ks});
         } else {
           for (int w : {8, 16, 32, 64})
             for (int m : {1, 2, 4, 8}) {
