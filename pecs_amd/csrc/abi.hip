@@ -51,6 +51,7 @@ struct pecs_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;            // second stream for the concurrent region solves
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipEvent_t rhs_ev[2] = {};  // a region's right-hand side done (it has read the other region's old state)
   // one time step captured as a graph (pecs_step replays it); cap is the
   // stream it is captured on (capture needs a stream other than the null stream)
   hipStream_t cap = nullptr;
@@ -120,6 +121,8 @@ struct pecs_ctx {
     }
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
+    for (hipEvent_t e : rhs_ev)
+      if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
   }
@@ -691,6 +694,21 @@ void region_recover(pecs_ctx* c, int reg, hipStream_t s) {
                                             c->u[c1], 2, s, c->B, N, n, N));
 }
 
+// The Poisson phases of a step on the context stream (cell condensation,
+// face assembly and multiplier solve, recovery), with phase marks if timed.
+void poisson_step(pecs_ctx* c, PhaseMarks* pm = nullptr) {
+  hipStream_t s = c->stream;
+  const int B = c->B;
+  poisson_all_cells(c);
+  if (pm) pm->mark();
+  const int64_t nH = c->FH.n;
+  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s, B, c->ysz, nH));
+  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, B, s));
+  if (pm) pm->mark();
+  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s, B, nH, c->ysz, c->NP));
+  if (pm) pm->mark();
+}
+
 void step_once(pecs_ctx* c) {
   hipStream_t s = c->stream;
   PhaseMarks pm{c, nullptr};
@@ -701,6 +719,34 @@ void step_once(pecs_ctx* c) {
   }
   const int64_t Ns = c->Ns, Ne = c->Ne, NP = c->NP;
   const int B = c->B;
+  if (!pm.ev) {
+    // Without per-phase timing each region's chain (right-hand side, Schur
+    // transform, solve, recovery) runs on its own stream, joined before the
+    // Poisson phase.  A region's recovery writes its new state, which the
+    // other region's right-hand side reads as the old one (the lagged
+    // interface traces): it waits for that right-hand side.
+    hipStream_t d = c->side;
+    HIPCHECK(hipEventRecord(c->fork_ev, s));
+    HIPCHECK(hipStreamWaitEvent(d, c->fork_ev, 0));
+    HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->phs_d, c->u[0], c->u[1],
+                                            c->u[2], c->u[3], c->pv, c->b[0], c->b[1], s, B, Ns, Ne, NP));
+    HIPCHECK(hipEventRecord(c->rhs_ev[0], s));
+    HIPCHECK((hipError_t)launch_rhs_carrier(c->k, false, c->Re, c->Rs, c->Pd, c->tabs, c->phs_d, c->u[2], c->u[3],
+                                            c->u[0], c->u[1], c->pv, c->b[2], c->b[3], d, B, Ne, Ns, NP));
+    HIPCHECK(hipEventRecord(c->rhs_ev[1], d));
+    region_schur(c, 0, s);
+    region_schur(c, 1, d);
+    region_chol(c, 0, s);
+    region_chol(c, 1, d);
+    HIPCHECK(hipStreamWaitEvent(s, c->rhs_ev[1], 0));
+    region_recover(c, 0, s);
+    HIPCHECK(hipStreamWaitEvent(d, c->rhs_ev[0], 0));
+    region_recover(c, 1, d);
+    HIPCHECK(hipEventRecord(c->join_ev, d));
+    HIPCHECK(hipStreamWaitEvent(s, c->join_ev, 0));
+    poisson_step(c);
+    return;
+  }
   HIPCHECK((hipError_t)launch_rhs_carrier(c->k, true, c->Rs, c->Re, c->Pd, c->tabs, c->phs_d, c->u[0], c->u[1],
                                           c->u[2], c->u[3], c->pv, c->b[0], c->b[1], s, B, Ns, Ne, NP));
   pm.mark();
@@ -722,14 +768,7 @@ void step_once(pecs_ctx* c) {
   region_recover(c, 0, s);
   region_recover(c, 1, s);
   pm.mark();
-  poisson_all_cells(c);
-  pm.mark();
-  const int64_t nH = c->FH.n;
-  HIPCHECK((hipError_t)launch_poisson_face(c->Pd, c->yP, c->rH, s, B, c->ysz, nH));
-  HIPCHECK((hipError_t)launch_chol_solve(c->FH.d, c->FH.lv(), c->rH, c->tH, B, s));
-  pm.mark();
-  HIPCHECK((hipError_t)launch_poisson_recover(c->k, c->Pd, c->rH, c->yP, c->pv, s, B, nH, c->ysz, NP));
-  pm.mark();
+  poisson_step(c, &pm);
 }
 
 // The state before the last step of a pecs_step call (for the steady residual).
@@ -923,6 +962,7 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     HIPCHECK(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+    for (hipEvent_t& e : c->rhs_ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     c->P = to_params(params);
     for (int i = 0; i < B; ++i) c->Pb.push_back(to_params(&params[i]));
     c->k = params->degree;
