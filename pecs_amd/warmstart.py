@@ -3,7 +3,7 @@
 A network trained on steady states that the time stepper itself produced
 predicts the steady densities of a new (applied bias, generation amplitude,
 donor doping, cone half-angle) draw; the stepper then starts from the
-prediction instead of the equilibrium state and remains the judge of
+prediction instead of the equilibrium state and still decides
 convergence (the same relative steady residual).  A prediction whose residual
 after the first block of steps is above `fallback_residual` is dropped and the
 instance restarts from the equilibrium state (its steps so far still count).

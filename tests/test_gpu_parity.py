@@ -4,7 +4,7 @@ Bars (FP64): right-hand sides 1e-13 relative (same quadrature, different
 summation order); single solves 1e-11 relative (the oracle solves the full
 [J; u] / mixed system with SuperLU, the GPU the SPD Schur / hybridised
 systems with a supernodal Cholesky); full trajectories 1e-10 relative in
-densities and potential.
+densities and potential (the trajectory bar).
 """
 import numpy as np
 import pytest
