@@ -32,11 +32,21 @@ Parity pins (what fixes each function without the reference, which cannot be
 built here: it needs deal.II 8.x + UMFPACK + TBB):
   * quadrature, Lagrange and RT tabulations: closed forms and numpy's
     Gauss-Legendre routine (tests/test_oracle_fe.py);
-  * element matrices: sympy-exact integrals on rectangles and a trapezoid
-    (tests/golden/element_fixtures.json, made by tests/golden/make_fixtures.py);
-  * LDG / mixed matrices and right-hand sides: the reference's manufactured
-    solutions at textbook rates, orientation invariance of the interior
-    flux, interface antisymmetry and mass conservation invariants
+  * element matrices: sympy-exact integrals on 2 x 2-cell rectangles
+    (tests/golden/ldg_*.json, mixed_*.json, made by
+    tests/golden/make_fixtures.py);
+  * LDG / mixed matrices: the reference's manufactured solutions at
+    textbook rates, orientation invariance of the interior flux
     (tests/test_oracle_manufactured.py, tests/test_oracle_invariants.py);
+  * carrier and Poisson right-hand sides (ldg.carrier_rhs,
+    mixed.poisson_rhs): 40-digit golden vectors on 2 x 2-cell rectangles and
+    trapezoids with every term switched on (tests/golden/rhs_*.json, made by
+    tests/golden/make_rhs_fixtures.py), with planted mistakes that must be
+    caught (tests/test_oracle_rhs_fixtures.py); insulated-boundary mass
+    conservation, the Boltzmann zero-current steady state at rate k+1 (drift
+    signs and the permittivity), generation moments and Poisson functionals
+    against closed-form integrals (tests/test_oracle_physics_pins.py);
+    interface antisymmetry and the neutral fixed point
+    (tests/test_oracle_invariants.py);
   * solves: residual checks of the sparse LU.
 """

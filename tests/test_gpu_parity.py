@@ -4,7 +4,15 @@ Bars (FP64): right-hand sides 1e-13 relative (same quadrature, different
 summation order); single solves 1e-11 relative (the oracle solves the full
 [J; u] / mixed system with SuperLU, the GPU the SPD Schur / hybridised
 systems with a supernodal Cholesky); full trajectories 1e-10 relative in
-densities and potential (the trajectory bar).
+densities and potential (the trajectory bar), each norm relative to the
+oracle's own norm with no floor.
+
+Cases: the small workload (refinement 4, k = 1, dt = 1e-3, 200 steps) in
+both physics modes; conic / insulated / Schottky and biased cases; and two
+cases at refinement 5, k = 2, dt = 5e-4 with the measured workload's physics
+(Beer-Lambert generation, SRH, transfer rates 1e-2), planar and conic, whose
+elimination trees, tuned tile shapes and sweep levels are those of the
+larger meshes.
 """
 import numpy as np
 import pytest
@@ -47,11 +55,18 @@ CASES = {
                                            phi_app=0.1, dt=5e-4),
     "k2-r3-bias": dict(k=2, r=3, phi_bi=1.0, phi_app=0.25, ne=2.0, rinf=30.0, oinf=29.0, dt=5e-4),
 }
+# refinement 5, k = 2, dt = 5e-4, the measured workload's physics (trajectory only)
+R5 = {
+    "k2-r5-config-physics": dict(k=2, r=5, dt=5e-4, gen_mode="config", srh=True, ni=0.1),
+    "k2-r5-conic-config-physics": dict(k=2, r=5, dt=5e-4, gen_mode="config", srh=True, ni=0.1,
+                                       r2=0.5 + np.tan(np.radians(12.0)), L=1.5, insulated=True, schottky=True,
+                                       v_rec_n=0.3, v_rec_p=0.2, phi_bi=0.4, phi_app=0.1),
+}
 
 
 def _prm(name):
     p = default_params()
-    p.update(CASES[name])
+    p.update(CASES[name] if name in CASES else R5[name])
     return p
 
 
@@ -93,8 +108,9 @@ def test_solve_parity(name, seed):
     g.close()
 
 
-@pytest.mark.parametrize("name,nsteps", [("C1", 200), ("C1-config-physics", 50),
-                                         ("k2-r4-conic-insulated-schottky", 20), ("k2-r3-bias", 40)])
+@pytest.mark.parametrize("name,nsteps", [("C1", 200), ("C1-config-physics", 200),
+                                         ("k2-r4-conic-insulated-schottky", 20), ("k2-r3-bias", 40),
+                                         ("k2-r5-config-physics", 20), ("k2-r5-conic-config-physics", 20)])
 def test_trajectory_parity(name, nsteps):
     _say(f"[gpu] trajectory case={name} steps={nsteps}")
     p = _prm(name)
@@ -105,17 +121,35 @@ def test_trajectory_parity(name, nsteps):
     gs = g.get_state()
     for k in "npro":
         assert np.array_equal(gs[k], st[k])
-    assert np.linalg.norm(gs["P"] - st["P"]) <= 1e-12 * max(1.0, np.linalg.norm(st["P"]))
+    if np.linalg.norm(st["P"]) == 0.0:
+        assert np.linalg.norm(gs["P"]) < 1e-300 + 1e-15
+    else:
+        assert _rel(gs["P"], st["P"]) < 1e-12, _rel(gs["P"], st["P"])
     ref = o.run(nsteps, st)
     g.step(nsteps)
     gs = g.get_state()
+    errs = {}
     for k in "npro":
         nb = o.cfe.nb
         nc = len(ref[k]) // (3 * nb)
-        d_ref = ref[k][2 * nb * nc:]
-        d_gpu = gs[k][2 * nb * nc:]
-        assert _rel(d_gpu, d_ref) < 1e-10, (k, _rel(d_gpu, d_ref))
-        assert _rel(gs[k], ref[k]) < 1e-10, (k, _rel(gs[k], ref[k]))
-    assert np.linalg.norm(gs["P"] - ref["P"]) <= 1e-10 * max(np.linalg.norm(ref["P"]), 1e-3), \
-        np.linalg.norm(gs["P"] - ref["P"])
+        errs[k + ".density"] = _rel(gs[k][2 * nb * nc:], ref[k][2 * nb * nc:])
+        errs[k] = _rel(gs[k], ref[k])
+    # The potential solves lambda^2 div D = rho with rho a difference of O(1)
+    # densities that nearly cancel (||P|| ~ 1e-5..1e-4 at zero bias), so a
+    # density difference of 1e-14 can move P by more than 1e-10 of its norm.
+    # Derived bar: the Poisson step alone (the oracle's solve on the GPU's own
+    # densities) must match to 1e-10 relative with no floor, and the
+    # trajectory's P may differ from the oracle's by 1e-10 of its norm plus
+    # the potential that the measured density difference induces (computed
+    # by the oracle: its solve on the GPU's densities minus its own P).
+    P_own = o.solve("P", o.rhs_poisson(gs))
+    errs["P.step"] = _rel(gs["P"], P_own)
+    induced = np.linalg.norm(P_own - ref["P"])
+    dP = np.linalg.norm(gs["P"] - ref["P"])
+    nP = np.linalg.norm(ref["P"])
+    _say(f"[gpu] trajectory {name}: ||P|| = {nP:.3e}, " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+         f", P {dP / max(nP, 1e-300):.2e} (induced by the density difference: {induced / max(nP, 1e-300):.2e})")
+    for k, v in errs.items():
+        assert v < 1e-10, (k, v)
+    assert dP <= 1e-10 * nP + induced, (dP, nP, induced)
     g.close()
