@@ -247,6 +247,20 @@ int pecs_rhs_electrolyte_dev(pecs_ctx* ctx, const double* u_r, const double* u_o
 int pecs_rhs_poisson_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* u_r,
                          const double* u_o, double* rhs);
 int pecs_solve_dev(pecs_ctx* ctx, int which, const double* rhs, double* x);
+/* pecs_solve_dev without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory at the point of the call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as pecs_solve_dev (same kernels, same order), but
+ * its scratch (the Schur or multiplier right-hand side, the triangular
+ * sweeps' update vectors, the condensed Poisson cell data) lives in one
+ * static device array of 64 MiB per GPU, held until the process ends, instead
+ * of the context's buffers.  Single-instance contexts only.  A system whose
+ * scratch exceeds 64 MiB is rejected with an error (refinement 8, k = 2
+ * needs 32.7 MB for a carrier and 42.2 MB for the Poisson system).  The
+ * array is found on the device that is current at the call; contexts created
+ * on that device set it up.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two host threads) race. */
+int pecs_solve_dev_noalloc(pecs_ctx* ctx, int which, const double* rhs, double* x);
 /* Device pointers of the resident state (valid until pecs_destroy); a batch
  * stores instance i at offset i * (n_semi, n_elec or n_poisson). */
 int pecs_state_dev(pecs_ctx* ctx, double** u_n, double** u_p, double** u_r, double** u_o, double** poisson);

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -25,6 +26,14 @@
 #include "vtu.hpp"
 
 using namespace pecs;
+
+// Device scratch of pecs_solve_dev_noalloc, the entry point that allocates
+// nothing: one array per GPU (the code object's own global), held until the
+// process ends.  Host code never uses the symbol itself (its host-side name is
+// not a device address): set-up looks up each device's address once with
+// hipGetSymbolAddress and the solves take that pointer.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
 
@@ -142,6 +151,23 @@ T* dalloc(pecs_ctx* c, size_t n) {
   HIPCHECK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
   c->allocs.push_back(p);
   return (T*)p;
+}
+
+// per-device address of `scratch`, looked up once per device (with that
+// device current) when the first context is created on it
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+void setup_noalloc_scratch() {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return;  // pecs_solve_dev_noalloc then reports it
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev]) return;
+  void* p = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_dev[dev] = static_cast<double*>(p);
 }
 
 template <class T>
@@ -958,6 +984,7 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
       throw std::runtime_error("no HIP device: the PECS path runs on the GPU only");
+    setup_noalloc_scratch();
     HIPCHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
@@ -1314,6 +1341,74 @@ int pecs_solve_dev(pecs_ctx* ctx, int which, const double* rhs, double* x) {
     } else if (which == PECS_POISSON) {
       HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, rhs, ctx->yP, ctx->stream));
       poisson_solve_from_y(ctx, 0, 1, x);
+    } else {
+      throw std::runtime_error("unknown system id");
+    }
+  })
+}
+
+int pecs_solve_dev_noalloc(pecs_ctx* ctx, int which, const double* rhs, double* x) {
+  PECS_TRY(ctx, {
+    if (ctx->B != 1)
+      throw std::runtime_error("pecs_solve_dev_noalloc: single-instance contexts only (a batch's MFMA-shaped "
+                               "factors need right-hand-side-interleaved scratch)");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    double* base = (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? g_scratch_dev[dev] : nullptr;
+    if (!base)
+      throw std::runtime_error("pecs_solve_dev_noalloc: no device scratch on the current device (create a "
+                               "context on it first)");
+    size_t cap = NOALLOC_SCRATCH_BYTES / sizeof(double);
+    // PECS_NOALLOC_LIMIT_BYTES may only lower the limit (to exercise the size check)
+    if (const char* lim = std::getenv("PECS_NOALLOC_LIMIT_BYTES")) {
+      const size_t l = (size_t)std::strtoull(lim, nullptr, 10) / sizeof(double);
+      if (l < cap) cap = l;
+    }
+    hipStream_t s = ctx->stream;
+    if (which >= PECS_ELECTRONS && which <= PECS_OXIDANTS) {
+      // scratch: X (n), Z (zlen) of the factor, then the Schur right-hand side
+      // g (n) and the forward output t (n); every entry is written before it is read
+      const int reg = which / 2, car = which % 2;
+      const DevRegion& R = reg == 0 ? ctx->Rs : ctx->Re;
+      FactorDev* F = reg == 0 ? ctx->Fs : ctx->Fe;
+      const bool share = reg == 0 ? ctx->share_s : ctx->share_e;
+      const FactorDev& Fc = (car == 1 && !share) ? F[1] : F[0];
+      const size_t n = (size_t)Fc.n, zl = (size_t)std::max<int64_t>(Fc.d.zlen, 1);
+      const size_t need = 3 * n + zl;
+      if (need > cap)
+        throw std::runtime_error("pecs_solve_dev_noalloc: carrier solve needs " + std::to_string(need * 8) +
+                                 " bytes of scratch, above the limit of " + std::to_string(cap * 8));
+      DevFactor d = Fc.d;
+      d.X = base;
+      d.Z = base + n;
+      d.Part = nullptr;
+      double* g = base + n + zl;
+      double* t = g + n;
+      const double mu = ctx->P.mu[which];
+      HIPCHECK((hipError_t)launch_schur_rhs(ctx->k, R, rhs, rhs, mu, mu, g, g, 1, s));
+      HIPCHECK((hipError_t)launch_chol_solve(d, Fc.lv(), g, t, 1, s));
+      HIPCHECK((hipError_t)launch_schur_recover(ctx->k, R, rhs, rhs, g, g, mu, mu, x, x, 1, s));
+    } else if (which == PECS_POISSON) {
+      // scratch: X (nH), Z (zlen) of the multiplier factor, the condensed
+      // cell data y (ysz), the multiplier right-hand side r (nH) and the
+      // forward output t (nH)
+      const FactorDev& FH = ctx->FH;
+      const size_t nH = (size_t)FH.n, zl = (size_t)std::max<int64_t>(FH.d.zlen, 1), ys = (size_t)ctx->ysz;
+      const size_t need = 3 * nH + zl + ys;
+      if (need > cap)
+        throw std::runtime_error("pecs_solve_dev_noalloc: Poisson solve needs " + std::to_string(need * 8) +
+                                 " bytes of scratch, above the limit of " + std::to_string(cap * 8));
+      DevFactor d = FH.d;
+      d.X = base;
+      d.Z = base + nH;
+      d.Part = nullptr;
+      double* y = base + nH + zl;
+      double* r = y + ys;
+      double* t = r + nH;
+      HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, rhs, y, s));
+      HIPCHECK((hipError_t)launch_poisson_face(ctx->Pd, y, r, s, 1, ctx->ysz, (int64_t)nH));
+      HIPCHECK((hipError_t)launch_chol_solve(d, FH.lv(), r, t, 1, s));
+      HIPCHECK((hipError_t)launch_poisson_recover(ctx->k, ctx->Pd, r, y, x, s, 1, (int64_t)nH, ctx->ysz, ctx->NP));
     } else {
       throw std::runtime_error("unknown system id");
     }
