@@ -11,13 +11,14 @@ value = DOF-updates/s = (4 N_c + N_P) * steps / seconds, state resident in HBM.
 The roofline object is for the dominant kernel family (the supernodal
 triangular solves of the semiconductor Schur factor, two right-hand sides),
 timed live with hipEvents on the context stream over the timed steps.
-cpu_baseline = the numpy/scipy oracle (a port of the reference algorithm,
-single core) on a bounded sample, reported beside it.
+cpu_baseline = the compiled CPU restatement of the step (oracle/cpu_step.cpp:
+own C++ FP64, OpenMP on all host cores; not deal.II, which cannot be built
+here) on the measured workload, a bounded number of steps, reported beside it.
 
 time_to_steady_state: wall seconds from the equilibrium initial state until
 the relative steady residual ||x^{n+1}-x^n|| / (||x^n|| dt) over the five
 state vectors falls to 1e-8 (the reference has no criterion of its own),
-on the measured workload and, beside the oracle on one core, on the small
+on the measured workload and, beside the CPU restatement, on the small
 workload (refinement 4, k = 1, dt = 1e-3).
 
 batched: an I-V sweep of 16 applied biases on the measured mesh stepped
@@ -42,26 +43,49 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """Oracle (numpy/scipy, one core) on a bounded sample: refinement 5, k = 2."""
-    from oracle.solarcell import OracleSolarCell
+CPU_STEP = os.path.join(ROOT, "oracle", "_build", "pecs_cpu_step")
+
+
+def _cpu_step(prm, steps, tol=0.0, timeout=1800):
+    """Run the compiled CPU restatement of the step (oracle/cpu_step.cpp) on
+    all host cores (OMP_NUM_THREADS if set); returns its JSON report."""
+    import subprocess
+    if not os.path.exists(CPU_STEP):
+        raise RuntimeError(f"{CPU_STEP} not built (run __graft_entry__.build())")
+    gen = {"ref": 0, "config": 1, "off": 2}[prm["gen_mode"]]
+    args = [f"{k}={float(prm[k])!r}" for k in ("k", "r", "dt", "penalty", "mu_n", "mu_p", "mu_r", "mu_o", "eps_s",
+                                                 "eps_e", "lam2", "k_et", "k_ht", "v_rec_n", "v_rec_p", "ne", "pe",
+                                                 "rinf", "oinf", "phi_bi", "phi_app", "G0", "alpha", "ni", "tau_n",
+                                                 "tau_p", "r1", "r2", "H", "L")]
+    args += [f"gen_mode={gen}", f"srh={int(bool(prm['srh']))}", f"insulated={int(bool(prm['insulated']))}",
+             f"schottky={int(bool(prm['schottky']))}", f"steps={steps}", f"tol={tol!r}"]
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(os.cpu_count() or 1))
+    r = subprocess.run([CPU_STEP] + args, capture_output=True, text=True, timeout=timeout, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"pecs_cpu_step failed ({r.returncode}): {r.stderr[-500:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(args, gpu_steps_to_steady=None):
+    """The compiled CPU restatement (own C++ FP64 step, not deal.II: OpenMP
+    right-hand sides, Schur / multiplier solves with host supernodal Cholesky
+    factors and level-parallel sweeps) on the measured workload itself, a
+    bounded number of steps, on all host cores."""
     from pecs_amd import workload_params
-    p = workload_params(5, 2, 5e-4)
-    o = OracleSolarCell(p)
-    st = o.initial_state()
-    st = o.step(st)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        st = o.step(st)
-        n += 1
-        if time.perf_counter() - t0 > seconds_budget:
-            break
-    dt = time.perf_counter() - t0
-    U = 4 * o.Nc + o.pfe.n
-    return {"value": U * n / dt, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy/scipy SuperLU, 1 core), refinement 5 (32x32 cells/region), k=2, "
-                      f"{n} steps, U={U} DOFs/step, setup excluded"}
+    p = workload_params(args.refine, args.degree, args.dt)
+    rep = _cpu_step(p, args.cpu_steps)
+    out = {"value": rep["dof_updates_per_step"] / rep["step_s"], "unit": "DOF-updates/s", "cores": rep["threads"],
+           "kind": "port",
+           "sample": f"own C++ FP64 restatement of the step (oracle/cpu_step.cpp), not deal.II: the measured "
+                     f"workload (refinement {args.refine}, k={args.degree}, dt={args.dt}), {rep['steps']} steps, "
+                     f"{rep['threads']} OpenMP threads; host factorisation (setup {rep['setup_s']:.1f} s) excluded",
+           "ms_per_step": rep["step_s"] * 1e3, "setup_s": rep["setup_s"]}
+    if gpu_steps_to_steady:
+        # the steady state needs the same number of steps on either path (same algorithm)
+        out["time_to_steady_s_estimate"] = gpu_steps_to_steady * rep["step_s"]
+        out["time_to_steady_basis"] = f"{gpu_steps_to_steady} steps (the GPU run's count) x the CPU step time"
+    return out
 
 
 def profiled_traffic(workload):
@@ -81,23 +105,12 @@ def profiled_traffic(workload):
 
 
 def cpu_time_to_steady(tol, max_steps=20000):
-    """Oracle (one core) on the small workload: refinement 4, k = 1, dt = 1e-3."""
-    import numpy as np
-    from oracle.solarcell import OracleSolarCell
+    """The compiled CPU restatement on the small workload (refinement 4, k = 1,
+    dt = 1e-3), stepped until the relative steady residual is at most tol."""
     from pecs_amd import workload_params
-    o = OracleSolarCell(workload_params(4, 1, 1e-3))
-    st = o.initial_state()
-    t0 = time.perf_counter()
-    vec = lambda s: np.concatenate([s[k] for k in "nproP"])
-    n, res = 0, float("inf")
-    while n < max_steps and res > tol:
-        new = o.step(st)
-        a, b = vec(new), vec(st)
-        res = float(np.linalg.norm(a - b) / (np.linalg.norm(b) * 1e-3))
-        st = new
-        n += 1
-    return {"s": time.perf_counter() - t0, "steps": n, "residual": res, "cores": 1, "kind": "port",
-            "reached": res <= tol}
+    rep = _cpu_step(workload_params(4, 1, 1e-3), max_steps, tol)
+    return {"s": rep["run_s"], "steps": rep["steps"], "residual": rep["residual"], "cores": rep["threads"],
+            "kind": "port", "reached": 0 <= rep["residual"] <= tol}
 
 
 def gpu_time_to_steady(g, tol, check_every, max_steps):
@@ -447,7 +460,7 @@ def main():
     ap.add_argument("--degree", type=int, default=2)
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="steps of the CPU baseline on the measured workload")
     ap.add_argument("--steady-tol", type=float, default=1e-8)
     ap.add_argument("--steady-max-steps", type=int, default=8000)
     ap.add_argument("--steady-check-every", type=int, default=25)
@@ -635,7 +648,8 @@ def main():
         out["sweep"] = sweep_leg(args)
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+            tts = out.get("time_to_steady_state", {}).get("gpu", {})
+            out["cpu_baseline"] = cpu_baseline(args, tts.get("steps") if tts.get("reached") else None)
         except Exception as e:  # report, do not hide
             out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)

@@ -108,6 +108,7 @@ struct pecs_ctx {
   std::vector<double> last_res;
   double setup_seconds = 0.0, factor_seconds = 0.0;
   std::vector<void*> allocs;
+  void* blas = nullptr;  // dense-library handle of the device factorisations (created at the first)
   // optional per-phase timing of pecs_step (hipEvents recorded between phases)
   int timing = 0;
   std::vector<hipEvent_t> events;  // NPHASE+1 per recorded step
@@ -134,6 +135,7 @@ struct pecs_ctx {
       if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
+    factor_handle_destroy(blas);
   }
 };
 
@@ -433,7 +435,8 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
       throw std::runtime_error("make_factor: out of device memory for the factor");
     }
     try {
-      const DeviceFactorStats st = factor_on_device(plan, F, P, dfwd, dbwd, c->stream);
+      if (!c->blas) c->blas = factor_handle_create();
+      const DeviceFactorStats st = factor_on_device(plan, F, P, dfwd, dbwd, c->stream, c->blas);
       D.device_seconds = st.seconds;
       D.device_flops = st.flops;
     } catch (...) {

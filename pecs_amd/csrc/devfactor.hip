@@ -215,11 +215,30 @@ struct Group {
 
 }  // namespace
 
+void* factor_handle_create() {
+  rocblas_handle h = nullptr;
+  RBCHECK(rocblas_create_handle(&h));
+  return h;
+}
+
+void factor_handle_destroy(void* handle) {
+  if (handle) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(handle));
+}
+
 DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor& F, const PackedFactor& P,
-                                   double* fwd, double* bwd, hipStream_t st) {
+                                   double* fwd, double* bwd, hipStream_t st, void* handle) {
   const auto t0 = std::chrono::steady_clock::now();
   DeviceFactorStats stats;
   const int nn = (int)F.nodes.size();
+  if (!handle) throw std::runtime_error("device factorisation: no dense-library handle");
+  {  // the fronts are the large allocation: fail with a clear message, not inside the dense calls
+    size_t free_b = 0, total_b = 0;
+    DFCHECK(hipMemGetInfo(&free_b, &total_b));
+    const double need = 8.0 * (double)plan.front_total + 8.0 * (double)plan.svals.size() + (256.0 * (1 << 20));
+    if (need > (double)free_b)
+      throw std::runtime_error("device factorisation: the fronts need " + std::to_string(need * 1e-9) +
+                               " GB, the device has " + std::to_string(free_b * 1e-9) + " GB free");
+  }
   Pool pool(st);
   // plan arrays
   std::vector<ScItem> sc(plan.sc_src.size());
@@ -278,8 +297,7 @@ DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor
     ev.resize(2 * F.nlevels + 2);
     for (auto& e : ev) DFCHECK(hipEventCreate(&e));
   }
-  rocblas_handle h = nullptr;
-  RBCHECK(rocblas_create_handle(&h));
+  rocblas_handle h = static_cast<rocblas_handle>(handle);
   try {
     RBCHECK(rocblas_set_stream(h, st));
     const double one = 1.0, minus_one = -1.0;
@@ -413,12 +431,10 @@ DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor
     }
   } catch (...) {
     (void)hipStreamSynchronize(st);
-    rocblas_destroy_handle(h);
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (auto& pe : opev) (void)hipEventDestroy(pe.second);
     throw;
   }
-  RBCHECK(rocblas_destroy_handle(h));
   stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return stats;
 }

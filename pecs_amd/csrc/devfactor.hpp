@@ -25,9 +25,17 @@ struct DeviceFactorStats {
   double front_bytes = 0.0;
 };
 
+// One dense-library (rocBLAS / rocSOLVER) handle, created once per context
+// and reused by all of its factorisations.
+void* factor_handle_create();
+void factor_handle_destroy(void* handle);
+
 // fwd / bwd: device arrays of P.nvals doubles (P from pack_layout of F).
+// handle: from factor_handle_create.  The fronts (plan.front_total doubles)
+// are allocated for the call and freed before it returns; the call checks
+// first that the device has room for them.
 DeviceFactorStats factor_on_device(const FrontPlan& plan, const SupernodalFactor& F, const PackedFactor& P,
-                                   double* fwd, double* bwd, hipStream_t stream);
+                                   double* fwd, double* bwd, hipStream_t stream, void* handle);
 
 // The MFMA-blocked copies of pack_blocked, built on the device from fwd / bwd
 // (P must hold blocked_layout's offsets; fblk / bblk have fblk_off.back() /
