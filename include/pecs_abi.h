@@ -15,7 +15,9 @@
  *   lexicographic DoFs per cell (CarrierPair.cpp:23-64);
  *   Poisson vector: [D | Phi] (Poisson.cpp:18-69); D uses a nodal RT basis
  *   (face DoFs: vertical faces j*(NX+1)+i then horizontal faces j*NX+i, k DoFs
- *   each; then 2(k-1)k interior DoFs per cell), Phi is FE_DGQ(k-1) per cell.
+ *   each; then 2(k-1)k interior DoFs per cell), Phi is FE_DGQ(k-1) per cell;
+ *   pecs_set_dof_layout switches the host-pointer calls to deal.II's basis
+ *   and numbering of D.
  * Host-pointer functions copy in and out; *_dev functions take device
  * pointers and do not synchronise.
  */
@@ -76,6 +78,16 @@ typedef struct {
 enum { PECS_ELECTRONS = 0, PECS_HOLES = 1, PECS_REDUCTANTS = 2, PECS_OXIDANTS = 3, PECS_POISSON = 4, PECS_MASS_SEMI = 5,
        PECS_MASS_ELEC = 6 };
 
+/* Layout of the Poisson vector [D | Phi] in the host-pointer calls.
+ * PECS_LAYOUT_OWN: this library's nodal RT basis for D (the layout above).
+ * PECS_LAYOUT_DEALII: the reference's own -- FE_RaviartThomas(k-1) node
+ * functionals (Legendre moments of the normal component on each face,
+ * interior moments) with deal.II's DoF distribution and component-wise
+ * renumbering (SolarCell.cpp:23-24, Poisson.cpp:18-30), so a deal.II caller
+ * passes Poisson_object.solution and .system_rhs unchanged.  Phi (FE_DGQ(k-1))
+ * is the same in both. */
+enum { PECS_LAYOUT_OWN = 0, PECS_LAYOUT_DEALII = 1 };
+
 /* Parameters::parse_and_scale_parameters, include/Parameters.hpp:117-287. */
 int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out);
 
@@ -97,6 +109,21 @@ const char* pecs_last_error(pecs_ctx* ctx);
 /* Error text of the last failed pecs_create / stateless call on this thread. */
 const char* pecs_global_error(void);
 int pecs_set_stream(pecs_ctx* ctx, void* hip_stream);
+/* Poisson layout of the host-pointer calls from now on: the Poisson vectors
+ * of pecs_set_state / pecs_get_state (and _batch), the Poisson input of
+ * pecs_rhs_semiconductor / pecs_rhs_electrolyte, the output of
+ * pecs_rhs_poisson and both vectors of pecs_solve(PECS_POISSON).  Solutions
+ * convert with the change of basis per cell, right-hand sides with its
+ * inverse transpose.  Device-pointer (_dev) calls, pecs_write_vtu and
+ * pecs_export_matrix keep PECS_LAYOUT_OWN.  The role of
+ * PoissonData::setup_dofs's numbering (Poisson.cpp:18-30). */
+int pecs_set_dof_layout(pecs_ctx* ctx, int layout);
+/* Stateless conversion of one Poisson vector between the two layouts
+ * (to_dealii: PECS_LAYOUT_OWN -> PECS_LAYOUT_DEALII, else back; rhs: the
+ * vector is a right-hand side rather than a solution).  n_poisson doubles in
+ * and out (pecs_sizes).  Host only; errors: pecs_global_error(). */
+int pecs_poisson_layout_convert(const pecs_mesh* mesh, const pecs_params* params, int to_dealii, int rhs,
+                                const double* in, double* out);
 int pecs_sizes(pecs_ctx* ctx, int64_t* n_semi, int64_t* n_elec, int64_t* n_poisson);
 /* Setup statistics, up to 12 values: factor sizes in doubles (semiconductor,
  * electrolyte, multiplier), setup seconds, factorisation seconds (symbolic +

@@ -25,6 +25,10 @@ What it restates (reference file:line):
                     BiasValues.cpp, SolarCell.hpp:86-98), the time loop
                     (SolarCell.cpp:1898-2095) with sparse direct solves in
                     the role of UMFPACK (Carrier.cpp:26-40, Poisson.cpp:90-105).
+  * dealii_rt.py  -- deal.II's FE_RaviartThomas(k-1) node functionals, dual
+                    basis and distribute_dofs / component_wise numbering
+                    (SolarCell.cpp:23-24, Poisson.cpp:18-30), to check the
+                    library's conversion to the reference's Poisson layout.
   * manufactured.py -- the four convergence studies
                     (test_functions.cpp, SolarCell.cpp:2105-3278).
 

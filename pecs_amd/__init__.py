@@ -196,6 +196,25 @@ def export_matrix(prm, which):
     return (A, cons.astype(bool)) if which == POISSON else A
 
 
+LAYOUT_OWN, LAYOUT_DEALII = 0, 1
+
+
+def poisson_layout_convert(prm, v, to_dealii=True, rhs=False):
+    """One Poisson vector between this library's layout and deal.II's
+    (FE_RaviartThomas node functionals, distribute_dofs + component_wise
+    numbering); rhs=True for a right-hand side (dual vector)."""
+    L = lib()
+    L.pecs_poisson_layout_convert.argtypes = [ctypes.POINTER(CMesh), ctypes.POINTER(CParams), ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double)]
+    m, p = to_c(prm)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros_like(v)
+    _check(L.pecs_poisson_layout_convert(ctypes.byref(m), ctypes.byref(p), int(bool(to_dealii)), int(bool(rhs)),
+                                         _ptr(v), _ptr(out)))
+    return out
+
+
 def write_vtu(prm, state, step, out_dir=".", char_length=1.0, char_density=1.0, char_time=1.0):
     """print_results: Semiconductor-NNN.vtu, Electrolyte-NNN.vtu, Poisson-NNN.vtu
     in out_dir from a state dict (keys n, p, r, o, P); host only."""
@@ -283,6 +302,12 @@ class SolarCellGPU:
         L = lib()
         L.pecs_set_mixed_precision.argtypes = [ctypes.c_void_p, ctypes.c_int]
         self._c(L.pecs_set_mixed_precision(self.ctx, 1 if enable else 0))
+
+    def set_dof_layout(self, layout):
+        """Poisson layout of the host-pointer calls: LAYOUT_OWN or LAYOUT_DEALII."""
+        L = lib()
+        L.pecs_set_dof_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        self._c(L.pecs_set_dof_layout(self.ctx, int(layout)))
 
     def uniform_lists(self):
         """Bit flags of the cell-local kernels that take wave-uniform cell

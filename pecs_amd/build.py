@@ -19,7 +19,7 @@ HOST = os.path.join(HERE, "host")
 RUNNER = os.path.join(BUILD, "pecs_run")
 HOST_TEST = os.path.join(BUILD, "pecs_host_test")
 
-HOST_SRC = ["mesh.cpp", "ops.cpp", "cholesky.cpp", "vtu.cpp"]
+HOST_SRC = ["mesh.cpp", "ops.cpp", "cholesky.cpp", "vtu.cpp", "dealii_layout.cpp"]
 HIP_SRC = ["kernels.hip", "devfactor.hip", "mlp.hip", "abi.hip"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-mavx2", "-mfma"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -18,6 +18,7 @@
 
 #include "../../include/pecs_abi.h"
 #include "cholesky.hpp"
+#include "dealii_layout.hpp"
 #include "devfactor.hpp"
 #include "fe.hpp"
 #include "kernels.hpp"
@@ -109,6 +110,10 @@ struct pecs_ctx {
   double setup_seconds = 0.0, factor_seconds = 0.0;
   std::vector<void*> allocs;
   void* blas = nullptr;  // dense-library handle of the device factorisations (created at the first)
+  // layout of the Poisson vectors of the host-pointer calls (PECS_LAYOUT_*)
+  int layout = 0;
+  std::unique_ptr<DealiiPoisson> dii;
+  std::vector<double> ptmp;  // host conversion buffer
   // optional per-phase timing of pecs_step (hipEvents recorded between phases)
   int timing = 0;
   std::vector<hipEvent_t> events;  // NPHASE+1 per recorded step
@@ -253,6 +258,27 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
       }
       rebuild_tiles(P);
     }
+    return;
+  }
+  // PECS_SWEEP_FORCE=m,rows,ks gives every level of a batched (MFMA-shaped)
+  // factor one shape of the MFMA sweep instead of the tuned ones -- m 16-row
+  // blocks per wave, rows per tile, column segments of ks (0: unsplit) -- so
+  // that tests can run the shapes the tuner picks only on large meshes.
+  if (const char* fs = std::getenv("PECS_SWEEP_FORCE"); fs && nrhs > 2) {
+    const int w = 0;
+    int m = 0, rows = 0, ks = -1;
+    if (std::sscanf(fs, "%d,%d,%d", &m, &rows, &ks) != 3 || (m != 1 && m != 2 && m != 4) || rows < 16 ||
+        rows > 64 * m || rows % 16 != 0 || ks < 0 || (ks > 0 && (ks < 16 || ks % 16 != 0)))
+      throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks (m 1, 2 or 4, rows a multiple of 16 up to "
+                               "64 m, ks 0 or a multiple of 16)");
+    for (int l = 0; l < P.nlevels; ++l) {
+      P.fw[l] = P.bw[l] = w;
+      P.fm[l] = P.bm[l] = m;
+      P.frows[l] = P.brows[l] = rows;
+    }
+    P.fks.assign(P.nlevels, ks);
+    P.bks.assign(P.nlevels, ks);
+    rebuild_tiles(P);
     return;
   }
   const size_t scrub_bytes = (size_t)512 << 20;
@@ -1208,6 +1234,44 @@ int pecs_set_stream(pecs_ctx* ctx, void* s) {
   return 0;
 }
 
+int pecs_set_dof_layout(pecs_ctx* ctx, int layout) {
+  if (!ctx) return -1;
+  try {
+    if (layout != PECS_LAYOUT_OWN && layout != PECS_LAYOUT_DEALII)
+      throw std::runtime_error("unknown DoF layout (PECS_LAYOUT_OWN or PECS_LAYOUT_DEALII)");
+    if (layout == PECS_LAYOUT_DEALII && !ctx->dii)
+      ctx->dii = std::make_unique<DealiiPoisson>(dealii_poisson(ctx->mesh, ctx->L, *ctx->T));
+    ctx->layout = layout;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pecs_poisson_layout_convert(const pecs_mesh* mesh, const pecs_params* params, int to_dealii, int rhs,
+                                const double* in, double* out) {
+  try {
+    if (!mesh || !params || !in || !out) throw std::runtime_error("null argument");
+    validate(mesh, params);
+    const SolarCellMesh m = make_solarcell_mesh(mesh->n_refine, to_geo(mesh));
+    const RefTables T(params->degree);
+    const PoissonLayout L = poisson_layout(m, params->degree - 1);
+    const DealiiPoisson X = dealii_poisson(m, L, T);
+    if (to_dealii) {
+      if (rhs) poisson_rhs_to_dealii(X, in, out);
+      else poisson_to_dealii(X, in, out);
+    } else {
+      if (rhs) poisson_rhs_from_dealii(X, in, out);
+      else poisson_from_dealii(X, in, out);
+    }
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+  return 0;
+}
+
 int pecs_sizes(pecs_ctx* ctx, int64_t* ns, int64_t* ne, int64_t* np) {
   if (!ctx) return -1;
   if (ns) *ns = ctx->Ns;
@@ -1303,6 +1367,26 @@ int pecs_export_matrix(const pecs_mesh* mesh, const pecs_params* params, int whi
     return fail(ctx, e);                        \
   }                                             \
   return 0;
+
+// A host Poisson vector handed in by the caller, in this library's layout
+// (converted from deal.II's when the context uses it).  rhs: a right-hand
+// side (dual vector) rather than a coefficient vector.
+static const double* poisson_in(pecs_ctx* c, const double* v, bool rhs) {
+  if (c->layout != PECS_LAYOUT_DEALII || !v) return v;
+  c->ptmp.resize(c->NP);
+  if (rhs) poisson_rhs_from_dealii(*c->dii, v, c->ptmp.data());
+  else poisson_from_dealii(*c->dii, v, c->ptmp.data());
+  return c->ptmp.data();
+}
+
+// A host Poisson vector about to be returned, converted in place to the
+// caller's layout.
+static void poisson_out(pecs_ctx* c, double* v, bool rhs) {
+  if (c->layout != PECS_LAYOUT_DEALII || !v) return;
+  c->ptmp.assign(v, v + c->NP);
+  if (rhs) poisson_rhs_to_dealii(*c->dii, c->ptmp.data(), v);
+  else poisson_to_dealii(*c->dii, c->ptmp.data(), v);
+}
 
 static void h2d(pecs_ctx* c, double* d, const double* h, int64_t n) {
   HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1425,7 +1509,7 @@ int pecs_rhs_semiconductor(pecs_ctx* ctx, const double* un, const double* up, co
     h2d(ctx, ctx->io[1], up, ctx->Ns);
     h2d(ctx, ctx->io[2], ur, ctx->Ne);
     h2d(ctx, ctx->io[3], uo, ctx->Ne);
-    h2d(ctx, ctx->old[4], pv, ctx->NP);
+    h2d(ctx, ctx->old[4], poisson_in(ctx, pv, false), ctx->NP);
     HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, true, ctx->Rs, ctx->Re, ctx->Pd, ctx->tabs, ctx->phs_d, ctx->io[0],
                                             ctx->io[1], ctx->io[2], ctx->io[3], ctx->old[4], ctx->b[0], ctx->b[1],
                                             ctx->stream));
@@ -1441,7 +1525,7 @@ int pecs_rhs_electrolyte(pecs_ctx* ctx, const double* ur, const double* uo, cons
     h2d(ctx, ctx->io[1], up, ctx->Ns);
     h2d(ctx, ctx->io[2], ur, ctx->Ne);
     h2d(ctx, ctx->io[3], uo, ctx->Ne);
-    h2d(ctx, ctx->old[4], pv, ctx->NP);
+    h2d(ctx, ctx->old[4], poisson_in(ctx, pv, false), ctx->NP);
     HIPCHECK((hipError_t)launch_rhs_carrier(ctx->k, false, ctx->Re, ctx->Rs, ctx->Pd, ctx->tabs, ctx->phs_d, ctx->io[2],
                                             ctx->io[3], ctx->io[0], ctx->io[1], ctx->old[4], ctx->b[2], ctx->b[3],
                                             ctx->stream));
@@ -1462,6 +1546,7 @@ int pecs_rhs_poisson(pecs_ctx* ctx, const double* un, const double* up, const do
                                                      ctx->io[2], ctx->io[3], ctx->mesh.semi.ncell,
                                                      ctx->mesh.elec.ncell, ctx->old[4], ctx->stream));
     d2h(ctx, rhs, ctx->old[4], ctx->NP);
+    poisson_out(ctx, rhs, true);
   })
 }
 
@@ -1472,7 +1557,7 @@ int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x) {
     else if (which == PECS_REDUCTANTS || which == PECS_OXIDANTS) n = ctx->Ne;
     else if (which == PECS_POISSON) n = ctx->NP;
     else throw std::runtime_error("unknown system id");
-    h2d(ctx, ctx->io[0], rhs, n);
+    h2d(ctx, ctx->io[0], which == PECS_POISSON ? poisson_in(ctx, rhs, true) : rhs, n);
     if (which == PECS_POISSON) {
       HIPCHECK((hipError_t)launch_poisson_cell_from_b(ctx->Pd, ctx->io[0], ctx->yP, ctx->stream));
       poisson_solve_from_y(ctx, 0, 1, ctx->io[1]);
@@ -1480,6 +1565,7 @@ int pecs_solve(pecs_ctx* ctx, int which, const double* rhs, double* x) {
       solve_carrier(ctx, which, ctx->io[0], ctx->io[1]);
     }
     d2h(ctx, x, ctx->io[1], n);
+    if (which == PECS_POISSON) poisson_out(ctx, x, false);
   })
 }
 
@@ -1493,7 +1579,7 @@ int pecs_set_state_batch(pecs_ctx* ctx, int i, const double* un, const double* u
     h2d(ctx, ctx->u[2] + i * Ne, ur, Ne);
     h2d(ctx, ctx->u[3] + i * Ne, uo, Ne);
     if (pv) {
-      h2d(ctx, ctx->pv + i * NP, pv, NP);
+      h2d(ctx, ctx->pv + i * NP, poisson_in(ctx, pv, false), NP);
     } else {
       poisson_cells(ctx, i, ctx->u[0] + i * Ns, ctx->u[1] + i * Ns, ctx->u[2] + i * Ne, ctx->u[3] + i * Ne);
       poisson_solve_from_y(ctx, i, 1, ctx->pv + i * NP);
@@ -1515,7 +1601,10 @@ int pecs_get_state_batch(pecs_ctx* ctx, int i, double* un, double* up, double* u
     if (up) d2h(ctx, up, ctx->u[1] + i * Ns, Ns);
     if (ur) d2h(ctx, ur, ctx->u[2] + i * Ne, Ne);
     if (uo) d2h(ctx, uo, ctx->u[3] + i * Ne, Ne);
-    if (pv) d2h(ctx, pv, ctx->pv + i * NP, NP);
+    if (pv) {
+      d2h(ctx, pv, ctx->pv + i * NP, NP);
+      poisson_out(ctx, pv, false);
+    }
   })
 }
 

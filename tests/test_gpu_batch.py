@@ -85,3 +85,54 @@ def test_batch_rejects_mismatched_matrix_parameters(seed):
     prms[1]["mu_n"] = 2.0 * prms[0]["mu_n"]
     with pytest.raises(RuntimeError, match="enters the matrices"):
         pecs_amd.SolarCellGPU(prms)
+
+
+# PECS_SWEEP_FORCE=m,rows,ks: one MFMA sweep shape on every level (m blocks of
+# 16 rows per wave; rows per tile; column segments of ks, 0 unsplit), so the
+# shapes the tuner picks only on large meshes (several blocks per wave, split
+# tiles and their reduction) run here against the oracle.  Split tiles add
+# their segments in a fixed order: they agree with unsplit ones to rounding.
+SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,128,256", "1,32,512"]
+
+
+@pytest.mark.parametrize("B,r", [(3, 4), (9, 3)])
+def test_forced_sweep_shapes(B, r, seed):
+    import os
+    import pecs_amd
+    prms = _batch(B, seed + 7, r=r)
+    refs = [OracleSolarCell(p).run(STEPS) for p in prms]
+    out = {}
+    for shape in SHAPES:
+        _say(f"[gpu] forced MFMA sweep shape {shape}: B={B} r={r} k=2, {STEPS} steps")
+        os.environ["PECS_SWEEP_FORCE"] = shape
+        try:
+            g = pecs_amd.SolarCellGPU(prms)
+        finally:
+            del os.environ["PECS_SWEEP_FORCE"]
+        m, rows = int(shape.split(",")[0]), int(shape.split(",")[1])
+        cfg = g.sweep_config(0)
+        assert all(t[0] == 0 and t[1] == m and t[2] == rows and t[3] == 0 and t[4] == m and t[5] == rows
+                   for t in cfg), cfg
+        g.initial_state()
+        g.step(STEPS)
+        out[shape] = [g.get_state(i) for i in range(B)]
+        g.close()
+        for i in range(B):
+            for c in "nproP":
+                assert _rel(out[shape][i][c], refs[i][c]) < 1e-10, (shape, i, c)
+    for shape in SHAPES[1:]:
+        for i in range(B):
+            for c in "nproP":
+                assert _rel(out[shape][i][c], out[SHAPES[0]][i][c]) < 1e-12, (shape, i, c)
+
+
+def test_forced_sweep_shape_rejected():
+    import os
+    import pecs_amd
+    prms = _batch(3, 5)
+    os.environ["PECS_SWEEP_FORCE"] = "3,64,0"
+    try:
+        with pytest.raises(RuntimeError, match="PECS_SWEEP_FORCE"):
+            pecs_amd.SolarCellGPU(prms)
+    finally:
+        del os.environ["PECS_SWEEP_FORCE"]
