@@ -266,6 +266,36 @@ int pecs_uniform_lists(pecs_ctx* ctx, int* flags);
  * counterpart in the reference (FP64 throughout); it replaces nothing. */
 int pecs_set_mixed_precision(pecs_ctx* ctx, int enable);
 
+/* Manufactured-solution data for the right-hand sides (test hooks).  target
+ * 0 / 1: the carrier right-hand sides of the semiconductor / electrolyte
+ * (both carriers of the region take the same data); src (ncell x nq) is
+ * added to the source at the cell quadrature points, dir (ncell x 4 x nfq)
+ * replaces the equilibrium constant as the Dirichlet datum at the face
+ * points of Dirichlet faces, flux (ncell x 4 x nfq) is a prescribed normal
+ * flux h on interface and Neumann faces, entering as -<v, h> (on top of the
+ * interface terms, which vanish with k_et = k_ht = 0).  target 2: the Poisson
+ * right-hand side; src is added to the charge at the cell quadrature points,
+ * dir is the Dirichlet datum g of -<psi.n, g> on Dirichlet faces (on top of
+ * the bias data, which vanishes with phi_bi = phi_app = phi_sch = 0); flux
+ * must be NULL.  A NULL array switches that term off.  Every instance of a
+ * batch takes the same data.  The arrays are copied; new contents take effect
+ * at the next right-hand side or step.  Cells are numbered as in the region
+ * vectors (Poisson: semiconductor cells, then electrolyte cells), faces in
+ * deal.II order, points as pecs_quadrature_points returns them.  Not
+ * combinable with pecs_set_mixed_precision.  Replaces the reference's test
+ * assemblies of its manufactured studies: LDG.cpp:681-982
+ * (assemble_local_test_rhs, assemble_local_test_transient_rhs),
+ * MixedFEM.cpp:166-254, SolarCell.cpp:2105-2745 (test_DD_Poisson and
+ * interface assemblies), test_functions.cpp:13-527 (the data). */
+int pecs_set_manufactured(pecs_ctx* ctx, int target, const double* src, const double* dir, const double* flux);
+
+/* Physical coordinates of the points pecs_set_manufactured takes data at:
+ * cell_xy (ncell x nq x 2), face_xy (ncell x 4 x nfq x 2); any pointer may be
+ * NULL (sizes only).  No counterpart in the reference (deal.II's FEValues
+ * quadrature_point()). */
+int pecs_quadrature_points(pecs_ctx* ctx, int target, int64_t* ncell, int* nq, int* nfq, double* cell_xy,
+                           double* face_xy);
+
 /* Device-pointer variants (no copies, no synchronisation). */
 int pecs_rhs_semiconductor_dev(pecs_ctx* ctx, const double* u_n, const double* u_p, const double* poisson,
                                const double* u_r, const double* u_o, double* rhs_n, double* rhs_p);

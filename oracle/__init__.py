@@ -30,7 +30,10 @@ What it restates (reference file:line):
                     (SolarCell.cpp:23-24, Poisson.cpp:18-30), to check the
                     library's conversion to the reference's Poisson layout.
   * manufactured.py -- the four convergence studies
-                    (test_functions.cpp, SolarCell.cpp:2105-3278).
+                    (test_functions.cpp, SolarCell.cpp:2105-3278), on the
+                    reference's unit-square grids and on the solar-cell grid
+                    the GPU path runs them on (tests/test_gpu_manufactured.py
+                    checks the GPU against the latter).
 
 Parity pins (what fixes each function without the reference, which cannot be
 built here: it needs deal.II 8.x + UMFPACK + TBB):

@@ -15,6 +15,15 @@ the errors so tests can assert the textbook rates (LDG u: k+1, mixed Phi and
 D: k).  The transient studies project 1 + cos(2 pi x) cos(2 pi y) as initial
 data (test_functions.cpp:429-448), which is not the exact u(0); `ic='exact'`
 uses the exact initial value instead, `ic='reference'` reproduces the quirk.
+
+geometry='unit' is the reference's grids (unit squares with the boundary
+ids of Grid.cpp:464-598).  geometry='cell' poses the same exact solutions on
+the solar-cell grid the library's GPU path is built for (r1 = r2 = 0.5,
+H = L = 1, outer faces Dirichlet): the carriers live on the semiconductor
+half [0, 0.5] x [0, 1], whose face x = 0.5 (the interface) takes the exact
+normal flux J.n as a natural boundary condition, and the mixed Poisson
+problem covers the merged unit square with Dirichlet data on all of its
+boundary.  The interface study is on that grid in the reference already.
 """
 import numpy as np
 import scipy.sparse.linalg as spla
@@ -57,6 +66,32 @@ def tran_grid_classify(cx, cy):  # Grid.cpp:531-568
 
 def dd_grid_classify(cx, cy):  # Grid.cpp:570-598
     return DIRICHLET
+
+
+CELL_GEO = dict(r1=0.5, r2=0.5, H=1.0, L=1.0)
+
+
+def study_mesh(r, geometry, classify):
+    """(carrier region, Poisson mesh) of a study on `geometry`."""
+    if geometry == "unit":
+        reg = unit_square(r, classify)
+        return reg, _PMesh(reg)
+    if geometry == "cell":
+        from .mesh import SolarCellMesh
+        m = SolarCellMesh(r, CELL_GEO)
+        return m.semi, m
+    raise ValueError(geometry)
+
+
+def _flux_rhs(reg, cfe, loc, h):
+    """-<v, h(x, n)> on interface faces: a prescribed normal flux J.n."""
+    for f in range(4):
+        cells = np.nonzero(reg.bid[:, f] == INTERFACE)[0]
+        if len(cells) == 0:
+            continue
+        _, VF = cfe.face_parts(f)
+        xf, nrm, ln = fe.face_geometry(reg.V[cells], f, cfe.fs)
+        loc[cells] += np.einsum("iq,cq->ci", VF, -h(xf, nrm) * cfe.fw[None] * ln)
 
 
 def _cell_rhs(reg, cfe, f_vals=None, p_field=None, u_old=None):
@@ -172,20 +207,29 @@ def poisson_D(x):
     return np.stack([1 + TP * np.cos(TP * x[..., 0]), TP * np.sin(TP * x[..., 1])], -1)
 
 
-def steady_state(k, r):
-    """test_steady_state: mixed Poisson and steady LDG (theta = 0) errors."""
-    reg = unit_square(r, test_grid_classify)
-    pm = _PMesh(reg)
+def _dot_n(vec):
+    return lambda x, n: np.sum(vec(x) * n, -1)
+
+
+def steady_state(k, r, geometry="unit", out=None):
+    """test_steady_state: mixed Poisson and steady LDG (theta = 0) errors.
+    `out` (a dict) receives the right-hand sides and solutions."""
+    reg, pm = study_mesh(r, geometry, test_grid_classify)
     pfe = PoissonFE(k, pm)
-    sol = _mixed_solve(pm, pfe, _mixed_rhs(pm, pfe, poisson_f, poisson_g))
+    bP = _mixed_rhs(pm, pfe, poisson_f, poisson_g)
+    sol = _mixed_solve(pm, pfe, bP)
     ephi, eD = mixed_errors(pm, pfe, sol, poisson_g, poisson_D)
     cfe = CarrierFE(k)
     A, _, _ = assemble_ldg(reg, cfe, 1.0, 1.0, 1.0, 0.0, 1.0)
     loc, _ = _cell_rhs(reg, cfe, f_vals=poisson_f)
     _dirichlet_rhs(reg, cfe, loc, poisson_g)
-    u = spla.spsolve(A.tocsc(), _assemble_vec(reg, cfe, loc))
+    _flux_rhs(reg, cfe, loc, _dot_n(poisson_D))  # J = -grad u = D (interface faces of 'cell' only)
+    b = _assemble_vec(reg, cfe, loc)
+    u = spla.spsolve(A.tocsc(), b)
     eu, eJ = ldg_errors(reg, cfe, u, poisson_g, lambda x: -(-poisson_D(x)))
     h = fe.diameter(reg.V).max()
+    if out is not None:
+        out.update(bP=bP, P=sol, b=b, u=u)
     return dict(h=h, Phi=ephi, D=eD, u=eu, J=eJ)
 
 
@@ -210,9 +254,20 @@ def project_density(reg, cfe, func):
     return u
 
 
-def transient(k, r, ic="exact"):
+def transient_J(x, t):
+    """Exact current -grad u - E u of the transient study, E = (1, 0)."""
+    return np.stack([TP * np.sin(TP * x[..., 0]) - _u_exact(x, t), TP * np.sin(TP * x[..., 1])], -1)
+
+
+def transient_f(x, t):
+    """Source of the transient study (test_functions.cpp:148-164)."""
+    return -np.exp(-t) + TP * TP * np.cos(TP * x[..., 0]) + TP * TP * np.cos(TP * x[..., 1]) \
+        + TP * np.sin(TP * x[..., 0])
+
+
+def transient(k, r, ic="exact", geometry="unit"):
     """test_transient: u_t + div(-grad u - E u) = f, E = (1,0), dt = h^(k+1), T = 1."""
-    reg = unit_square(r, tran_grid_classify)
+    reg, _ = study_mesh(r, geometry, tran_grid_classify)
     cfe = CarrierFE(k)
     h = fe.diameter(reg.V).max()
     dt = h ** (k + 1)
@@ -222,36 +277,48 @@ def transient(k, r, ic="exact"):
     Efield = np.zeros((reg.ncell, cfe.qpts.shape[0], 2))
     Efield[..., 0] = 1.0
     t = 0.0
+    f = transient_f
 
-    def f(x, t):
-        return -np.exp(-t) + TP * TP * np.cos(TP * x[..., 0]) + TP * TP * np.cos(TP * x[..., 1]) \
-            + TP * np.sin(TP * x[..., 0])
-
-    def gI(x, t):
-        return -np.exp(-t) - np.cos(TP * x[..., 1]) - 1.0
+    def gI(x, n, t):
+        if geometry == "unit":  # x = 1 (test_functions.cpp:184-196)
+            return -np.exp(-t) - np.cos(TP * x[..., 1]) - 1.0
+        return np.sum(transient_J(x, t) * n, -1)
 
     while t < 1.0:
         loc, _ = _cell_rhs(reg, cfe, f_vals=lambda x: f(x, t), p_field=Efield, u_old=u)
         _dirichlet_rhs(reg, cfe, loc, lambda x: _u_exact(x, t))
-        for fc in range(4):
-            cells = np.nonzero(reg.bid[:, fc] == INTERFACE)[0]
-            if len(cells) == 0:
-                continue
-            PF, VF = cfe.face_parts(fc)
-            xf, nrm, ln = fe.face_geometry(reg.V[cells], fc, cfe.fs)
-            loc[cells] += np.einsum("iq,cq->ci", VF, -gI(xf, t) * cfe.fw[None] * ln)
+        _flux_rhs(reg, cfe, loc, lambda x, n: gI(x, n, t))
         u = lu.solve(M @ u + _assemble_vec(reg, cfe, loc))
         t += dt
-    eu, eJ = ldg_errors(reg, cfe, u, lambda x: _u_exact(x, t),
-                        lambda x: np.stack([TP * np.sin(TP * x[..., 0]) - _u_exact(x, t),
-                                            TP * np.sin(TP * x[..., 1])], -1))
-    return dict(h=h, u=eu, J=eJ, t=t)
+    eu, eJ = ldg_errors(reg, cfe, u, lambda x: _u_exact(x, t), lambda x: transient_J(x, t))
+    return dict(h=h, u=eu, J=eJ, t=t, dt=dt, u_h=u)
 
 
-def dd_poisson(k, r, ic="exact"):
+def dd_J(x, t):
+    """Exact current -grad u - D u of the DD-Poisson study."""
+    ut = _u_exact(x, t)
+    return np.stack([TP * np.sin(TP * x[..., 0]) - (TP * np.cos(TP * x[..., 0]) + 1) * ut,
+                     TP * np.sin(TP * x[..., 1]) - TP * np.sin(TP * x[..., 1]) * ut], -1)
+
+
+def dd_poisson_f(x, t):
+    """Poisson source of the DD-Poisson study (test_functions.cpp:300-314)."""
+    return poisson_f(x) + _u_exact(x, t)
+
+
+def dd_f(x, t):
+    """Drift-diffusion source of the DD-Poisson study (test_functions.cpp:276-298)."""
+    X, Y = x[..., 0], x[..., 1]
+    ut = _u_exact(x, t)
+    div_E_u = TP * TP * (np.cos(TP * Y) - np.sin(TP * X)) * ut \
+        - TP * (TP * np.cos(TP * X) + 1) * np.sin(TP * X) - TP * TP * np.sin(TP * Y) ** 2
+    return -np.exp(-t) + TP * TP * np.cos(TP * X) + TP * TP * np.cos(TP * Y) - div_E_u
+
+
+def dd_poisson(k, r, ic="exact", geometry="unit"):
     """test_DD_Poisson: mixed Poisson coupled to drift-diffusion via D, all Dirichlet."""
-    reg = unit_square(r, dd_grid_classify)
-    pm = _PMesh(reg)
+    reg, pm = study_mesh(r, geometry, dd_grid_classify)
+    ncs = reg.ncell  # the carrier cells come first in the Poisson mesh
     pfe = PoissonFE(k, pm)
     cfe = CarrierFE(k)
     h = fe.diameter(reg.V).max()
@@ -264,40 +331,46 @@ def dd_poisson(k, r, ic="exact"):
     from .ldg import field_at_carrier_q
     t = 0.0
 
-    def fP(x, t):
-        return poisson_f(x) + _u_exact(x, t)
-
-    def fDD(x, t):
-        X, Y = x[..., 0], x[..., 1]
-        ut = _u_exact(x, t)
-        div_E_u = TP * TP * (np.cos(TP * Y) - np.sin(TP * X)) * ut \
-            - TP * (TP * np.cos(TP * X) + 1) * np.sin(TP * X) - TP * TP * np.sin(TP * Y) ** 2
-        return -np.exp(-t) + TP * TP * np.cos(TP * X) + TP * TP * np.cos(TP * Y) - div_E_u
-
+    fP, fDD = dd_poisson_f, dd_f
     sol = None
     while t < 1.0:
         # Poisson with the old density (SolarCell.cpp:2105-2185)
         x, JxW, psi, div = pfe.cell_parts(pm.pV)
         ud = density_at(cfe, u, reg.ncell, cfe.phi)
-        bP = _mixed_rhs(pm, pfe, lambda xx: fP(xx, t), poisson_g)
-        loc = np.zeros((reg.ncell, pfe.nloc))
-        loc[:, pfe.nrt:] = np.einsum("jq,cq->cj", pfe.phi_q, ud * JxW)
-        np.add.at(bP, pfe.idx, loc)
+        # the data carries u only where the density lives (the whole square
+        # on 'unit', the semiconductor half on 'cell')
+        bP = _mixed_rhs(pm, pfe, lambda xx: fP(xx, t) if geometry == "unit" else poisson_f(xx), poisson_g)
+        loc = np.zeros((ncs, pfe.nloc))
+        if geometry != "unit":
+            loc[:, pfe.nrt:] = -np.einsum("jq,cq->cj", pfe.phi_q, _u_exact(x[:ncs], t) * JxW[:ncs])
+        loc[:, pfe.nrt:] += np.einsum("jq,cq->cj", pfe.phi_q, ud * JxW[:ncs])
+        np.add.at(bP, pfe.idx[:ncs], loc)
         sol = luP.solve(bP)
-        Ef = field_at_carrier_q(cfe, pfe, reg.V, sol[pfe.idx])
+        Ef = field_at_carrier_q(cfe, pfe, reg.V, sol[pfe.idx[:ncs]])
         loc, _ = _cell_rhs(reg, cfe, f_vals=lambda xx: fDD(xx, t), p_field=Ef, u_old=u)
         _dirichlet_rhs(reg, cfe, loc, lambda xx: _u_exact(xx, t))
+        _flux_rhs(reg, cfe, loc, lambda xx, n: np.sum(dd_J(xx, t) * n, -1))
         u = lu.solve(M @ u + _assemble_vec(reg, cfe, loc))
         t += dt
 
-    def J_ex(x):
-        ut = _u_exact(x, t)
-        return np.stack([TP * np.sin(TP * x[..., 0]) - (TP * np.cos(TP * x[..., 0]) + 1) * ut,
-                         TP * np.sin(TP * x[..., 1]) - TP * np.sin(TP * x[..., 1]) * ut], -1)
-
-    eu, eJ = ldg_errors(reg, cfe, u, lambda x: _u_exact(x, t), J_ex)
+    eu, eJ = ldg_errors(reg, cfe, u, lambda x: _u_exact(x, t), lambda x: dd_J(x, t))
     ephi, eD = mixed_errors(pm, pfe, sol, poisson_g, poisson_D)
-    return dict(h=h, u=eu, J=eJ, Phi=ephi, D=eD, t=t)
+    return dict(h=h, u=eu, J=eJ, Phi=ephi, D=eD, t=t, dt=dt, u_h=u, P=sol)
+
+
+def interface_f(x, t):
+    """Source of the interface study (test_functions.cpp:450-464)."""
+    return -np.exp(-t) + TP * TP * np.cos(TP * x[..., 0]) + TP * TP * np.cos(TP * x[..., 1])
+
+
+def interface_gI(x, t):
+    """Interface datum I = u^2 (test_functions.cpp:484-505)."""
+    return _u_exact(x, t) ** 2
+
+
+def u_exact(x, t):
+    """exp(-t) + cos(2 pi x) + cos(2 pi y), the transient studies' solution."""
+    return _u_exact(x, t)
 
 
 def interface(k, r, ic="exact"):
@@ -315,12 +388,7 @@ def interface(k, r, ic="exact"):
     us = project_density(m.semi, cfe, _ic(ic))
     ue = project_density(m.elec, cfe, _ic(ic))
 
-    def f(x, t):
-        return -np.exp(-t) + TP * TP * np.cos(TP * x[..., 0]) + TP * TP * np.cos(TP * x[..., 1])
-
-    def gI(x, t):
-        return _u_exact(x, t) ** 2
-
+    f, gI = interface_f, interface_gI
     t = 0.0
     while t < 1.0:
         new = []
@@ -346,4 +414,4 @@ def interface(k, r, ic="exact"):
     J_ex = lambda x: np.stack([TP * np.sin(TP * x[..., 0]), TP * np.sin(TP * x[..., 1])], -1)
     e1 = ldg_errors(m.semi, cfe, us, lambda x: _u_exact(x, t), J_ex)
     e2 = ldg_errors(m.elec, cfe, ue, lambda x: _u_exact(x, t), J_ex)
-    return dict(h=h, u1=e1[0], J1=e1[1], u2=e2[0], J2=e2[1], t=t)
+    return dict(h=h, u1=e1[0], J1=e1[1], u2=e2[0], J2=e2[1], t=t, dt=dt, us=us, ue=ue)

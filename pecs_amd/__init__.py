@@ -24,7 +24,7 @@ ABI_FUNCTIONS = [
     "pecs_initial_state", "pecs_step", "pecs_last_residual", "pecs_rhs_semiconductor_dev",
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config", "pecs_uniform_lists",
-    "pecs_set_mixed_precision",
+    "pecs_set_mixed_precision", "pecs_set_manufactured", "pecs_quadrature_points",
     "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
     "pecs_last_residual_batch", "pecs_write_vtu", "pecs_use_graph", "pecs_sweep_create", "pecs_sweep_set_active",
     "pecs_sweep_step", "pecs_sweep_destroy", "pecs_mlp_create", "pecs_mlp_destroy", "pecs_mlp_num_params",
@@ -308,6 +308,45 @@ class SolarCellGPU:
         L = lib()
         L.pecs_set_dof_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
         self._c(L.pecs_set_dof_layout(self.ctx, int(layout)))
+
+    def set_manufactured(self, target, src=None, dir=None, flux=None):
+        """Manufactured-solution data of the right-hand sides (test hooks):
+        target 0 / 1 the semiconductor / electrolyte carriers (src at the
+        cell points, dir the Dirichlet datum, flux a normal flux on
+        interface and Neumann faces), 2 the Poisson right-hand side (src
+        added to the charge, dir the Dirichlet datum).  None switches a term
+        off; arrays shaped as quadrature_points returns the points."""
+        L = lib()
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.pecs_set_manufactured.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]
+        nc, nq, nfq = self._qsizes(target)
+        arrs = []
+        for a, n in ((src, nc * nq), (dir, nc * 4 * nfq), (flux, nc * 4 * nfq)):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = self._f(a).ravel()
+            assert a.size == n, (a.size, n)
+            arrs.append(a)
+        self._c(L.pecs_set_manufactured(self.ctx, int(target), *[None if a is None else _ptr(a) for a in arrs]))
+
+    def _qsizes(self, target):
+        L = lib()
+        L.pecs_quadrature_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        nc, nq, nfq = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        self._c(L.pecs_quadrature_points(self.ctx, int(target), ctypes.byref(nc), ctypes.byref(nq),
+                                         ctypes.byref(nfq), None, None))
+        return nc.value, nq.value, nfq.value
+
+    def quadrature_points(self, target):
+        """(cell_xy [ncell, nq, 2], face_xy [ncell, 4, nfq, 2]) of target 0 / 1 / 2."""
+        nc, nq, nfq = self._qsizes(target)
+        cx = np.zeros((nc, nq, 2))
+        fx = np.zeros((nc, 4, nfq, 2))
+        self._c(lib().pecs_quadrature_points(self.ctx, int(target), None, None, None, _ptr(cx), _ptr(fx)))
+        return cx, fx
 
     def uniform_lists(self):
         """Bit flags of the cell-local kernels that take wave-uniform cell

@@ -114,6 +114,9 @@ struct pecs_ctx {
   int layout = 0;
   std::unique_ptr<DealiiPoisson> dii;
   std::vector<double> ptmp;  // host conversion buffer
+  // manufactured-solution data per target (semiconductor, electrolyte,
+  // Poisson): source, Dirichlet datum, flux (allocated at the first set)
+  double* man_dev[3][3] = {};
   // optional per-phase timing of pecs_step (hipEvents recorded between phases)
   int timing = 0;
   std::vector<hipEvent_t> events;  // NPHASE+1 per recorded step
@@ -1069,6 +1072,9 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
     tb.phiP = upload(c.get(), T.phiP);
     tb.psix = upload(c.get(), T.psix);
     tb.psiy = upload(c.get(), T.psiy);
+    tb.fs = upload(c.get(), T.fs);
+    tb.fpsix = upload(c.get(), T.fpsix);
+    tb.fpsiy = upload(c.get(), T.fpsiy);
     // regions
     c->Rs = make_region_dev(c.get(), 0, c->mesh.semi, os, c->mesh.semi_sigma_nb, c->Fs[0]);
     c->Re = make_region_dev(c.get(), 1, c->mesh.elec, oe, c->mesh.elec_sigma_nb, c->Fe[0]);
@@ -1737,6 +1743,9 @@ int pecs_sweep_destroy(pecs_sweep* w) {
 
 int pecs_set_mixed_precision(pecs_ctx* ctx, int enable) {
   PECS_TRY(ctx, {
+    if (enable && (ctx->Rs.msrc || ctx->Rs.mdir || ctx->Rs.mflux || ctx->Re.msrc || ctx->Re.mdir ||
+                   ctx->Re.mflux || ctx->Pd.msrc || ctx->Pd.mdir))
+      throw std::runtime_error("mixed precision and manufactured data do not combine");
     if (enable) {
       if (!(ctx->Rs.ucell_own && ctx->Rs.ucell_nbh && ctx->Re.ucell_own && ctx->Re.ucell_nbh && ctx->Pd.ucell) ||
           ctx->tabf_host[0].empty() || ctx->tabf_host[1].empty() || ctx->tabf_host[2].empty())
@@ -1751,6 +1760,73 @@ int pecs_set_mixed_precision(pecs_ctx* ctx, int enable) {
     ctx->Pd.table_f = enable ? ctx->tabf_dev[2] : nullptr;
     // the captured step graphs hold the old kernel arguments: capture again
     drop_graphs(ctx);
+  })
+}
+
+int pecs_set_manufactured(pecs_ctx* ctx, int target, const double* src, const double* dir, const double* flux) {
+  PECS_TRY(ctx, {
+    if (target < 0 || target > 2) throw std::runtime_error("target must be 0 (semiconductor), 1 (electrolyte) or 2 (Poisson)");
+    if (target == 2 && flux) throw std::runtime_error("the Poisson right-hand side takes no manufactured flux");
+    if (ctx->Rs.table_f) throw std::runtime_error("mixed precision and manufactured data do not combine");
+    const RefTables& T = *ctx->T;
+    const int64_t ncell = target == 0 ? ctx->mesh.semi.ncell : target == 1 ? ctx->mesh.elec.ncell : ctx->mesh.ncellP;
+    const int64_t len[3] = {ncell * T.nq, ncell * 4 * T.nfq, ncell * 4 * T.nfq};
+    const double* in[3] = {src, dir, flux};
+    double* dev[3] = {};
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int j = 0; j < 3; ++j) {
+      if (!in[j]) continue;
+      if (!ctx->man_dev[target][j]) ctx->man_dev[target][j] = dalloc<double>(ctx, len[j]);
+      dev[j] = ctx->man_dev[target][j];
+      HIPCHECK(hipMemcpy(dev[j], in[j], len[j] * sizeof(double), hipMemcpyHostToDevice));
+    }
+    bool changed;
+    if (target < 2) {
+      DevRegion& R = target == 0 ? ctx->Rs : ctx->Re;
+      changed = R.msrc != dev[0] || R.mdir != dev[1] || R.mflux != dev[2];
+      R.msrc = dev[0];
+      R.mdir = dev[1];
+      R.mflux = dev[2];
+    } else {
+      changed = ctx->Pd.msrc != dev[0] || ctx->Pd.mdir != dev[1];
+      ctx->Pd.msrc = dev[0];
+      ctx->Pd.mdir = dev[1];
+    }
+    // captured step graphs hold the kernel arguments: capture again when the
+    // pointers change (new contents in the same buffers need no capture)
+    if (changed) drop_graphs(ctx);
+  })
+}
+
+int pecs_quadrature_points(pecs_ctx* ctx, int target, int64_t* ncell, int* nq, int* nfq, double* cell_xy,
+                           double* face_xy) {
+  PECS_TRY(ctx, {
+    if (target < 0 || target > 2) throw std::runtime_error("target must be 0 (semiconductor), 1 (electrolyte) or 2 (Poisson)");
+    const RefTables& T = *ctx->T;
+    const std::vector<double>& V = target == 0 ? ctx->mesh.semi.V : target == 1 ? ctx->mesh.elec.V : ctx->mesh.pV;
+    const int64_t nc = (int64_t)V.size() / 8;
+    if (ncell) *ncell = nc;
+    if (nq) *nq = T.nq;
+    if (nfq) *nfq = T.nfq;
+    for (int64_t c = 0; c < nc; ++c) {
+      const double* v = &V[8 * c];
+      if (cell_xy)
+        for (int q = 0; q < T.nq; ++q) {
+          const Geo g = bilinear(v, T.qx[q], T.qy[q]);
+          cell_xy[2 * (c * T.nq + q)] = g.x;
+          cell_xy[2 * (c * T.nq + q) + 1] = g.y;
+        }
+      if (face_xy)
+        for (int f = 0; f < 4; ++f)
+          for (int q = 0; q < T.nfq; ++q) {
+            double xi, eta;
+            face_point(f, T.fs[q], xi, eta);
+            const Geo g = bilinear(v, xi, eta);
+            const int64_t o = 2 * ((c * 4 + f) * T.nfq + q);
+            face_xy[o] = g.x;
+            face_xy[o + 1] = g.y;
+          }
+    }
   })
 }
 

@@ -158,6 +158,7 @@ __global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, De
       if (ph.srh)
         src += (ph.ni * ph.ni - n1 * n2) / (ph.tau_n * (n1 + ph.ni) + ph.tau_p * (n2 + ph.ni));
     }
+    if (R.msrc) src += R.msrc[(size_t)c * NQ + q];
     const double sv1 = jxw * (n1 * idt + src), sv2 = jxw * (n2 * idt + src);
     const double cp1 = -1.0 * inv_perm * n1 * jxw, cp2 = 1.0 * inv_perm * n2 * jxw;
 #pragma unroll
@@ -175,16 +176,28 @@ __global__ __launch_bounds__(64) void k_rhs_carrier(DevRegion R, DevRegion O, De
   for (int f = 0; f < 4; ++f) {
     const int bid = R.bid[4 * (size_t)c + f];
     if (bid < 0) continue;
-    if (bid == 2) continue;  // Neumann: nothing (SolarCell.cpp:1403-1406)
-    if (!SEMI && bid == 3) continue;
     double nx, ny, len;
     face_nrm(v, f, nx, ny, len);
     const double* fph = T.fphi + (size_t)f * NB * NF;
+    if (R.mflux && (bid == 0 || bid == 2)) {  // manufactured data: prescribed normal flux, -<v, h>
+      for (int q = 0; q < NF; ++q) {
+        const double w = -R.mflux[(4 * (size_t)c + f) * NF + q] * T.fw[q] * len;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          a1v[a] += fph[a * NF + q] * w;
+          a2v[a] += fph[a * NF + q] * w;
+        }
+      }
+    }
+    if (bid == 2) continue;  // Neumann: nothing (SolarCell.cpp:1403-1406)
+    if (!SEMI && bid == 3) continue;
     if (bid == 1) {  // Dirichlet
-      const double e1 = SEMI ? ph.ne : ph.rinf, e2 = SEMI ? ph.pe : ph.oinf;
+      const double c1 = SEMI ? ph.ne : ph.rinf, c2 = SEMI ? ph.pe : ph.oinf;
       const double pen = ph.penalty / hc;
       for (int q = 0; q < NF; ++q) {
         const double jw = T.fw[q] * len;
+        const double e1 = R.mdir ? R.mdir[(4 * (size_t)c + f) * NF + q] : c1;
+        const double e2 = R.mdir ? e1 : c2;
 #pragma unroll
         for (int a = 0; a < NB; ++a) {
           const double fa = fph[a * NF + q];
@@ -1027,7 +1040,8 @@ __device__ __forceinline__ void poisson_fphi(const DevPoisson& P, const DevTabs&
       n1 += d1[a] * T.phi[a * NQ + q];
       n2 += d2[a] * T.phi[a * NQ + q];
     }
-    const double rho = rho0 + (-1.0 * n1 + 1.0 * n2);
+    double rho = rho0 + (-1.0 * n1 + 1.0 * n2);
+    if (P.msrc) rho += P.msrc[(size_t)Kc * NQ + q];
 #pragma unroll
     for (int l = 0; l < NPHI; ++l) fphi[l] -= T.phiP[l * NQ + q] * rho * jxw;
   }
@@ -1039,6 +1053,33 @@ __device__ __forceinline__ double poisson_bias(const double* tb, const DevPhys& 
   constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
   const double* U = tb + NL * NL;
   return ph.phi_bi * U[i] - ph.phi_app * U[NRT + i] - ph.phi_sch * U[2 * NRT + i];
+}
+
+// D rows of the cell's right-hand side from manufactured Dirichlet data:
+// -<psi_i . n, g> over its Dirichlet faces (the same integral as the unit
+// bias parts, with g given at the face points)
+template <int K>
+__device__ __forceinline__ void poisson_mdir(const DevPoisson& P, const DevTabs& T, int Kc, double* f) {
+  constexpr int NRT = Sz<K>::NRT, NF = Sz<K>::NF;
+  double v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) v[t] = P.V[8 * (size_t)Kc + t];
+  for (int fc = 0; fc < 4; ++fc) {
+    if (P.pbid[4 * (size_t)Kc + fc] != 1) continue;  // Dirichlet faces only
+    double nx, ny, len;
+    face_nrm(v, fc, nx, ny, len);
+    for (int q = 0; q < NF; ++q) {
+      const double s = T.fs[q];
+      const double xi = fc == 0 ? 0.0 : fc == 1 ? 1.0 : s, eta = fc == 2 ? 0.0 : fc == 3 ? 1.0 : s;
+      const GeoD g = bilin(v, xi, eta);
+      const double w = -P.mdir[(4 * (size_t)Kc + fc) * NF + q] * T.fw[q] * len;
+#pragma unroll
+      for (int i = 0; i < NRT; ++i) {
+        const double hx = T.fpsix[(fc * NRT + i) * NF + q], hy = T.fpsiy[(fc * NRT + i) * NF + q];
+        f[i] += ((g.J00 * hx + g.J01 * hy) * nx + (g.J10 * hx + g.J11 * hy) * ny) / g.det * w;
+      }
+    }
+  }
 }
 
 // UNI: cells from P.ucell (waves share the cell type: scalar table reads)
@@ -1067,6 +1108,7 @@ __global__ __launch_bounds__(64) void k_poisson_cell(DevPoisson P, DevTabs T, co
   const double* tb = P.table + (size_t)ty * P.stride;
 #pragma unroll
   for (int i = 0; i < NRT; ++i) f[i] = poisson_bias<K>(tb, ph, i);
+  if (P.mdir) poisson_mdir<K>(P, T, Kc, f);
   poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, f + NRT);
   for (int i = 0; i < NL; ++i) {
     double s = 0.0;
@@ -1214,6 +1256,7 @@ __global__ __launch_bounds__(64) void k_poisson_cell_f32(DevPoisson P, DevTabs T
   double f[NL];
 #pragma unroll
   for (int i = 0; i < NRT; ++i) f[i] = poisson_bias<K>(tb, ph, i);
+  if (P.mdir) poisson_mdir<K>(P, T, Kc, f);
   poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, f + NRT);
   float ff[NL];
 #pragma unroll
@@ -1282,11 +1325,15 @@ __global__ __launch_bounds__(64) void k_poisson_rhs_assemble(DevPoisson P, DevTa
   poisson_fphi<K>(P, T, ph, Kc, un, up, ur, uo, ncs, nce, fphi);
   const int* id = P.idx + (size_t)Kc * NL;
   const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  double fd[NRT];
+#pragma unroll
+  for (int i = 0; i < NRT; ++i) fd[i] = poisson_bias<K>(tb, ph, i);
+  if (P.mdir) poisson_mdir<K>(P, T, Kc, fd);
   for (int i = 0; i < NRT; ++i) {
     int m;
     const int fc = face_of_local<K>(i, m);
     if (fc >= 0 && !owns_face<K>(P, Kc, fc)) continue;
-    b[id[i]] = poisson_bias<K>(tb, ph, i);
+    b[id[i]] = fd[i];
   }
 #pragma unroll
   for (int l = 0; l < NPHI; ++l) b[id[NRT + l]] = fphi[l];

@@ -9,6 +9,7 @@ namespace pecs {
 struct DevTabs {
   const double *qx, *qy, *qw, *fw, *phi, *fphi, *phiP;
   const double *psix, *psiy;
+  const double *fs, *fpsix, *fpsiy;  // face points and RT values there (manufactured Dirichlet data)
 };
 
 struct DevRegion {
@@ -32,6 +33,14 @@ struct DevRegion {
   // mixed precision (null: FP64): FP32 copy of the table; with the
   // wave-uniform lists the Schur kernels then contract in FP32
   const float* table_f;
+  // manufactured-solution data of the right-hand side (null: the solar-cell
+  // data), for both carriers of the region: a source at the cell quadrature
+  // points (ncell x nq), the Dirichlet datum at the face points replacing the
+  // equilibrium constants, and a prescribed normal flux h on interface and
+  // Neumann faces, entering as -<v, h> (ncell x 4 x nfq each)
+  const double* msrc;
+  const double* mdir;
+  const double* mflux;
 };
 
 struct DevPoisson {
@@ -54,6 +63,12 @@ struct DevPoisson {
   const int* ucell;      // cells grouped by type in whole waves, padded with -1 (null: cells in order)
   int nucell;
   const float* table_f;  // mixed precision (null: FP64): FP32 copy of the table (Ainv used)
+  // manufactured-solution data (null: the solar-cell data): a source added
+  // to the charge at the cell quadrature points (ncell x nq), and a Dirichlet
+  // datum at the face points of Dirichlet faces (ncell x 4 x nfq), entering
+  // as -<psi.n, g> in place of the bias data
+  const double* msrc;
+  const double* mdir;
 };
 
 struct DevPhys {
