@@ -209,13 +209,25 @@ int pecs_sweep_destroy(pecs_sweep* sweep);
 /* Learned warm start (no reference counterpart): a fully connected network,
  * nin inputs -> nlayers hidden layers of nhidden ReLU units -> nout linear
  * outputs, trained in FP32 on the GPU (MSE loss, Adam) and evaluated with
- * BF16 weights and activations and FP32 accumulation (bf16 = 1) or in FP32.
+ * BF16 weights and activations and FP32 accumulation (bf16 = 1) or in FP32;
+ * every matrix product runs on the matrix cores (FP32 / BF16 MFMA).
  * Parameters are one FP32 array: W_1 (nin x nhidden, row-major), b_1, W_2,
  * b_2, ...  x: batch x nin, y: batch x nout, host arrays, row-major.
  * pecs_mlp_gradients returns the loss and the gradient of every parameter
- * (same layout) without updating them.  Errors: pecs_mlp_last_error(). */
+ * (same layout) without updating them.  Errors: pecs_mlp_last_error().
+ *
+ * pecs_mlp_create_decoder: the same encoder, whose last layer (ReLU) has
+ * c0 g0^2 units read as a c0 x g0 x g0 feature map, then nconv decoder
+ * stages: nearest x2 upsampling if conv_up[j] (else none) and a 3x3
+ * convolution with zero padding to conv_cout[j] channels, ReLU after all but
+ * the last.  Outputs (pecs_mlp_num_outputs): cout_last x G x G per sample,
+ * G = g0 2^(number of upsamplings), channel-major.  Convolution parameters
+ * follow the dense ones: W (cout x cin x 3 x 3), b (cout) per stage. */
 typedef struct pecs_mlp pecs_mlp;
 int pecs_mlp_create(int nin, int nhidden, int nlayers, int nout, unsigned seed, pecs_mlp** out);
+int pecs_mlp_create_decoder(int nin, int nhidden, int nlayers, int c0, int g0, int nconv, const int* conv_cout,
+                            const int* conv_up, unsigned seed, pecs_mlp** out);
+int64_t pecs_mlp_num_outputs(pecs_mlp* net);
 int pecs_mlp_destroy(pecs_mlp* net);
 int64_t pecs_mlp_num_params(pecs_mlp* net);
 int pecs_mlp_get_params(pecs_mlp* net, float* params);
@@ -265,6 +277,27 @@ int pecs_uniform_lists(pecs_ctx* ctx, int* flags);
  * pecs_uniform_lists; else an error).  Takes effect from the next step.  No
  * counterpart in the reference (FP64 throughout); it replaces nothing. */
 int pecs_set_mixed_precision(pecs_ctx* ctx, int enable);
+
+/* Densities on a regular grid (the warm-start network's output space; no
+ * counterpart in the reference).  Each region is read as its logical
+ * G x G grid of cell centres ((a + 1/2) / G, (b + 1/2) / G) of the
+ * region's coarse quadrilateral, the same points for every refinement and
+ * cone angle.  pecs_sample_grid writes the densities of instance i's four
+ * carriers there: out (4 x G x G: electrons, holes on the semiconductor;
+ * reductants, oxidants on the electrolyte; row b, column a).
+ * pecs_set_state_from_grid sets instance i's densities from such a grid --
+ * every nodal DoF takes the grid's bilinear interpolant at its logical
+ * position (constant continuation in the half-cell band along the edges) --
+ * zeroes the currents (the next step recomputes them) and solves the
+ * potential of the new densities, as pecs_set_state without a Poisson
+ * vector does. */
+int pecs_sample_grid(pecs_ctx* ctx, int instance, int G, double* out);
+/* Per instance, ||u^{n+1} - u^n|| / ||u^n|| over the densities of the four
+ * carriers across the last step of the last pecs_step / pecs_sweep_step call
+ * (the warm start's predicted residual after one step from a prediction).
+ * out: one value per instance.  No counterpart in the reference. */
+int pecs_last_density_change(pecs_ctx* ctx, double* out);
+int pecs_set_state_from_grid(pecs_ctx* ctx, int instance, int G, const double* grid);
 
 /* Manufactured-solution data for the right-hand sides (test hooks).  target
  * 0 / 1: the carrier right-hand sides of the semiconductor / electrolyte

@@ -24,12 +24,13 @@ ABI_FUNCTIONS = [
     "pecs_initial_state", "pecs_step", "pecs_last_residual", "pecs_rhs_semiconductor_dev",
     "pecs_rhs_electrolyte_dev", "pecs_rhs_poisson_dev", "pecs_solve_dev", "pecs_state_dev",
     "pecs_enable_timing", "pecs_phase_times", "pecs_factor_bytes", "pecs_sweep_config", "pecs_uniform_lists",
-    "pecs_set_mixed_precision", "pecs_set_manufactured", "pecs_quadrature_points",
+    "pecs_set_mixed_precision", "pecs_set_manufactured", "pecs_quadrature_points", "pecs_sample_grid",
+    "pecs_set_state_from_grid", "pecs_last_density_change",
     "pecs_create_batch", "pecs_batch_size", "pecs_set_state_batch", "pecs_get_state_batch",
     "pecs_last_residual_batch", "pecs_write_vtu", "pecs_use_graph", "pecs_sweep_create", "pecs_sweep_set_active",
     "pecs_sweep_step", "pecs_sweep_destroy", "pecs_mlp_create", "pecs_mlp_destroy", "pecs_mlp_num_params",
     "pecs_mlp_get_params", "pecs_mlp_set_params", "pecs_mlp_gradients", "pecs_mlp_train_step", "pecs_mlp_predict",
-    "pecs_mlp_last_error",
+    "pecs_mlp_last_error", "pecs_mlp_create_decoder", "pecs_mlp_num_outputs",
 ]
 
 
@@ -330,6 +331,34 @@ class SolarCellGPU:
             arrs.append(a)
         self._c(L.pecs_set_manufactured(self.ctx, int(target), *[None if a is None else _ptr(a) for a in arrs]))
 
+    def sample_grid(self, G, instance=0):
+        """Densities of the four carriers at the G x G logical cell centres of
+        their regions: array [4, G, G] (n, p, r, o; row = y index)."""
+        L = lib()
+        L.pecs_sample_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        out = np.zeros((4, G, G))
+        self._c(L.pecs_sample_grid(self.ctx, int(instance), int(G), _ptr(out)))
+        return out
+
+    def density_change(self):
+        """Per instance, the relative change of the four densities over the
+        last step (pecs_last_density_change)."""
+        L = lib()
+        L.pecs_last_density_change.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+        out = np.zeros(self.batch)
+        self._c(L.pecs_last_density_change(self.ctx, _ptr(out)))
+        return out
+
+    def set_state_from_grid(self, grid, instance=0):
+        """Densities from a [4, G, G] grid (nodal values of its bilinear
+        interpolant), currents zero, the potential solved from them."""
+        L = lib()
+        L.pecs_set_state_from_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double)]
+        g = self._f(grid)
+        assert g.ndim == 3 and g.shape[0] == 4 and g.shape[1] == g.shape[2], g.shape
+        self._c(L.pecs_set_state_from_grid(self.ctx, int(instance), int(g.shape[1]), _ptr(g)))
+
     def _qsizes(self, target):
         L = lib()
         L.pecs_quadrature_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
@@ -477,7 +506,11 @@ class WarmStartNet:
     """The warm-start network (pecs_mlp_*): nin -> nlayers x nhidden (ReLU) ->
     nout, trained in FP32 on the GPU, evaluated in BF16 (or FP32)."""
 
-    def __init__(self, nin, nhidden, nlayers, nout, seed=0):
+    def __init__(self, nin, nhidden, nlayers, nout, seed=0, decoder=None):
+        """decoder=(c0, g0, [(cout, up), ...]): the encoder's last layer is a
+        c0 x g0 x g0 feature map and each stage upsamples by 2 (up) and
+        applies a 3x3 convolution to cout channels; nout is then ignored (the
+        output is cout_last x G x G per sample)."""
         L = lib()
         fp = ctypes.POINTER(ctypes.c_float)
         L.pecs_mlp_create.argtypes = [ctypes.c_int] * 4 + [ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]
@@ -490,9 +523,21 @@ class WarmStartNet:
         L.pecs_mlp_train_step.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_int, ctypes.c_float, fp]
         L.pecs_mlp_predict.argtypes = [ctypes.c_void_p, fp, ctypes.c_int, fp, ctypes.c_int]
         L.pecs_mlp_last_error.restype = ctypes.c_char_p
-        self.nin, self.nout = nin, nout
+        L.pecs_mlp_create_decoder.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int)] * 2 + \
+            [ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]
+        L.pecs_mlp_num_outputs.argtypes = [ctypes.c_void_p]
+        L.pecs_mlp_num_outputs.restype = ctypes.c_int64
         self.h = ctypes.c_void_p()
-        self._c(L.pecs_mlp_create(nin, nhidden, nlayers, nout, seed, ctypes.byref(self.h)))
+        if decoder is None:
+            self._c(L.pecs_mlp_create(nin, nhidden, nlayers, nout, seed, ctypes.byref(self.h)))
+        else:
+            c0, g0, stages = decoder
+            co = (ctypes.c_int * len(stages))(*[int(c) for c, _ in stages])
+            up = (ctypes.c_int * len(stages))(*[int(u) for _, u in stages])
+            self._c(L.pecs_mlp_create_decoder(nin, nhidden, nlayers, c0, g0, len(stages), co, up, seed,
+                                              ctypes.byref(self.h)))
+        self.nin = nin
+        self.nout = int(L.pecs_mlp_num_outputs(self.h))
         self.nparams = int(L.pecs_mlp_num_params(self.h))
 
     @staticmethod

@@ -117,6 +117,10 @@ struct pecs_ctx {
   // manufactured-solution data per target (semiconductor, electrolyte,
   // Poisson): source, Dirichlet datum, flux (allocated at the first set)
   double* man_dev[3][3] = {};
+  // grid values of pecs_sample_grid / pecs_set_state_from_grid (4 G^2)
+  double* grid_dev = nullptr;
+  int64_t grid_cap = 0;
+  double* dacc = nullptr;  // pecs_last_density_change: 2 per instance
   // optional per-phase timing of pecs_step (hipEvents recorded between phases)
   int timing = 0;
   std::vector<hipEvent_t> events;  // NPHASE+1 per recorded step
@@ -1616,6 +1620,74 @@ int pecs_get_state_batch(pecs_ctx* ctx, int i, double* un, double* up, double* u
 
 int pecs_get_state(pecs_ctx* ctx, double* un, double* up, double* ur, double* uo, double* pv) {
   return pecs_get_state_batch(ctx, 0, un, up, ur, uo, pv);
+}
+
+int pecs_last_density_change(pecs_ctx* ctx, double* out) {
+  PECS_TRY(ctx, {
+    if (!out) throw std::runtime_error("pecs_last_density_change: null output");
+    const int B = ctx->B;
+    if (!ctx->dacc) ctx->dacc = dalloc<double>(ctx, 2 * (size_t)B);
+    const int64_t nsv[4] = {ctx->Ns, ctx->Ns, ctx->Ne, ctx->Ne};
+    for (int b = 0; b < B; ++b)
+      for (int j = 0; j < 4; ++j) {
+        const int64_t n = nsv[j], off = b * n + 2 * n / 3;  // the density block of [Jx | Jy | u]
+        HIPCHECK((hipError_t)launch_diff_norms(ctx->u[j] + off, ctx->old[j] + off, n / 3,
+                                               ctx->rpart + (size_t)(4 * b + j) * 2 * RES_PARTS, ctx->stream));
+      }
+    HIPCHECK((hipError_t)launch_sum_partials(ctx->rpart, 4, ctx->dacc, B, ctx->stream));
+    std::vector<double> h(2 * (size_t)B);
+    d2h(ctx, h.data(), ctx->dacc, 2 * (int64_t)B);
+    for (int b = 0; b < B; ++b) out[b] = h[2 * b + 1] > 0 ? std::sqrt(h[2 * b] / h[2 * b + 1]) : 0.0;
+  })
+}
+
+namespace {
+double* grid_buffer(pecs_ctx* c, int G) {
+  if (G < 1 || G > 4096) throw std::runtime_error("grid side must be 1..4096");
+  const int64_t n = 4 * (int64_t)G * G;
+  if (n > c->grid_cap) {
+    c->grid_dev = dalloc<double>(c, n);
+    c->grid_cap = n;
+  }
+  return c->grid_dev;
+}
+}  // namespace
+
+int pecs_sample_grid(pecs_ctx* ctx, int i, int G, double* out) {
+  PECS_TRY(ctx, {
+    if (i < 0 || i >= ctx->B) throw std::runtime_error("instance index out of range");
+    if (!out) throw std::runtime_error("pecs_sample_grid: null output");
+    double* g = grid_buffer(ctx, G);
+    const int64_t G2 = (int64_t)G * G;
+    for (int car = 0; car < 4; ++car) {
+      const int64_t n = car < 2 ? ctx->Ns : ctx->Ne;
+      const int rc = launch_grid_sample(ctx->k, ctx->u[car] + i * n, ctx->mesh.r, G, g + car * G2, ctx->stream);
+      if (rc) throw std::runtime_error("launch_grid_sample failed");
+    }
+    d2h(ctx, out, g, 4 * G2);
+  })
+}
+
+int pecs_set_state_from_grid(pecs_ctx* ctx, int i, int G, const double* grid) {
+  PECS_TRY(ctx, {
+    if (i < 0 || i >= ctx->B) throw std::runtime_error("instance index out of range");
+    if (!grid) throw std::runtime_error("pecs_set_state_from_grid: null grid");
+    double* g = grid_buffer(ctx, G);
+    const int64_t G2 = (int64_t)G * G;
+    h2d(ctx, g, grid, 4 * G2);
+    double* u[4];
+    for (int car = 0; car < 4; ++car) {
+      const int64_t n = car < 2 ? ctx->Ns : ctx->Ne;
+      u[car] = ctx->u[car] + i * n;
+      HIPCHECK(hipMemsetAsync(u[car], 0, n * sizeof(double), ctx->stream));  // currents 0
+      const int rc = launch_grid_project(ctx->k, g + car * G2, G, ctx->mesh.r, u[car], ctx->stream);
+      if (rc) throw std::runtime_error("launch_grid_project failed");
+    }
+    // the potential of these densities, as pecs_set_state without one
+    poisson_cells(ctx, i, u[0], u[1], u[2], u[3]);
+    poisson_solve_from_y(ctx, i, 1, ctx->pv + i * ctx->NP);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  })
 }
 
 int pecs_initial_state(pecs_ctx* ctx) {

@@ -1401,6 +1401,79 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
   }
 }
 
+// ------------------------------------------------------------------------
+// Density fields on a regular grid (the warm-start network's output space)
+// ------------------------------------------------------------------------
+// A region's cells are the logical grid (i, j) = Morton decode of c, cell
+// (i, j) covering logical [i, i+1] / N x [j, j+1] / N with its reference
+// coordinates along i and j (the mesh is a uniform logical subdivision of
+// the region's coarse quadrilateral).
+__device__ __forceinline__ int morton_code(int i, int j, int r) {
+  int c = 0;
+  for (int b = 0; b < r; ++b) c |= (((i >> b) & 1) << (2 * b)) | (((j >> b) & 1) << (2 * b + 1));
+  return c;
+}
+
+// Lagrange basis of the K+1 equispaced nodes on [0, 1], node a, at x
+template <int K>
+__device__ __forceinline__ double lagrange_eq(int a, double x) {
+  double v = 1.0;
+#pragma unroll
+  for (int m = 0; m <= K; ++m)
+    if (m != a) v *= (x - (double)m / K) / ((double)(a - m) / K);
+  return v;
+}
+
+// out[b G + a] = density of carrier vector u at logical ((a + 1/2) / G, (b + 1/2) / G)
+template <int K>
+__global__ __launch_bounds__(256) void k_grid_sample(const double* __restrict__ u, int r, int G,
+                                                     double* __restrict__ out) {
+  constexpr int NB = (K + 1) * (K + 1);
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= G * G) return;
+  const int N = 1 << r, nc = N * N;
+  const int a = e % G, b = e / G;
+  const double s = (a + 0.5) / G * N, t = (b + 0.5) / G * N;
+  const int i = min((int)s, N - 1), j = min((int)t, N - 1);
+  const double xi = s - i, eta = t - j;
+  const double* d = u + 2 * (size_t)nc * NB + (size_t)morton_code(i, j, r) * NB;
+  double v = 0.0;
+#pragma unroll
+  for (int q = 0; q <= K; ++q) {
+    const double ly = lagrange_eq<K>(q, eta);
+#pragma unroll
+    for (int p = 0; p <= K; ++p) v += d[p + (K + 1) * q] * lagrange_eq<K>(p, xi) * ly;
+  }
+  out[e] = v;
+}
+
+// Density block of carrier vector u from a G x G grid of values at the
+// logical cell centres ((a + 1/2) / G, (b + 1/2) / G): each nodal DoF takes
+// the bilinear interpolant of the grid at its logical position (constant
+// continuation in the half-cell band along the region's edges).
+template <int K>
+__global__ __launch_bounds__(256) void k_grid_project(const double* __restrict__ grid, int G, int r,
+                                                      double* __restrict__ u) {
+  constexpr int NB = (K + 1) * (K + 1);
+  const int N = 1 << r, nc = N * N;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= nc * NB) return;
+  const int c = e / NB, m = e % NB;
+  int i = 0, j = 0;
+  for (int b = 0; b < r; ++b) {
+    i |= ((c >> (2 * b)) & 1) << b;
+    j |= ((c >> (2 * b + 1)) & 1) << b;
+  }
+  const double s = ((double)i + (double)(m % (K + 1)) / K) / N, t = ((double)j + (double)(m / (K + 1)) / K) / N;
+  const double gx = fmin(fmax(s * G - 0.5, 0.0), (double)(G - 1)), gy = fmin(fmax(t * G - 0.5, 0.0), (double)(G - 1));
+  const int x0 = min((int)gx, G - 2 < 0 ? 0 : G - 2), y0 = min((int)gy, G - 2 < 0 ? 0 : G - 2);
+  const int x1 = min(x0 + 1, G - 1), y1 = min(y0 + 1, G - 1);
+  const double fx = gx - x0, fy = gy - y0;
+  const double v = (1.0 - fy) * ((1.0 - fx) * grid[y0 * G + x0] + fx * grid[y0 * G + x1]) +
+                   fy * ((1.0 - fx) * grid[y1 * G + x0] + fx * grid[y1 * G + x1]);
+  u[2 * (size_t)nc * NB + e] = v;
+}
+
 inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -1667,6 +1740,17 @@ int launch_diff_norms(const double* a, const double* b, int64_t n, double* part,
 int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s) {
   if (n <= 0) return 0;
   k_copy<<<(unsigned)std::min<int64_t>(nblk(n, 256), 4096), 256, 0, s>>>(src, dst, n);
+  return (int)hipGetLastError();
+}
+
+int launch_grid_sample(int k, const double* u, int r, int G, double* out, hipStream_t s) {
+  PECS_DISPATCH_K(k, (k_grid_sample<KK><<<nblk((int64_t)G * G, 256), 256, 0, s>>>(u, r, G, out)));
+  return (int)hipGetLastError();
+}
+
+int launch_grid_project(int k, const double* grid, int G, int r, double* u, hipStream_t s) {
+  const int64_t n = ((int64_t)1 << (2 * r)) * (k + 1) * (k + 1);
+  PECS_DISPATCH_K(k, (k_grid_project<KK><<<nblk(n, 256), 256, 0, s>>>(grid, G, r, u)));
   return (int)hipGetLastError();
 }
 

@@ -191,5 +191,10 @@ int launch_diff_norms(const double* a, const double* b, int64_t n, double* part,
 int launch_sum_partials(const double* part, int nvec, double* acc, int ninst, hipStream_t s);
 // dst[0:n] = src[0:n] as a kernel (stays on the stream's compute queue)
 int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
+// Density of a carrier vector (region refinement r) at the G x G logical
+// cell centres of its region, and the density block from such a grid
+// (nodal values of the grid's bilinear interpolant).
+int launch_grid_sample(int k, const double* u, int r, int G, double* out, hipStream_t s);
+int launch_grid_project(int k, const double* grid, int G, int r, double* u, hipStream_t s);
 
 }  // namespace pecs

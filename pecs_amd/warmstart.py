@@ -3,24 +3,35 @@
 A network trained on steady states that the time stepper itself produced
 predicts the steady densities of a new (applied bias, generation amplitude,
 donor doping, cone half-angle) draw; the stepper then starts from the
-prediction instead of the equilibrium state and still decides
-convergence (the same relative steady residual).  A prediction whose residual
-after the first block of steps is above `fallback_residual` is dropped and the
-instance restarts from the equilibrium state (its steps so far still count).
-(After the first 25 steps at refinement 5 the equilibrium start's residual is
-about 1.5 and a prediction with 0.4 % error about 0.3.)
+prediction instead of the equilibrium state and still decides convergence
+(the same relative steady residual).
 
-The network (pecs_amd.WarmStartNet: hand-written HIP kernels, FP32 training,
-BF16 inference) maps 4 scaled parameters to the density coefficients of the
-four carriers on the logically Cartesian cell grid of one refinement
-(4 x 9 N^2 values at degree 2).  Because each region's cells are numbered by
-their (i, j) position, the same output vector serves every cone angle: it is
-the field resampled on the region's cell grid, applied to each instance's own
-mesh.  Currents are not predicted: the step recomputes them from the
-densities (they carry no time derivative), and the Poisson state is
-recomputed from the densities when the state is set.
+Output space: the densities of the four carriers resampled on a G x G grid
+per region (G = 64): the logical cell centres of each region's coarse
+quadrilateral, the same points for every refinement and cone angle
+(pecs_sample_grid).  A prediction is put on any instance's own DG mesh by
+pecs_set_state_from_grid (nodal values of the grid's bilinear interpolant;
+currents zero; the potential solved from the densities), so one network
+serves refinements 5-8 and every cone angle.
 
-Draws come in groups that share one cone angle (one mesh per group); each
+Network (pecs_amd.WarmStartNet, hand-written HIP kernels on the matrix
+cores, FP32 training, BF16 inference): a 4-layer MLP encoder, 4 -> 3 x 256
+(ReLU) -> 64 x 8 x 8 feature map, and a convolutional decoder: three stages
+of nearest x2 upsampling + 3x3 convolution (64, 32, 16 channels, ReLU) and a
+final 3x3 convolution to the 4 density channels on the 64 x 64 grid.
+
+Fallback: the predicted residual is the steady residual of the predicted
+state preconditioned by one implicit step, relative to the state: the
+change of the four densities over the first step from the prediction,
+||u^1 - u_pred|| / ||u_pred|| (pecs_last_density_change).  Above
+`fallback_residual` (1e-2: the prediction would move by more than 1 % in one
+step) the instance restarts from the equilibrium state; the step it took
+still counts.  (Measured on refinements 3-4 at dt = 1e-3: 2e-3 for the
+equilibrium start, 1e-3 for the exact steady densities resampled through
+the 64 x 64 grid, 6e-3 - 8e-3 for predictions with 1.3 - 1.6 % error, 0.29
+for the steady state with log-normal noise of width 0.7 on every grid value.)
+
+Draws come in groups that share one mesh (refinement and cone angle); each
 group is one batched context (pecs_create_batch), and the groups are stepped
 together (pecs_sweep_*).
 """
@@ -31,17 +42,22 @@ import numpy as np
 from . import SolarCellGPU, SolarCellSweep, WarmStartNet, workload_params
 
 CARRIERS = "npro"
+GRID = 64
+DECODER = dict(hidden=256, layers=3, c0=64, g0=8, stages=[(64, 1), (32, 1), (16, 1), (4, 0)])
 
 
-def draw_groups(rng, ngroups, group):
-    """ngroups angles U[0, 15] degrees; per instance applied bias U[-0.5, 0.5],
-    generation amplitude logU[0.1, 10], donor doping logU[0.5, 5]."""
+def draw_groups(rng, ngroups, group, refines=(5,)):
+    """ngroups meshes: refinement drawn uniformly from `refines`, cone angle
+    U[0, 15] degrees; per instance applied bias U[-0.5, 0.5], generation
+    amplitude logU[0.1, 10], donor doping logU[0.5, 5].
+    Returns [(refine, angle, [(bias, amplitude, doping), ...]), ...]."""
     out = []
     for _ in range(ngroups):
+        ref = int(rng.choice(np.asarray(refines)))
         ang = float(rng.uniform(0.0, 15.0))
         inst = [(float(rng.uniform(-0.5, 0.5)), float(10 ** rng.uniform(-1, 1)),
                  float(10 ** rng.uniform(np.log10(0.5), np.log10(5.0)))) for _ in range(group)]
-        out.append((ang, inst))
+        out.append((ref, ang, inst))
     return out
 
 
@@ -56,27 +72,32 @@ def instance_params(refine, degree, dt, ang, app, g0, ne):
     return p
 
 
-def density_slices(g):
-    """Slices of the density blocks [u] in each carrier vector of context g."""
-    out = {}
-    for c, n in zip(CARRIERS, (g.n_semi, g.n_semi, g.n_elec, g.n_elec)):
-        out[c] = slice(2 * n // 3, n)
-    return out
+def make_contexts(groups, degree, dt):
+    return [SolarCellGPU([instance_params(ref, degree, dt, ang, *d) for d in inst]) for ref, ang, inst in groups]
 
 
 def step_groups(ctxs, tol, check_every, max_steps, log=None, fallback=None):
     """Step batched contexts together until every instance is steady.
     Returns the steps per (context, instance), the wall seconds and the
-    residuals after the first block of steps.  fallback(ci, i, residual) is
-    called once per instance after the first block; returning True means it
-    was restarted."""
+    predicted residuals (empty without a fallback).
+
+    With fallback(ci, i, rho): one step first; rho is the relative change of
+    the densities over it; fallback returns True when it restarted the
+    instance (its count then goes on from 1)."""
     sw = SolarCellSweep(ctxs, min(4, len(ctxs)))
     done = [np.full(g.batch, -1) for g in ctxs]
-    first_res = []
+    rho = []
     active = np.ones(len(ctxs), bool)
     t0 = time.perf_counter()
     n = 0
-    first = True
+    if fallback is not None:
+        sw.step(1)
+        n = 1
+        for ci, g in enumerate(ctxs):
+            r = g.density_change()
+            rho.extend(r.tolist())
+            for i in range(g.batch):
+                fallback(ci, i, r[i])
     while active.any() and n < max_steps:
         k = min(check_every, max_steps - n)
         sw.step(k)
@@ -85,65 +106,73 @@ def step_groups(ctxs, tol, check_every, max_steps, log=None, fallback=None):
             if not active[ci]:
                 continue
             r = g.residuals()
-            if first:
-                first_res.extend(r.tolist())
-                if fallback is not None:
-                    for i in range(g.batch):
-                        if fallback(ci, i, r[i]):
-                            r[i] = np.inf
             d = done[ci]
             d[(d < 0) & (r <= tol)] = n
             if np.all(d >= 0):
                 active[ci] = False
-        first = False
         sw.set_active(active)
         if log and n % 500 == 0:
             log(f"step {n}: {int(active.sum())} of {len(ctxs)} groups still stepping")
     wall = time.perf_counter() - t0
     sw.close()
-    return done, wall, np.array(first_res)
+    return done, wall, np.array(rho)
 
 
-def generate(groups, refine, degree, dt, tol, check_every, max_steps, log=None):
+def generate(groups, degree, dt, tol, check_every, max_steps, G=GRID, log=None):
     """Steady states of every draw, cold-started from the equilibrium state.
-    Returns (X features, Y densities [n_draws, 4 nu], steps, wall, setup)."""
+    Returns (X features, Y grids [n_draws, 4 G^2], steps, wall, setup)."""
     t0 = time.perf_counter()
-    ctxs = [SolarCellGPU([instance_params(refine, degree, dt, ang, *d) for d in inst]) for ang, inst in groups]
+    ctxs = make_contexts(groups, degree, dt)
     setup = time.perf_counter() - t0
     for g in ctxs:
         g.initial_state()
-    done, wall, first_res = step_groups(ctxs, tol, check_every, max_steps, log)
+    done, wall, _ = step_groups(ctxs, tol, check_every, max_steps, log)
     X, Y, steps = [], [], []
-    for (ang, inst), g, d in zip(groups, ctxs, done):
-        sl = density_slices(g)
+    for (ref, ang, inst), g, d in zip(groups, ctxs, done):
         for i, dr in enumerate(inst):
-            st = g.get_state(i)
             X.append(features(ang, *dr))
-            Y.append(np.concatenate([st[c][sl[c]] for c in CARRIERS]))
+            Y.append(g.sample_grid(G, i).ravel())
             steps.append(int(d[i]))
         g.close()
-    return np.array(X, np.float32), np.array(Y), np.array(steps), wall, setup, first_res
+    return np.array(X, np.float32), np.array(Y), np.array(steps), wall, setup
 
 
 class Normaliser:
-    def __init__(self, Y):
-        self.mu = Y.mean(axis=0)
-        sd = Y.std(axis=0)
-        self.sd = np.maximum(sd, 1e-6 * max(np.abs(Y).max(), 1e-30))
+    """Per-output standardisation, of log(y) when log=True (the densities
+    span orders of magnitude across the draws: the network then works in
+    relative errors)."""
+
+    FLOOR = 1e-12
+
+    def __init__(self, Y, log=False):
+        self.log = log
+        T = np.log(np.maximum(Y, self.FLOOR)) if log else Y
+        self.mu = T.mean(axis=0)
+        sd = T.std(axis=0)
+        self.sd = np.maximum(sd, 1e-6 * max(np.abs(T).max(), 1e-30))
 
     def fwd(self, Y):
-        return ((Y - self.mu) / self.sd).astype(np.float32)
+        T = np.log(np.maximum(Y, self.FLOOR)) if self.log else Y
+        return ((T - self.mu) / self.sd).astype(np.float32)
 
     def back(self, Z):
-        return Z.astype(np.float64) * self.sd + self.mu
+        T = Z.astype(np.float64) * self.sd + self.mu
+        return np.exp(T) if self.log else T
 
 
-def train(X, Y, hidden=512, layers=3, epochs=600, batch=64, lr=2e-3, seed=0, log=None):
-    """FP32 training with Adam (learning rate decayed by cosine); returns the
-    network, the output normaliser and the loss history (per epoch)."""
-    norm = Normaliser(Y)
+def make_net(spec=None, seed=0):
+    s = dict(DECODER, **(spec or {}))
+    return WarmStartNet(4, s["hidden"], s["layers"], 0, seed=seed, decoder=(s["c0"], s["g0"], s["stages"]))
+
+
+def train(X, Y, spec=None, epochs=300, batch=32, lr=2e-3, seed=0, log=None, log_targets=True):
+    """FP32 training with Adam (learning rate decayed by cosine) on the
+    standardised log-densities; returns the network, the output normaliser
+    and the loss history (per epoch)."""
+    norm = Normaliser(Y, log=log_targets)
     Z = norm.fwd(Y)
-    net = WarmStartNet(X.shape[1], hidden, layers, Y.shape[1], seed=seed)
+    net = make_net(spec, seed)
+    assert net.nout == Y.shape[1], (net.nout, Y.shape)
     rng = np.random.default_rng(seed)
     nb = max(1, len(X) // batch)
     total = epochs * nb
@@ -163,44 +192,58 @@ def train(X, Y, hidden=512, layers=3, epochs=600, batch=64, lr=2e-3, seed=0, log
     return net, norm, hist
 
 
-def evaluate(net, norm, groups, refine, degree, dt, tol, check_every, max_steps, fallback_residual=10.0,
-             bf16=True, log=None):
-    """Warm-started time to steady state for the draws of `groups`.
-    Returns per-instance steps, fallbacks, prediction errors and wall time."""
-    ctxs = [SolarCellGPU([instance_params(refine, degree, dt, ang, *d) for d in inst]) for ang, inst in groups]
-    X = np.array([features(ang, *d) for ang, inst in groups for d in inst], np.float32)
-    t0 = time.perf_counter()
-    pred = norm.back(net.predict(X, bf16=bf16))
-    t_pred = time.perf_counter() - t0
-    cold_state = {}
-    row = 0
+def predict(net, norm, groups, bf16=True, G=GRID):
+    X = np.array([features(ang, *d) for _, ang, inst in groups for d in inst], np.float32)
+    return norm.back(net.predict(X, bf16=bf16)).reshape(len(X), 4, G, G)
+
+
+def evaluate(net, norm, groups, degree, dt, tol, check_every, max_steps, fallback_residual=1e-2, bf16=True,
+             log=None, G=GRID, perturb=None):
+    """Warm-started time to steady state for the draws of `groups`: predict
+    the grids, project them onto every instance's mesh, step with the
+    predicted-residual fallback.  perturb(grids) may alter the predictions
+    (tests of the fallback).  Returns per-instance steps, fallbacks,
+    predicted residuals and the times (predict + projection, stepping)."""
+    ctxs = make_contexts(groups, degree, dt)
+    cold = {}
     for ci, g in enumerate(ctxs):
         g.initial_state()
-        sl = density_slices(g)
-        nu = [sl[c].stop - sl[c].start for c in CARRIERS]
         for i in range(g.batch):
-            st = g.get_state(i)
-            cold_state[(ci, i)] = st
-            new = {c: np.zeros_like(st[c]) for c in CARRIERS}
-            off = 0
-            for c, m in zip(CARRIERS, nu):
-                new[c][sl[c]] = pred[row, off:off + m]
-                off += m
-            new["P"] = None
-            g.set_state(new, i)
+            cold[(ci, i)] = g.get_state(i)
+    t0 = time.perf_counter()
+    pred = predict(net, norm, groups, bf16, G)
+    if perturb is not None:
+        pred = perturb(pred)
+    row = 0
+    for g in ctxs:
+        for i in range(g.batch):
+            g.set_state_from_grid(pred[row], i)
             row += 1
+    t_pred = time.perf_counter() - t0
     fell = set()
 
-    def fallback(ci, i, res):
-        if res > fallback_residual:
-            ctxs[ci].set_state(cold_state[(ci, i)], i)
+    def fallback(ci, i, rho):
+        if not rho <= fallback_residual:  # NaN falls back too
+            ctxs[ci].set_state(cold[(ci, i)], i)
             fell.add((ci, i))
             return True
         return False
 
-    done, wall, first_res = step_groups(ctxs, tol, check_every, max_steps, log, fallback)
-    steps = np.concatenate(done)
+    done, wall, rho = step_groups(ctxs, tol, check_every, max_steps, log, fallback)
     for g in ctxs:
         g.close()
-    return {"steps": steps, "fallbacks": len(fell), "wall": wall, "predict_s": t_pred, "pred": pred,
-            "first_block_residual": first_res}
+    return {"steps": np.concatenate(done), "fallbacks": len(fell), "fell": sorted(fell), "wall": wall,
+            "predict_s": t_pred, "pred": pred, "predicted_residual": rho}
+
+
+def cold_start(groups, degree, dt, tol, check_every, max_steps, log=None, G=GRID):
+    """The same draws from the equilibrium state (the method without the
+    network): steps, wall seconds and the steady densities on the grid."""
+    ctxs = make_contexts(groups, degree, dt)
+    for g in ctxs:
+        g.initial_state()
+    done, wall, _ = step_groups(ctxs, tol, check_every, max_steps, log)
+    grids = np.array([g.sample_grid(G, i) for g in ctxs for i in range(g.batch)])
+    for g in ctxs:
+        g.close()
+    return np.concatenate(done), wall, grids

@@ -67,3 +67,65 @@ def test_training_and_bf16_inference():
 def test_rejects_bad_sizes():
     with pytest.raises(RuntimeError, match="sizes"):
         pecs_amd.WarmStartNet(0, 8, 2, 4)
+
+
+# encoder + convolutional decoder: (nin, nh, nl, c0, g0, stages, batch); the
+# shapes cross tile edges (64-row / 64-column tiles, 16-deep k steps)
+DECODERS = [
+    (4, 16, 2, 4, 2, [(8, 1), (3, 0)], 3),
+    (4, 40, 3, 8, 4, [(16, 1), (8, 1), (4, 0)], 5),
+    (3, 24, 2, 5, 3, [(7, 1), (6, 1), (2, 1)], 2),
+    (4, 32, 2, 8, 4, [(8, 1), (8, 1), (2, 1)], 16),  # weight gradients with split k (K = 16 x 32^2)
+]
+
+
+@pytest.mark.parametrize("spec", DECODERS)
+def test_decoder_gradients_match_reference(spec):
+    from mlp_reference import forward_decoder, layout_decoder, loss_and_grad_decoder
+    nin, nh, nl, c0, g0, stages, b = spec
+    _say(f"[gpu] decoder gradients {spec}")
+    net = pecs_amd.WarmStartNet(nin, nh, nl, 0, seed=4, decoder=(c0, g0, stages))
+    lay, npar = layout_decoder(nin, nh, nl, c0, g0, stages)
+    assert npar == net.nparams
+    G = g0 * 2 ** sum(u for _, u in stages)
+    assert net.nout == stages[-1][0] * G * G
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(b, nin)).astype(np.float32)
+    y = rng.normal(size=(b, net.nout)).astype(np.float32)
+    p = net.params().astype(np.float64)
+    loss, g = net.gradients(x, y)
+    rloss, rg = loss_and_grad_decoder(p, lay, x, y)
+    assert abs(loss - rloss) <= 1e-5 * abs(rloss)
+    # per parameter block: every layer's weights and biases
+    for yl in lay["layers"]:
+        for lo, hi in ((yl["w"], yl["b"]), (yl["b"], yl["b"] + (yl["cout"] if yl["conv"] else yl["nout"]))):
+            scale = np.abs(rg[lo:hi]).max()
+            assert np.abs(g[lo:hi] - rg[lo:hi]).max() <= 1e-4 * scale + 1e-12, (yl, lo, hi)
+    yp = net.predict(x, bf16=False)
+    yr = forward_decoder(p, lay, x)[-1].reshape(b, -1)
+    assert np.abs(yp - yr).max() <= 1e-5 * np.abs(yr).max()
+    y16 = net.predict(x, bf16=True)
+    rel = np.abs(y16 - yr).max() / np.abs(yr).max()
+    assert 0 < rel < 3e-2, rel
+    net.close()
+
+
+def test_decoder_training():
+    _say("[gpu] decoder training")
+    rng = np.random.default_rng(7)
+    G = 16
+    yy, xx = np.meshgrid(np.arange(G) / G, np.arange(G) / G, indexing="ij")
+    x = rng.uniform(-1, 1, size=(128, 3)).astype(np.float32)
+    # smooth fields whose shape depends on the inputs
+    y = np.stack([np.sin(np.pi * (xx[None] * (1 + 0.5 * x[:, :1, None]) + 0.3 * x[:, 1:2, None])) *
+                  np.cos(np.pi * yy[None] * (1 + 0.3 * x[:, 2:3, None])),
+                  np.exp(-((xx[None] - 0.5 - 0.2 * x[:, :1, None]) ** 2 + (yy[None] - 0.5) ** 2) * 8)], 1)
+    y = y.reshape(128, -1).astype(np.float32)
+    net = pecs_amd.WarmStartNet(3, 64, 2, 0, seed=2, decoder=(16, 4, [(16, 1), (2, 1)]))
+    first = net.train_step(x[:32], y[:32], 2e-3)
+    loss = first
+    for it in range(400):
+        i = (it * 32) % 128
+        loss = net.train_step(x[i:i + 32], y[i:i + 32], 2e-3)
+    assert loss < 0.1 * first, (first, loss)
+    net.close()
