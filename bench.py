@@ -293,8 +293,9 @@ def sweep_leg(args):
     instances that fit in HBM at once form a wave, stepped together by one
     graph launch per stream and step (pecs_sweep_*) until each reaches the
     steady tolerance (it then leaves the sweep).  The plain path is each
-    instance stepped alone, launch by launch: its time per step is measured
-    per instance and multiplied by the instance's steps to steady state."""
+    instance stepped alone (graph-replayed steps): its time per step is
+    measured per instance and multiplied by the instance's steps to steady
+    state."""
     import numpy as np
     import torch
     import pecs_amd
@@ -330,17 +331,15 @@ def sweep_leg(args):
                   f"were free", file=sys.stderr, flush=True)
         setup = time.perf_counter() - t0
         total_setup += setup
-        # plain path: each member alone, launch by launch
+        # plain path: each member alone, graph-replayed steps
         for g, i in zip(members, idx):
             g.initial_state()
-            g.use_graph(False)
             g.step(2)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             g.step(args.sweep_plain_steps)
             torch.cuda.synchronize()
             plain_ms[i] = (time.perf_counter() - t1) / args.sweep_plain_steps * 1e3
-            g.use_graph(True)
             g.initial_state()
         sw = pecs_amd.SolarCellSweep(members, args.sweep_streams)
         torch.cuda.synchronize()
@@ -385,70 +384,86 @@ def sweep_leg(args):
 
 def warm_start_leg(args):
     """Learned warm start: steady states of args.ws_train_groups x 16 draws
-    (cone angle per group; applied bias, generation amplitude and doping per
-    instance) at refinement args.ws_refine train the network (FP32, on the
-    GPU); on args.ws_test_groups x 16 held-out draws the time to steady state
-    from the network's prediction (BF16 inference) is compared with the cold
+    (refinement and cone angle per group, drawn from --ws-refines; applied
+    bias, generation amplitude and doping per instance) train the encoder +
+    convolutional decoder (FP32, on the GPU) on 64 x 64 density grids per
+    region; on args.ws_test_groups x 16 held-out draws the time to steady
+    state from the network's prediction (BF16 inference, projected onto each
+    instance's mesh, predicted-residual fallback) is compared with the cold
     start from the equilibrium state (both batched per group and stepped
-    together by pecs_sweep)."""
+    together by pecs_sweep) and with the plain path: each draw alone,
+    graph-replayed, its steps to steady state times its measured time per
+    step."""
     import numpy as np
+    import torch
+    from pecs_amd import SolarCellGPU
     from pecs_amd import warmstart as ws
     rng = np.random.default_rng(args.ws_seed)
     log = lambda m: print(f"[bench] warm start: {m}", file=sys.stderr, flush=True)
-    k, dt, r = args.degree, args.dt, args.ws_refine
-    train_groups = ws.draw_groups(rng, args.ws_train_groups, 16)
-    test_groups = ws.draw_groups(rng, args.ws_test_groups, 16)
-    log(f"generating {16 * len(train_groups)} training steady states (refinement {r})")
-    Xtr, Ytr, steps_tr, wall_tr, setup_tr, _ = ws.generate(train_groups, r, k, dt, args.steady_tol,
-                                                        args.steady_check_every, args.steady_max_steps, log)
+    k, dt = args.degree, args.dt
+    refines = tuple(int(r) for r in args.ws_refines.split(","))
+    train_groups = ws.draw_groups(rng, args.ws_train_groups, 16, refines)
+    test_groups = ws.draw_groups(rng, args.ws_test_groups, 16, refines)
+    log(f"generating {16 * len(train_groups)} training steady states (refinements {refines})")
+    Xtr, Ytr, steps_tr, wall_tr, setup_tr = ws.generate(train_groups, k, dt, args.steady_tol,
+                                                        args.steady_check_every, args.steady_max_steps, log=log)
     log(f"training set done: {wall_tr:.1f} s stepping, {setup_tr:.1f} s setup")
     t0 = time.perf_counter()
-    net, norm, hist = ws.train(Xtr, Ytr, hidden=args.ws_hidden, epochs=args.ws_epochs, log=log)
+    net, norm, hist = ws.train(Xtr, Ytr, epochs=args.ws_epochs, log=log)
     t_train = time.perf_counter() - t0
     log(f"cold start of {16 * len(test_groups)} held-out draws")
-    Xte, Yte, steps_cold, wall_cold, _, res_cold = ws.generate(test_groups, r, k, dt, args.steady_tol,
-                                                     args.steady_check_every, args.steady_max_steps, log)
+    steps_cold, wall_cold, steady = ws.cold_start(test_groups, k, dt, args.steady_tol, args.steady_check_every,
+                                                  args.steady_max_steps, log)
     log("warm start of the held-out draws")
-    ev = ws.evaluate(net, norm, test_groups, r, k, dt, args.steady_tol, args.steady_check_every,
+    ev = ws.evaluate(net, norm, test_groups, k, dt, args.steady_tol, args.steady_check_every,
                      args.steady_max_steps, fallback_residual=args.ws_fallback, log=log)
-    # the plain path: one instance alone, launch by launch (4 held-out draws)
-    import torch
-    from pecs_amd import SolarCellGPU
-    plain = []
-    ang, inst = test_groups[0]
-    for d in inst[:4]:
-        g = SolarCellGPU(ws.instance_params(r, k, dt, ang, *d))
-        g.use_graph(False)
-        g.initial_state()
-        g.step(5)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        g.step(100)
-        torch.cuda.synchronize()
-        plain.append((time.perf_counter() - t1) / 100)
-        g.close()
-    plain_s = float(np.mean(plain)) * float(steps_cold.mean())
-    p32 = norm.back(net.predict(Xte, bf16=False))
-    err = np.linalg.norm(ev["pred"] - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
-    err32 = np.linalg.norm(p32 - Yte, axis=1) / np.linalg.norm(Yte, axis=1)
-    n_te = len(Yte)
+    # the plain path: each draw alone, graph-replayed; its time per step
+    # measured on the first 4 draws of each test group
+    plain_ms = []
+    for ref, ang, inst in test_groups:
+        ms = []
+        for d in inst[:4]:
+            g = SolarCellGPU(ws.instance_params(ref, k, dt, ang, *d))
+            g.initial_state()
+            g.step(5)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            g.step(100)
+            torch.cuda.synchronize()
+            ms.append((time.perf_counter() - t1) / 100 * 1e3)
+            g.close()
+        plain_ms.append(float(np.mean(ms)))
+    per_draw_ms = np.repeat(plain_ms, [len(inst) for _, _, inst in test_groups])
+    plain_s = float(np.sum(np.abs(steps_cold) * per_draw_ms * 1e-3))
+    p32 = ws.predict(net, norm, test_groups, bf16=False)
+    err = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(ev["pred"], steady)])
+    err32 = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(p32, steady)])
+    n_te = len(steady)
     net.close()
-    return {"refinement": r, "train_draws": len(Ytr), "test_draws": n_te, "outputs": int(Ytr.shape[1]),
-            "network": f"4 -> 3 x {args.ws_hidden} ReLU -> outputs; FP32 Adam training, BF16 inference",
+    warm_total = ev["wall"] + ev["predict_s"]
+    d = ws.DECODER
+    return {"refinements_train": sorted({g[0] for g in train_groups}),
+            "refinements_test": [g[0] for g in test_groups], "train_draws": len(Ytr), "test_draws": n_te,
+            "grid": f"{ws.GRID} x {ws.GRID} per region, 4 density channels",
+            "network": (f"encoder 4 -> {d['layers']} x {d['hidden']} ReLU -> {d['c0']} x {d['g0']} x {d['g0']}; "
+                        f"decoder {d['stages']} (channels, x2 upsampling) 3x3 convolutions; FP32 MFMA training, "
+                        f"BF16 MFMA inference"),
             "epochs": args.ws_epochs, "fallback_residual": args.ws_fallback,
             "train_s": round(t_train, 2), "train_loss_first_last": [hist[0], hist[-1]],
             "prediction_rel_error_bf16": {"median": float(np.median(err)), "max": float(err.max())},
             "prediction_rel_error_fp32": {"median": float(np.median(err32)), "max": float(err32.max())},
-            "first_block_residual_median": {"warm": float(np.median(ev["first_block_residual"])),
-                                            "cold": float(np.median(res_cold))},
+            "predicted_residual": {"median": float(np.median(ev["predicted_residual"])),
+                                   "max": float(ev["predicted_residual"].max())},
             "fallbacks": ev["fallbacks"], "tol": args.steady_tol,
-            "warm_steps_median": float(np.median(ev["steps"])), "cold_steps_median": float(np.median(steps_cold)),
-            "cold_steps_mean": float(steps_cold.mean()), "warm_steps_mean": float(ev["steps"].mean()),
-            "cold_s_per_bias_point": wall_cold / n_te, "warm_s_per_bias_point": ev["wall"] / n_te,
-            "warm_reached": bool(np.all(ev["steps"] > 0)),
-            "speedup_warm_vs_cold": wall_cold / ev["wall"] if ev["wall"] > 0 else None,
-            "plain_ms_per_step": 1e3 * float(np.mean(plain)), "plain_s_per_bias_point": plain_s,
-            "speedup_batched_warm_vs_plain": plain_s / (ev["wall"] / n_te) if ev["wall"] > 0 else None}
+            "cold_steps_mean": float(np.mean(steps_cold)), "warm_steps_mean": float(ev["steps"].mean()),
+            "warm_reached": bool(np.all(ev["steps"] > 0)), "cold_reached": bool(np.all(steps_cold > 0)),
+            "cold_s_per_bias_point": wall_cold / n_te,
+            "warm_s_per_bias_point": warm_total / n_te,
+            "warm_predict_and_project_s": ev["predict_s"],
+            "speedup_warm_vs_cold": wall_cold / warm_total if warm_total > 0 else None,
+            "plain_ms_per_step": plain_ms, "plain_s_per_bias_point": plain_s / n_te,
+            "plain_path": "each draw alone, graph-replayed steps",
+            "speedup_batched_warm_vs_plain": plain_s / warm_total if warm_total > 0 else None}
 
 
 def main():
@@ -468,12 +483,12 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="also run an I-V sweep of this many bias points (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
-    ap.add_argument("--sweep", type=int, default=0,
+    ap.add_argument("--sweep", type=int, default=16,
                     help="uneven-mesh sweep of this many instances (refinement, cone angle, bias, generation and "
-                         "doping drawn per instance); 0: skip")
+                         "doping drawn per instance); 0: skip.  The configuration: --sweep 64 --sweep-rmax 8")
     ap.add_argument("--sweep-only", action="store_true", help="run only the uneven-mesh sweep")
     ap.add_argument("--sweep-rmin", type=int, default=5)
-    ap.add_argument("--sweep-rmax", type=int, default=8)
+    ap.add_argument("--sweep-rmax", type=int, default=7)
     ap.add_argument("--sweep-seed", type=int, default=20261202)
     ap.add_argument("--sweep-streams", type=int, default=4)
     ap.add_argument("--sweep-plain-steps", type=int, default=20)
@@ -483,14 +498,15 @@ def main():
                     help="run only the mixed-precision leg (FP32 cell-local contractions) on a batch of --batch")
     ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--warm-start", action="store_true", help="run only the learned warm-start leg")
-    ap.add_argument("--ws-refine", type=int, default=5)
-    ap.add_argument("--ws-train-groups", type=int, default=64, help="16 draws per group (one cone angle)")
-    ap.add_argument("--ws-test-groups", type=int, default=8)
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg of the default run")
+    ap.add_argument("--ws-refines", default="5,6", help="refinements drawn per group (comma separated)")
+    ap.add_argument("--ws-train-groups", type=int, default=16, help="16 draws per group (one mesh)")
+    ap.add_argument("--ws-test-groups", type=int, default=2)
     ap.add_argument("--ws-epochs", type=int, default=600)
     ap.add_argument("--ws-seed", type=int, default=20261202)
-    ap.add_argument("--ws-fallback", type=float, default=10.0,
-                    help="restart a warm-started instance cold if its residual after the first block is above this")
-    ap.add_argument("--ws-hidden", type=int, default=512)
+    ap.add_argument("--ws-fallback", type=float, default=1e-2,
+                    help="restart a warm-started instance cold if its predicted residual (relative density "
+                         "change over the first step) is above this")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying captured step graphs (counter collection)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
@@ -646,6 +662,14 @@ def main():
         out["batched"] = batched_leg(args, workload)
     if args.sweep > 0:
         out["sweep"] = sweep_leg(args)
+    if not args.no_warm_start:
+        out["warm_start"] = warm_start_leg(args)
+    # every speed-up against the plain path: one instance alone, graph-replayed
+    plain = out.get("time_to_steady_state", {}).get("gpu", {})
+    if plain.get("reached"):
+        b = out.get("batched", {}).get("time_to_steady_state", {})
+        if b.get("reached"):
+            out["batched"]["speedup_vs_plain_graph_replay"] = plain["s"] / b["s_per_bias_point"]
     if not args.no_cpu_baseline:
         try:
             tts = out.get("time_to_steady_state", {}).get("gpu", {})
