@@ -104,6 +104,21 @@ def profiled_traffic(workload):
     return float(vals["traffic_bytes_per_launch"])
 
 
+def profiled_values(name, workload, keys):
+    """key: value lines of a committed profile summary measured on this
+    workload (else None)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    vals = dict(line.split(": ", 1) for line in text.splitlines()
+                if line.startswith(tuple(k + ": " for k in list(keys) + ["workload"])))
+    if vals.get("workload", "").strip() != workload:
+        return None
+    return {k: float(vals[k]) for k in keys if k in vals}
+
+
 def cpu_time_to_steady(tol, max_steps=20000):
     """The compiled CPU restatement on the small workload (refinement 4, k = 1,
     dt = 1e-3), stepped until the relative steady residual is at most tol."""
@@ -143,8 +158,18 @@ def assembly_rate(args, g, phase):
     nbytes = 8.0 * ncell * (2 * nb + nrt + 8 + 2 * 3 * nb)
     ms = phase[0] + phase[1]
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"kernel": "k_rhs_carrier (both regions)", "algorithmic_bytes": nbytes, "avg_ms": ms,
-            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    out = {"kernel": "k_rhs_carrier (both regions)", "algorithmic_bytes": nbytes, "avg_ms": ms,
+           "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    wl = (f"PECS half cell, refinement {args.refine} ({1 << args.refine}x{1 << args.refine} quads/region), "
+          f"LDG k={args.degree}, RT{args.degree - 1}xDGQ{args.degree - 1} Poisson, dt={args.dt}")
+    pv = profiled_values("rhs_carrier_r8k2_hbm_traffic.md", wl, ["rhs_traffic_bytes_per_step"])
+    if pv and ms > 0:
+        t = pv["rhs_traffic_bytes_per_step"]
+        out["counter"] = {"traffic_bytes": t, "achieved": t / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                          "frac": t / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "source": "profiles/rhs_carrier_r8k2_hbm_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                                    "WRITE_SIZE; copy-calibrated) over this run's kernel time"}
+    return out
 
 
 def mixed_leg(args, workload):
@@ -262,6 +287,18 @@ def batched_leg(args, workload):
                                    "flops": flops, "avg_ms": cms, "achieved": tf, "peak": FP64_PEAK_TFS,
                                    "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS},
            "sweep_tiles_semi": [list(t) for t in g.sweep_config(0)]}
+    pv = profiled_values("carrier_solves_r8k2_batch16_counters.md", out["workload"],
+                         ["mfma_flops_per_solve", "mfma_busy_cycles_per_solve", "batch_traffic_bytes_per_solve"])
+    if pv and cms > 0:
+        cf = pv["mfma_flops_per_solve"] / (cms * 1e-3) / 1e12
+        out["carrier_solves_mfma"]["counter"] = {
+            "flops": pv["mfma_flops_per_solve"], "achieved": cf, "unit": "TFLOP/s", "frac": cf / FP64_PEAK_TFS,
+            "mfma_busy_frac": pv["mfma_busy_cycles_per_solve"] / (1024 * cms * 1e-3 * 2.4e9),
+            "traffic_bytes": pv["batch_traffic_bytes_per_solve"],
+            "hbm_frac": pv["batch_traffic_bytes_per_solve"] / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "source": "profiles/carrier_solves_r8k2_batch16_counters.md (rocprofv3 --pmc "
+                      "SQ_INSTS_VALU_MFMA_MOPS_F64, SQ_VALU_MFMA_BUSY_CYCLES, FETCH_SIZE, WRITE_SIZE) over this "
+                      "run's kernel time; busy cycles over 1024 SIMDs at 2.4 GHz"}
     if not args.no_steady:
         g.initial_state()
         torch.cuda.synchronize()
@@ -511,6 +548,9 @@ def main():
                     help="launch kernel by kernel instead of replaying captured step graphs (counter collection)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (counter calibration) and exit")
+    ap.add_argument("--mfma-calibration", type=int, default=0,
+                    help="only run an FP64 matrix product of this size (torch.mm, 2 n^3 flops; MFMA counter "
+                         "calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -530,6 +570,16 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 8 * n, "bytes_written": 8 * n}), flush=True)
+        return
+    if args.mfma_calibration > 0:
+        n = args.mfma_calibration
+        a = torch.rand(n, n, dtype=torch.float64, device="cuda")
+        b = torch.rand(n, n, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            c = torch.mm(a, b)
+        torch.cuda.synchronize()
+        print(json.dumps({"mfma_calibration_flops_per_product": 2.0 * n ** 3, "n": n,
+                          "check": float(c[0, 0])}), flush=True)
         return
     if args.warm_start:
         print(json.dumps({"warm_start": warm_start_leg(args)}), flush=True)

@@ -13,11 +13,17 @@ k_poisson_recover for the Poisson multipliers.
 `trace` prints per-kernel statistics and, per window, the sum of kernel
 durations and the span from the first dispatch's start to the last one's end
 (what a hipEvent pair around the concurrent solves measures).
-`pmc` sums FETCH_SIZE / WRITE_SIZE (KiB per dispatch) over each window; with
+`pmc` sums each counter over each window (FETCH_SIZE / WRITE_SIZE in KiB per
+dispatch); with the FP64 MFMA counters it also gives the window's MFMA flops
+(SQ_INSTS_VALU_MFMA_MOPS_F64 x 512, rocprofv3's MfmaFlopsF64) and MFMA
+utilisation (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs), its
+MfmaUtil, GRBM_GUI_ACTIVE summed over the serialised dispatches).  With
 --copy-bytes it also reports the counter reading of the largest plain copy
 dispatch (__amd_rocclr_copyBuffer) against its
 known size, the scale factor that converts the counters into HBM bytes
 (torch's device-to-device copy_ of 1 GiB, run by bench.py --copy-calibration-gib 1).
+`pmc ... --kernel NAME` prints every counter's average per dispatch of the
+kernels whose name starts with NAME (all their dispatches).
 """
 import argparse
 import csv
@@ -96,7 +102,15 @@ def trace(path, top=25, start="k_rhs_carrier"):
               f"first-start..last-end span avg {sum(spans) / max(len(spans), 1):.4f} ms")
 
 
-def pmc(path, copy_bytes=None):
+def _derived(label, sums, simds):
+    if "SQ_INSTS_VALU_MFMA_MOPS_F64" in sums:
+        print(f"{label}: FP64 MFMA flops {sums['SQ_INSTS_VALU_MFMA_MOPS_F64'] * 512:.6g}")
+    if "SQ_VALU_MFMA_BUSY_CYCLES" in sums and sums.get("GRBM_GUI_ACTIVE", 0) > 0:
+        u = sums["SQ_VALU_MFMA_BUSY_CYCLES"] / (sums["GRBM_GUI_ACTIVE"] * simds)
+        print(f"{label}: MFMA busy {100 * u:.2f}% of {simds} SIMDs x {sums['GRBM_GUI_ACTIVE']:.6g} active cycles")
+
+
+def pmc(path, copy_bytes=None, kernel=None, simds=1024):
     rows = _read(path)
     per = OrderedDict()
     for r in rows:
@@ -106,11 +120,23 @@ def pmc(path, copy_bytes=None):
     disp = [(d, e["name"], e["c"]) for d, e in sorted(per.items())]
     counters = sorted({c for _, _, cs in disp for c in cs})
     print(f"dispatches with counters: {len(disp)}; counters: {', '.join(counters)}")
+    if kernel:
+        sel = [cs for _, n, cs in disp if _short(n).startswith(kernel)]
+        print(f"kernel {kernel}: {len(sel)} dispatches")
+        avg = {c: sum(cs.get(c, 0.0) for cs in sel) / max(len(sel), 1) for c in counters}
+        for c in counters:
+            unit = f" KiB = {avg[c] * 1024:.6g} B" if c in ("FETCH_SIZE", "WRITE_SIZE") else ""
+            print(f"kernel {kernel}: {c} per dispatch avg {avg[c]:.6g}{unit}")
+        _derived(f"kernel {kernel} per dispatch", avg, simds)
+        return
     for label, ws in windows(disp).items():
+        avg = {}
         for c in counters:
             vals = [sum(p.get(c, 0.0) for p in w) for w in ws]
-            print(f"solve window {label}: {c} per solve avg {sum(vals) / len(vals):.6g} KiB "
-                  f"= {sum(vals) / len(vals) * 1024:.6g} B over {len(ws)} solves")
+            avg[c] = sum(vals) / len(vals)
+            unit = f" KiB = {avg[c] * 1024:.6g} B" if c in ("FETCH_SIZE", "WRITE_SIZE") else ""
+            print(f"solve window {label}: {c} per solve avg {avg[c]:.6g}{unit} over {len(ws)} solves")
+        _derived(f"solve window {label} per solve", avg, simds)
     if copy_bytes:
         copies = [(d, n, cs) for d, n, cs in disp if "copyBuffer" in n or "copy_kernel" in n]
         if copies:
@@ -127,11 +153,13 @@ def main(argv=None):
     ap.add_argument("mode", choices=["trace", "pmc"])
     ap.add_argument("csv")
     ap.add_argument("--copy-bytes", type=float, default=None)
+    ap.add_argument("--kernel", default=None)
+    ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the GPU (MI355X: 256 CUs x 4)")
     a = ap.parse_args(argv)
     if a.mode == "trace":
         trace(a.csv)
     else:
-        pmc(a.csv, a.copy_bytes)
+        pmc(a.csv, a.copy_bytes, a.kernel, a.simds)
 
 
 if __name__ == "__main__":
