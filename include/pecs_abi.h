@@ -94,7 +94,7 @@ int pecs_scale_parameters(const pecs_physical* in, pecs_scaled* out);
 /* SolarCellProblem constructor + run_full_system setup (SolarCell.cpp:15-92,
  * 1898-1970): meshes, maps, constant matrices, factorisation, device upload. */
 int pecs_create(const pecs_mesh* mesh, const pecs_params* params, pecs_ctx** out);
-/* A batch of `batch` (1..16) instances on one mesh that share every constant
+/* A batch of `batch` (1..64) instances on one mesh that share every constant
  * matrix and factor: an I-V bias sweep, illumination or doping series.  The
  * reference runs one SolarCellProblem per bias point (main.cpp:22-23); here the
  * instances step together and every triangular sweep reads each factor once

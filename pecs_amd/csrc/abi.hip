@@ -167,6 +167,10 @@ T* dalloc(pecs_ctx* c, size_t n) {
   return (T*)p;
 }
 
+// instances of one batched context: two carriers sharing a Schur factor
+// then solve 128 right-hand sides together, the widest MFMA sweep
+constexpr int MAX_BATCH = MAX_MFMA_RHS / 2;
+
 // per-device address of `scratch`, looked up once per device (with that
 // device current) when the first context is created on it
 constexpr int NOALLOC_MAX_DEVICES = 64;
@@ -275,9 +279,10 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
     const int w = 0;
     int m = 0, rows = 0, ks = -1;
     if (std::sscanf(fs, "%d,%d,%d", &m, &rows, &ks) != 3 || (m != 1 && m != 2 && m != 4) || rows < 16 ||
-        rows > 64 * m || rows % 16 != 0 || ks < 0 || (ks > 0 && (ks < 16 || ks % 16 != 0)))
-      throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks (m 1, 2 or 4, rows a multiple of 16 up to "
-                               "64 m, ks 0 or a multiple of 16)");
+        rows > 64 * m || rows % 16 != 0 || ks < 0 || (ks > 0 && (ks < 16 || ks % 16 != 0)) ||
+        !mfma_shape_ok(mfma_rhs_pad(nrhs), m))
+      throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks (m 1, 2 or 4 -- 1 or 2 above 64 right-hand "
+                               "sides -- rows a multiple of 16 up to 64 m, ks 0 or a multiple of 16)");
     for (int l = 0; l < P.nlevels; ++l) {
       P.fw[l] = P.bw[l] = w;
       P.fm[l] = P.bm[l] = m;
@@ -349,7 +354,8 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
         if (nrhs > 2) {
           for (int m : {1, 2, 4})
             for (int rows : {64 * m, 32 * m})
-              for (int ks : {0, 4096, 2048, 1024, 512, 256, 128}) cand.push_back({0, m, rows, ks});
+              for (int ks : {0, 4096, 2048, 1024, 512, 256, 128})
+                if (mfma_shape_ok(mfma_rhs_pad(nrhs), m)) cand.push_back({0, m, rows, ks});
         } else {
           for (int w : {8, 16, 32, 64})
             for (int m : {1, 2, 4, 8}) {
@@ -1011,7 +1017,7 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
   *out = nullptr;
   auto c = std::make_unique<pecs_ctx>();
   try {
-    if (B < 1 || B > 16) throw std::runtime_error("batch size must be 1..16");
+    if (B < 1 || B > MAX_BATCH) throw std::runtime_error("batch size must be 1..64");
     if (!params) throw std::runtime_error("null params");
     for (int i = 0; i < B; ++i) validate(mesh, &params[i]);
     validate_batch(params, B);

@@ -870,12 +870,14 @@ __global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restri
 // ks > 0: the tile is column segment tl.w (columns [tl.w ks, (tl.w + 1) ks))
 // of its row tile and writes its partial sums to Part (slot blockIdx.x, rows
 // relative to row0, 64 MB rows per slot); k_sweep_reduce adds the segments.
-// Single-block tiles (MB = 1) ask for 4 waves per SIMD: the forward sweep
-// then fits its accumulators in the VGPRs (111, no spills) instead of taking
-// 130 registers with AGPRs, which allowed only 3 waves; the backward sweep
-// goes from 4 to 5 waves the same way.
+// Single-block tiles (MB = 1) with up to 32 right-hand sides ask for 4 waves
+// per SIMD: the forward sweep then fits its accumulators in the VGPRs (111,
+// no spills) instead of taking 130 registers with AGPRs, which allowed only 3
+// waves; the backward sweep goes from 4 to 5 waves the same way.  With 64 or
+// 128 right-hand sides the accumulators (NRP / 4 doubles per block) and the
+// double-buffered vector operands need the full register file.
 template <int NRP, int MB, bool FWD>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MB == 1 ? 4 : 1))) void k_sweep_mfma(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MB == 1 && NRP <= 32 ? 4 : 1))) void k_sweep_mfma(
     DevFactor F, const int4* __restrict__ tiles, double* __restrict__ Y, int ks, double* __restrict__ Part) {
   constexpr int NH = NRP / 16;
   const int4 tl = tiles[blockIdx.x];
@@ -1590,17 +1592,21 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
   }
   PECS_MFMA_CASE(16, 1) PECS_MFMA_CASE(16, 2) PECS_MFMA_CASE(16, 4)
   PECS_MFMA_CASE(32, 1) PECS_MFMA_CASE(32, 2) PECS_MFMA_CASE(32, 4)
+  PECS_MFMA_CASE(64, 1) PECS_MFMA_CASE(64, 2) PECS_MFMA_CASE(64, 4)
+  PECS_MFMA_CASE(128, 1) PECS_MFMA_CASE(128, 2)
 #undef PECS_MFMA_CASE
   return (int)hipErrorInvalidValue;
 }
 
-int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : 32; }
+int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : nrhs <= 64 ? 64 : 128; }
+
+bool mfma_shape_ok(int nrp, int mb) { return mb == 1 || mb == 2 || (mb == 4 && nrp <= 64); }
 
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
                  double* t, hipStream_t s, int ks, const int4* rtiles, int nrt, const double* src) {
   if (nt <= 0) return 0;
   if (w == 0) {
-    if (nrhs > 32) return (int)hipErrorInvalidValue;
+    if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
     const int nrp = mfma_rhs_pad(nrhs);
     return fwd ? sweep_mfma_dispatch<true>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s)
                : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s);
@@ -1643,8 +1649,13 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
 
 int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s) {
   if (L.nlevels > 0 && L.fw[0] == 0) {  // MFMA-shaped factor: interleaved vectors
-    if (nrhs > 32) return (int)hipErrorInvalidValue;
-    return mfma_rhs_pad(nrhs) == 16 ? chol_solve_il<16>(F, L, v, t, nrhs, s) : chol_solve_il<32>(F, L, v, t, nrhs, s);
+    if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
+    switch (mfma_rhs_pad(nrhs)) {
+      case 16: return chol_solve_il<16>(F, L, v, t, nrhs, s);
+      case 32: return chol_solve_il<32>(F, L, v, t, nrhs, s);
+      case 64: return chol_solve_il<64>(F, L, v, t, nrhs, s);
+      default: return chol_solve_il<128>(F, L, v, t, nrhs, s);
+    }
   }
   if (nrhs > 2) return (int)hipErrorInvalidValue;
   for (int l = 0; l < L.nlevels; ++l) {

@@ -166,7 +166,12 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
                  double* t, hipStream_t s, int ks = 0, const int4* rtiles = nullptr, int nrt = 0,
                  const double* src = nullptr);
 // Padded right-hand-side count of the MFMA sweep (16 or 32).
+// Right-hand sides of the MFMA sweeps: padded to 16, 32, 64 or 128
+// interleaved columns; at most MAX_MFMA_RHS.  mfma_shape_ok: whether the
+// sweep is built for nrp padded columns with mb 16-row blocks per wave.
+constexpr int MAX_MFMA_RHS = 128;
 int mfma_rhs_pad(int nrhs);
+bool mfma_shape_ok(int nrp, int mb);
 // x = S^{-1} v for nrhs h-major right-hand sides in the factor's permuted
 // order (v is overwritten with x).  t is scratch: nrhs * n doubles for a
 // factor shaped for 1-2 right-hand sides, mfma_rhs_pad(nrhs) * n for an

@@ -2,8 +2,9 @@
 doping points on one mesh share every factor; each instance must follow its
 own oracle trajectory to the trajectory bar (1e-10 relative) and match a
 single-instance GPU run to rounding.  B = 3 (6 carrier right-hand sides per
-region factor) and B = 9 (18) exercise the FP64-MFMA sweep with 16 and 32
-padded right-hand sides; the multiplier solve runs with B right-hand sides.
+region factor), 9 (18), 20 (40) and 40 (80) exercise the FP64-MFMA sweep
+with 16, 32, 64 and 128 padded right-hand sides; the multiplier solve runs
+with B right-hand sides.
 """
 import numpy as np
 import pytest
@@ -35,7 +36,7 @@ def _batch(B, seed, r=3, k=2):
     return prms
 
 
-@pytest.mark.parametrize("B", [3, 9])
+@pytest.mark.parametrize("B", [3, 9, 20, 40])
 def test_batch_matches_oracle(B, seed):
     import pecs_amd
     prms = _batch(B, seed)
@@ -46,12 +47,19 @@ def test_batch_matches_oracle(B, seed):
     g.step(STEPS)
     res = g.residuals()
     assert res.shape == (B,) and np.all(np.isfinite(res)) and np.all(res > 0)
-    for i, p in enumerate(prms):
+    check = range(B) if B <= 20 else range(0, B, 5)  # every instance up to 20, then every fifth
+    for i in check:
         st = g.get_state(i)
-        ref = OracleSolarCell(p).run(STEPS)
+        ref = OracleSolarCell(prms[i]).run(STEPS)
         for c in "nproP":
             assert _rel(st[c], ref[c]) < 1e-10, (B, i, c, _rel(st[c], ref[c]))
     g.close()
+
+
+def test_batch_size_limit():
+    import pecs_amd
+    with pytest.raises(RuntimeError, match="batch size"):
+        pecs_amd.SolarCellGPU(_batch(65, 3, r=1))
 
 
 def test_batch_matches_single_instance(seed):
@@ -95,14 +103,20 @@ def test_batch_rejects_mismatched_matrix_parameters(seed):
 SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,128,256", "1,32,512"]
 
 
-@pytest.mark.parametrize("B,r", [(3, 4), (9, 3)])
+# m = 4 is not built for 128 padded right-hand sides (the accumulators and
+# operands of 4 blocks would not fit the register file)
+WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512"]
+
+
+@pytest.mark.parametrize("B,r", [(3, 4), (9, 3), (40, 3)])
 def test_forced_sweep_shapes(B, r, seed):
     import os
     import pecs_amd
     prms = _batch(B, seed + 7, r=r)
+    shapes = SHAPES if B <= 32 else WIDE
     refs = [OracleSolarCell(p).run(STEPS) for p in prms]
     out = {}
-    for shape in SHAPES:
+    for shape in shapes:
         _say(f"[gpu] forced MFMA sweep shape {shape}: B={B} r={r} k=2, {STEPS} steps")
         os.environ["PECS_SWEEP_FORCE"] = shape
         try:
@@ -120,19 +134,19 @@ def test_forced_sweep_shapes(B, r, seed):
         for i in range(B):
             for c in "nproP":
                 assert _rel(out[shape][i][c], refs[i][c]) < 1e-10, (shape, i, c)
-    for shape in SHAPES[1:]:
+    for shape in shapes[1:]:
         for i in range(B):
             for c in "nproP":
-                assert _rel(out[shape][i][c], out[SHAPES[0]][i][c]) < 1e-12, (shape, i, c)
+                assert _rel(out[shape][i][c], out[shapes[0]][i][c]) < 1e-12, (shape, i, c)
 
 
 def test_forced_sweep_shape_rejected():
     import os
     import pecs_amd
-    prms = _batch(3, 5)
-    os.environ["PECS_SWEEP_FORCE"] = "3,64,0"
-    try:
-        with pytest.raises(RuntimeError, match="PECS_SWEEP_FORCE"):
-            pecs_amd.SolarCellGPU(prms)
-    finally:
-        del os.environ["PECS_SWEEP_FORCE"]
+    for B, shape in ((3, "3,64,0"), (40, "4,256,0")):
+        os.environ["PECS_SWEEP_FORCE"] = shape
+        try:
+            with pytest.raises(RuntimeError, match="PECS_SWEEP_FORCE"):
+                pecs_amd.SolarCellGPU(_batch(B, 5, r=2))
+        finally:
+            del os.environ["PECS_SWEEP_FORCE"]
