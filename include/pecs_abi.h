@@ -191,9 +191,12 @@ int pecs_last_residual(pecs_ctx* ctx, double* residual);       /* max over the i
 int pecs_last_residual_batch(pecs_ctx* ctx, double* residual);  /* one per instance */
 
 /* Uneven-mesh sweeps: contexts with different meshes (refinement, geometry)
- * stepped together.  Each time step launches every active member's captured
- * step graph; members are spread round-robin over nstreams (1..4) streams, so
- * their steps overlap.  pecs_sweep_step
+ * stepped together.  With nstreams 1..4 each time step launches every active
+ * member's captured step graph, members spread round-robin over the streams
+ * so that their steps overlap; with nstreams 0 each time step is one graph
+ * launch, of a graph holding every active member's step graph as a child
+ * graph (rebuilt when the active set or a member's graphs change).
+ * pecs_sweep_step
  * updates every active member's steady residual (pecs_last_residual[_batch]);
  * set_active takes one flag per member (0: left out of later steps, e.g. once
  * steady).  The members must not be used elsewhere during pecs_sweep_step and

@@ -121,6 +121,7 @@ struct pecs_ctx {
   double* grid_dev = nullptr;
   int64_t grid_cap = 0;
   double* dacc = nullptr;  // pecs_last_density_change: 2 per instance
+  uint64_t graph_gen = 0;  // step graphs captured so far (sweeps rebuild their combined graphs on a change)
   // optional per-phase timing of pecs_step (hipEvents recorded between phases)
   int timing = 0;
   std::vector<hipEvent_t> events;  // NPHASE+1 per recorded step
@@ -903,6 +904,7 @@ void capture(pecs_ctx* c, bool last, hipGraph_t& graph, hipGraphExec_t& exec) {
     HIPCHECK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
     graph = g;
     exec = e;
+    ++c->graph_gen;
   } catch (...) {
     c->stream = keep;
     c->timing = timing;
@@ -1744,20 +1746,55 @@ struct pecs_sweep {
   std::vector<pecs_ctx*> m;
   std::vector<char> active;
   std::vector<hipStream_t> st;
-  ~pecs_sweep() {
-    for (hipStream_t s : st) {
-      (void)hipStreamSynchronize(s);
-      (void)hipStreamDestroy(s);
+  // one launch per step (nstreams 0 at creation): the active members' step
+  // graphs (and last-step graphs) as child graphs of one graph each, rebuilt
+  // when the active set or a member's captured graphs change
+  bool one_launch = false;
+  hipGraph_t pg[2] = {};
+  hipGraphExec_t pe[2] = {};
+  std::vector<char> built_active;
+  std::vector<uint64_t> built_gen;
+  void drop() {
+    for (int j = 0; j < 2; ++j) {
+      if (pe[j]) (void)hipGraphExecDestroy(pe[j]);
+      if (pg[j]) (void)hipGraphDestroy(pg[j]);
+      pe[j] = nullptr;
+      pg[j] = nullptr;
     }
   }
+  ~pecs_sweep() {
+    for (hipStream_t s : st) (void)hipStreamSynchronize(s);
+    drop();
+    for (hipStream_t s : st) (void)hipStreamDestroy(s);
+  }
 };
+
+namespace {
+void ensure_sweep_graphs(pecs_sweep* w) {
+  std::vector<uint64_t> gen(w->m.size());
+  for (size_t i = 0; i < w->m.size(); ++i) gen[i] = w->active[i] ? w->m[i]->graph_gen : 0;
+  if (w->pe[0] && w->built_active == w->active && w->built_gen == gen) return;
+  w->drop();
+  for (int j = 0; j < 2; ++j) {
+    HIPCHECK(hipGraphCreate(&w->pg[j], 0));
+    for (size_t i = 0; i < w->m.size(); ++i) {
+      if (!w->active[i]) continue;
+      hipGraphNode_t node;
+      HIPCHECK(hipGraphAddChildGraphNode(&node, w->pg[j], nullptr, 0, j ? w->m[i]->last_graph : w->m[i]->step_graph));
+    }
+    HIPCHECK(hipGraphInstantiate(&w->pe[j], w->pg[j], nullptr, nullptr, 0));
+  }
+  w->built_active = w->active;
+  w->built_gen = gen;
+}
+}  // namespace
 
 int pecs_sweep_create(pecs_ctx** ctxs, int n, int nstreams, pecs_sweep** out) {
   if (!out) return -1;
   *out = nullptr;
   try {
     if (!ctxs || n < 1) throw std::runtime_error("pecs_sweep_create: need at least one context");
-    if (nstreams < 1 || nstreams > 4) throw std::runtime_error("pecs_sweep_create: nstreams must be 1..4");
+    if (nstreams < 0 || nstreams > 4) throw std::runtime_error("pecs_sweep_create: nstreams must be 0..4");
     for (int i = 0; i < n; ++i) {
       if (!ctxs[i]) throw std::runtime_error("pecs_sweep_create: null context");
       for (int j = 0; j < i; ++j)
@@ -1766,7 +1803,8 @@ int pecs_sweep_create(pecs_ctx** ctxs, int n, int nstreams, pecs_sweep** out) {
     auto w = std::make_unique<pecs_sweep>();
     w->m.assign(ctxs, ctxs + n);
     w->active.assign(n, 1);
-    w->st.assign(nstreams, nullptr);
+    w->one_launch = nstreams == 0;
+    w->st.assign(std::max(nstreams, 1), nullptr);
     for (hipStream_t& s : w->st) HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     *out = w.release();
   } catch (const std::exception& e) {
@@ -1796,10 +1834,17 @@ int pecs_sweep_step(pecs_sweep* w, int nsteps) {
       ensure_step_graph(w->m[i]);
       q_of[i] = slot++ % ns;
     }
-    for (int t = 0; t < nsteps; ++t)
-      for (size_t i = 0; i < w->m.size(); ++i)
-        if (q_of[i] >= 0)
-          HIPCHECK(hipGraphLaunch(t == nsteps - 1 ? w->m[i]->last_exec : w->m[i]->step_exec, w->st[q_of[i]]));
+    if (w->one_launch) {
+      if (slot > 0 && nsteps > 0) {
+        ensure_sweep_graphs(w);
+        for (int t = 0; t < nsteps; ++t) HIPCHECK(hipGraphLaunch(w->pe[t == nsteps - 1 ? 1 : 0], w->st[0]));
+      }
+    } else {
+      for (int t = 0; t < nsteps; ++t)
+        for (size_t i = 0; i < w->m.size(); ++i)
+          if (q_of[i] >= 0)
+            HIPCHECK(hipGraphLaunch(t == nsteps - 1 ? w->m[i]->last_exec : w->m[i]->step_exec, w->st[q_of[i]]));
+    }
     if (nsteps > 0) {
       for (hipStream_t s : w->st) HIPCHECK(hipStreamSynchronize(s));
       for (size_t i = 0; i < w->m.size(); ++i)

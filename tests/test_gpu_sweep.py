@@ -1,6 +1,7 @@
 """GPU tests of uneven-mesh sweeps (pecs_sweep_*): contexts with different
 refinements, geometries and degrees stepped together (their captured step
-graphs launched on several streams).  Each member must follow exactly the trajectory it follows
+graphs launched on several streams, or as child graphs of one graph launched
+once per step).  Each member must follow exactly the trajectory it follows
 when stepped alone (same context: bitwise), the oracle to the trajectory bar,
 and inactive members must stay untouched."""
 import numpy as np
@@ -31,7 +32,7 @@ def _state(g):
     return g.get_state(0)
 
 
-@pytest.mark.parametrize("nstreams", [1, 2])
+@pytest.mark.parametrize("nstreams", [0, 1, 2])  # 0: one graph launch per step for all members
 def test_sweep_matches_members_alone(nstreams):
     prms = _members()
     _say(f"[gpu] sweep of 3 uneven meshes, {nstreams} stream(s)")
