@@ -277,13 +277,16 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
   // blocks per wave, rows per tile, column segments of ks (0: unsplit) -- so
   // that tests can run the shapes the tuner picks only on large meshes.
   if (const char* fs = std::getenv("PECS_SWEEP_FORCE"); fs && nrhs > 2) {
-    const int w = 0;
-    int m = 0, rows = 0, ks = -1;
-    if (std::sscanf(fs, "%d,%d,%d", &m, &rows, &ks) != 3 || (m != 1 && m != 2 && m != 4) || rows < 16 ||
-        rows > 64 * m || rows % 16 != 0 || ks < 0 || (ks > 0 && (ks < 16 || ks % 16 != 0)) ||
-        !mfma_shape_ok(mfma_rhs_pad(nrhs), m))
-      throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks (m 1, 2 or 4 -- 1 or 2 above 64 right-hand "
-                               "sides -- rows a multiple of 16 up to 64 m, ks 0 or a multiple of 16)");
+    int m = 0, rows = 0, ks = -1, split = 0;
+    const int nf = std::sscanf(fs, "%d,%d,%d,%d", &m, &rows, &ks, &split);
+    const int nrp = mfma_rhs_pad(nrhs);
+    if ((nf != 3 && nf != 4) || (m != 1 && m != 2 && m != 4) || (split != 0 && split != 1) || rows < 16 ||
+        rows > mfma_tile_rows(nrp, m, split) || rows % 16 != 0 || ks < 0 || (ks > 0 && (ks < 16 || ks % 16 != 0)) ||
+        !mfma_shape_ok(nrp, m))
+      throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks[,split] (m 1, 2 or 4 -- 1 or 2 above 64 "
+                               "right-hand sides -- rows a multiple of 16 up to the rows a workgroup covers, 64 m "
+                               "or with split 1 fewer, ks 0 or a multiple of 16)");
+    const int w = split && nrp > 16 ? -1 : 0;
     for (int l = 0; l < P.nlevels; ++l) {
       P.fw[l] = P.bw[l] = w;
       P.fm[l] = P.bm[l] = m;
@@ -353,10 +356,14 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
         // than 2 right-hand sides
         std::vector<std::array<int, 4>> cand;
         if (nrhs > 2) {
-          for (int m : {1, 2, 4})
-            for (int rows : {64 * m, 32 * m})
-              for (int ks : {0, 4096, 2048, 1024, 512, 256, 128})
-                if (mfma_shape_ok(mfma_rhs_pad(nrhs), m)) cand.push_back({0, m, rows, ks});
+          const int nrp = mfma_rhs_pad(nrhs);
+          for (int split = 0; split < (nrp > 16 ? 2 : 1); ++split)
+            for (int m : {1, 2, 4}) {
+              const int tr = mfma_tile_rows(nrp, m, split);
+              for (int rows : {tr, tr / 2})
+                for (int ks : {0, 4096, 2048, 1024, 512, 256, 128})
+                  if (rows >= 16 && mfma_shape_ok(nrp, m)) cand.push_back({split ? -1 : 0, m, rows, ks});
+            }
         } else {
           for (int w : {8, 16, 32, 64})
             for (int m : {1, 2, 4, 8}) {

@@ -837,12 +837,13 @@ void rebuild_tiles(PackedFactor& P) {
   if ((int)P.fks.size() != P.nlevels) P.fks.assign(P.nlevels, 0);
   if ((int)P.bks.size() != P.nlevels) P.bks.assign(P.nlevels, 0);
   for (int l = 0; l < P.nlevels; ++l) {
-    // lanes per row W > 0: (256 / W) * MAXR rows per tile; W == 0 (the MFMA
-    // sweep): 64 * MAXR rows per tile; only the MFMA sweep splits columns
+    // lanes per row W > 0: (256 / W) * MAXR rows per tile; W <= 0 (the MFMA
+    // sweep): at most 64 * MAXR rows per tile (fewer with its right-hand
+    // sides split over waves, W == -1); only the MFMA sweep splits columns
     auto cap = [](int w, int m) { return w > 0 ? (256 / w) * m : 64 * m; };
     if (P.frows[l] > cap(P.fw[l], P.fm[l]) || P.brows[l] > cap(P.bw[l], P.bm[l]))
       throw std::runtime_error("rebuild_tiles: tile height exceeds the rows one workgroup covers");
-    if ((P.fks[l] && P.fw[l]) || (P.bks[l] && P.bw[l]))
+    if ((P.fks[l] && P.fw[l] > 0) || (P.bks[l] && P.bw[l] > 0))
       throw std::runtime_error("rebuild_tiles: only the MFMA sweep splits columns");
     P.foff[l] = (int)(P.ftiles.size() / 4);
     P.boff[l] = (int)(P.btiles.size() / 4);

@@ -876,17 +876,29 @@ __global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restri
 // waves; the backward sweep goes from 4 to 5 waves the same way.  With 64 or
 // 128 right-hand sides the accumulators (NRP / 4 doubles per block) and the
 // double-buffered vector operands need the full register file.
-template <int NRP, int MB, bool FWD>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MB == 1 && NRP <= 32 ? 4 : 1))) void k_sweep_mfma(
+// HPW < NRP / 16 splits the right-hand-side groups of a row block over
+// NRP / 16 / HPW waves (each loads the row block's factor and its own groups'
+// vector entries): a workgroup then covers 64 MB HPW 16 / NRP rows, with
+// fewer registers per wave and so more waves in flight.
+constexpr int mfma_waves_per_eu(int nrp, int mb, int hpw) {
+  if (hpw == nrp / 16) return mb == 1 && nrp <= 32 ? 4 : 1;
+  return hpw == 1 ? (mb == 1 ? 6 : mb == 2 ? 4 : 2) : 1;
+}
+
+template <int NRP, int MB, bool FWD, int HPW = NRP / 16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(NRP, MB, HPW)))) void k_sweep_mfma(
     DevFactor F, const int4* __restrict__ tiles, double* __restrict__ Y, int ks, double* __restrict__ Part) {
-  constexpr int NH = NRP / 16;
+  constexpr int NH = HPW;                  // right-hand-side groups of this wave
+  constexpr int WPR = NRP / 16 / HPW;      // waves per row block
+  static_assert(WPR >= 1 && 4 % WPR == 0, "a row block's waves must divide the workgroup");
   const int4 tl = tiles[blockIdx.x];
   const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
   const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
   const int VL = FWD ? ns : ns + ms;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ri = lane & 15, kq = lane >> 4;
-  const int first = row0 + wave * MB * 16;               // wave-uniform
+  const int hg0 = (wave % WPR) * HPW;                    // this wave's first right-hand-side group
+  const int first = row0 + (wave / WPR) * MB * 16;       // wave-uniform
   if (first >= row0 + nr) return;                        // no rows for this wave (no barriers below)
   const int last = min(row0 + nr, first + MB * 16) - 1;
   const int lo = FWD ? 0 : first;
@@ -941,7 +953,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MB == 1 && 
     for (int q = 0; q < 4; ++q) {
       const int col = cc + 4 * kq + q;
 #pragma unroll
-      for (int hg = 0; hg < NH; ++hg) Bv[q][hg] = vecv(col, hg * 16 + ri);
+      for (int hg = 0; hg < NH; ++hg) Bv[q][hg] = vecv(col, (hg0 + hg) * 16 + ri);
     }
   };
   auto mma = [&](const double (&L)[MB][4], const double (&Bv)[4][NH]) {
@@ -969,7 +981,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MB == 1 && 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int lr = first + mb * 16 + kq + 4 * r;
-        const int h = hg * 16 + ri;
+        const int h = (hg0 + hg) * 16 + ri;
         if (lr >= row0 + nr) continue;
         const double v = acc[mb][hg][r];
         if (ks > 0) {
@@ -1581,12 +1593,18 @@ static int sweep_dispatch(const DevFactor& F, const int4* tl, int nt, int w, int
 }
 
 template <bool FWD>
-static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int mb, int nrp, double* Y, int ks,
-                               const int4* rtl, int nrt, hipStream_t s) {
+static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int mb, int nrp, bool split, double* Y,
+                               int ks, const int4* rtl, int nrt, hipStream_t s) {
   if (ks > 0 && (!rtl || !F.Part)) return (int)hipErrorInvalidValue;
 #define PECS_MFMA_CASE(NN, MM)                                                              \
-  if (nrp == NN && mb == MM) {                                                              \
+  if (!split && nrp == NN && mb == MM) {                                                    \
     k_sweep_mfma<NN, MM, FWD><<<nt, 256, 0, s>>>(F, tl, Y, ks, F.Part);                     \
+    if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<nrt, 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
+    return 0;                                                                               \
+  }
+#define PECS_MFMA_SPLIT(NN, MM, HH)                                                         \
+  if (split && nrp == NN && mb == MM) {                                                     \
+    k_sweep_mfma<NN, MM, FWD, HH><<<nt, 256, 0, s>>>(F, tl, Y, ks, F.Part);                 \
     if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<nrt, 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
     return 0;                                                                               \
   }
@@ -1594,6 +1612,10 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
   PECS_MFMA_CASE(32, 1) PECS_MFMA_CASE(32, 2) PECS_MFMA_CASE(32, 4)
   PECS_MFMA_CASE(64, 1) PECS_MFMA_CASE(64, 2) PECS_MFMA_CASE(64, 4)
   PECS_MFMA_CASE(128, 1) PECS_MFMA_CASE(128, 2)
+  PECS_MFMA_SPLIT(32, 1, 1) PECS_MFMA_SPLIT(32, 2, 1) PECS_MFMA_SPLIT(32, 4, 1)
+  PECS_MFMA_SPLIT(64, 1, 1) PECS_MFMA_SPLIT(64, 2, 1) PECS_MFMA_SPLIT(64, 4, 1)
+  PECS_MFMA_SPLIT(128, 1, 2) PECS_MFMA_SPLIT(128, 2, 2)
+#undef PECS_MFMA_SPLIT
 #undef PECS_MFMA_CASE
   return (int)hipErrorInvalidValue;
 }
@@ -1602,14 +1624,21 @@ int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : nrhs <= 
 
 bool mfma_shape_ok(int nrp, int mb) { return mb == 1 || mb == 2 || (mb == 4 && nrp <= 64); }
 
+int mfma_tile_rows(int nrp, int mb, bool split) {
+  if (!split || nrp <= 16) return 64 * mb;
+  const int hpw = nrp >= 128 ? 2 : 1, wpr = nrp / 16 / hpw;
+  return 4 / wpr * 16 * mb;
+}
+
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
                  double* t, hipStream_t s, int ks, const int4* rtiles, int nrt, const double* src) {
   if (nt <= 0) return 0;
-  if (w == 0) {
+  if (w <= 0) {  // 0: MFMA sweep, -1: its right-hand-side groups split over waves
     if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
     const int nrp = mfma_rhs_pad(nrhs);
-    return fwd ? sweep_mfma_dispatch<true>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s)
-               : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, t, ks, rtiles, nrt, s);
+    const bool split = w < 0 && nrp > 16;
+    return fwd ? sweep_mfma_dispatch<true>(F, tiles, nt, maxr, nrp, split, t, ks, rtiles, nrt, s)
+               : sweep_mfma_dispatch<false>(F, tiles, nt, maxr, nrp, split, t, ks, rtiles, nrt, s);
   }
   if (nrhs > 2 || ks > 0) return (int)hipErrorInvalidValue;
   const double* v0 = fwd ? src : nullptr;
@@ -1629,8 +1658,8 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
     if (ng > 0)
       k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP / 2, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v, nrhs);
     const int nt = L.foff[l + 1] - L.foff[l];
-    if (L.fw[l] != 0) return (int)hipErrorInvalidValue;
-    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, 0, L.fm[l], true, nrhs, t, s, L.fks[l],
+    if (L.fw[l] > 0) return (int)hipErrorInvalidValue;
+    const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s, L.fks[l],
                                F.frtiles + L.froff[l], L.froff[l + 1] - L.froff[l]);
     if (e) return e;
   }
@@ -1638,8 +1667,8 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
     const int nx = L.xoff[l + 1] - L.xoff[l];
     if (nx > 0) k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP / 2, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     const int nt = L.boff[l + 1] - L.boff[l];
-    if (L.bw[l] != 0) return (int)hipErrorInvalidValue;
-    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, 0, L.bm[l], false, nrhs, t, s, L.bks[l],
+    if (L.bw[l] > 0) return (int)hipErrorInvalidValue;
+    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s, L.bks[l],
                                F.brtiles + L.broff[l], L.broff[l + 1] - L.broff[l]);
     if (e) return e;
   }
@@ -1648,7 +1677,7 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
 }
 
 int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s) {
-  if (L.nlevels > 0 && L.fw[0] == 0) {  // MFMA-shaped factor: interleaved vectors
+  if (L.nlevels > 0 && L.fw[0] <= 0) {  // MFMA-shaped factor: interleaved vectors
     if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
     switch (mfma_rhs_pad(nrhs)) {
       case 16: return chol_solve_il<16>(F, L, v, t, nrhs, s);

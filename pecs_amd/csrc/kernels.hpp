@@ -172,6 +172,9 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
 constexpr int MAX_MFMA_RHS = 128;
 int mfma_rhs_pad(int nrhs);
 bool mfma_shape_ok(int nrp, int mb);
+// rows a workgroup of the MFMA sweep covers: 64 mb, or with the right-hand
+// sides split over waves (sweep width -1) 64 mb / (waves per row block)
+int mfma_tile_rows(int nrp, int mb, bool split);
 // x = S^{-1} v for nrhs h-major right-hand sides in the factor's permuted
 // order (v is overwritten with x).  t is scratch: nrhs * n doubles for a
 // factor shaped for 1-2 right-hand sides, mfma_rhs_pad(nrhs) * n for an

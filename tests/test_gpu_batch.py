@@ -100,12 +100,13 @@ def test_batch_rejects_mismatched_matrix_parameters(seed):
 # shapes the tuner picks only on large meshes (several blocks per wave, split
 # tiles and their reduction) run here against the oracle.  Split tiles add
 # their segments in a fixed order: they agree with unsplit ones to rounding.
-SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,128,256", "1,32,512"]
+SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,128,256", "1,32,512",
+          "1,32,0,1", "2,64,128,1", "4,128,0,1"]  # split: right-hand-side groups over waves
 
 
 # m = 4 is not built for 128 padded right-hand sides (the accumulators and
 # operands of 4 blocks would not fit the register file)
-WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512"]
+WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512", "1,16,0,1", "2,32,256,1"]
 
 
 @pytest.mark.parametrize("B,r", [(3, 4), (9, 3), (40, 3)])
@@ -123,9 +124,11 @@ def test_forced_sweep_shapes(B, r, seed):
             g = pecs_amd.SolarCellGPU(prms)
         finally:
             del os.environ["PECS_SWEEP_FORCE"]
-        m, rows = int(shape.split(",")[0]), int(shape.split(",")[1])
+        f = [int(x) for x in shape.split(",")]
+        m, rows = f[0], f[1]
+        w = -1 if len(f) > 3 and f[3] and 2 * B > 16 else 0
         cfg = g.sweep_config(0)
-        assert all(t[0] == 0 and t[1] == m and t[2] == rows and t[3] == 0 and t[4] == m and t[5] == rows
+        assert all(t[0] == w and t[1] == m and t[2] == rows and t[3] == w and t[4] == m and t[5] == rows
                    for t in cfg), cfg
         g.initial_state()
         g.step(STEPS)
@@ -143,7 +146,7 @@ def test_forced_sweep_shapes(B, r, seed):
 def test_forced_sweep_shape_rejected():
     import os
     import pecs_amd
-    for B, shape in ((3, "3,64,0"), (40, "4,256,0")):
+    for B, shape in ((3, "3,64,0"), (40, "4,256,0"), (9, "1,64,0,1")):  # split covers 32 rows at 32 columns
         os.environ["PECS_SWEEP_FORCE"] = shape
         try:
             with pytest.raises(RuntimeError, match="PECS_SWEEP_FORCE"):
