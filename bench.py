@@ -476,10 +476,44 @@ def warm_start_leg(args):
     err = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(ev["pred"], steady)])
     err32 = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(p32, steady)])
     n_te = len(steady)
-    net.close()
     warm_total = ev["wall"] + ev["predict_s"]
+    # the measured workload's refinement: one group of 16 held-out draws on
+    # one planar mesh (cone angle 0: the cell-type lists of the planar
+    # workload apply), a batched context warm-started from the same network
+    # (its 64 x 64 grids projected onto refinement args.ws_eval_refine)
+    big = None
+    if args.ws_eval_refine > 0:
+        grp = [(args.ws_eval_refine, 0.0, inst) for _, _, inst in ws.draw_groups(rng, 1, 16, refines)]
+        log(f"refinement {args.ws_eval_refine}: cold start of 16 held-out draws")
+        bc, bwall, bsteady = ws.cold_start(grp, k, dt, args.steady_tol, args.steady_check_every,
+                                           args.steady_max_steps, log)
+        log(f"refinement {args.ws_eval_refine}: warm start of the same draws")
+        bev = ws.evaluate(net, norm, grp, k, dt, args.steady_tol, args.steady_check_every, args.steady_max_steps,
+                          fallback_residual=args.ws_fallback, log=log)
+        g = SolarCellGPU(ws.instance_params(args.ws_eval_refine, k, dt, grp[0][1], *grp[0][2][0]))
+        g.initial_state()
+        g.step(5)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g.step(50)
+        torch.cuda.synchronize()
+        bms = (time.perf_counter() - t1) / 50 * 1e3
+        g.close()
+        bplain = float(np.mean(np.abs(bc))) * bms * 1e-3
+        bwarm = (bev["wall"] + bev["predict_s"]) / 16
+        berr = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(bev["pred"], bsteady)])
+        big = {"refinement": args.ws_eval_refine, "draws": 16, "cold_steps_mean": float(np.mean(bc)),
+               "warm_steps_mean": float(bev["steps"].mean()), "fallbacks": bev["fallbacks"],
+               "prediction_rel_error": {"median": float(np.median(berr)), "max": float(berr.max())},
+               "cold_s_per_bias_point": bwall / 16, "warm_s_per_bias_point": bwarm,
+               "plain_ms_per_step": bms, "plain_s_per_bias_point": bplain,
+               "speedup_warm_vs_cold": (bwall / 16) / bwarm if bwarm > 0 else None,
+               "speedup_batched_warm_vs_plain": bplain / bwarm if bwarm > 0 else None}
+        log(f"refinement {args.ws_eval_refine}: {big}")
+    net.close()
     d = ws.DECODER
-    return {"refinements_train": sorted({g[0] for g in train_groups}),
+    return {"measured_refinement": big,
+            "refinements_train": sorted({g[0] for g in train_groups}),
             "refinements_test": [g[0] for g in test_groups], "train_draws": len(Ytr), "test_draws": n_te,
             "grid": f"{ws.GRID} x {ws.GRID} per region, 4 density channels",
             "network": (f"encoder 4 -> {d['layers']} x {d['hidden']} ReLU -> {d['c0']} x {d['g0']} x {d['g0']}; "
@@ -540,6 +574,8 @@ def main():
     ap.add_argument("--ws-train-groups", type=int, default=16, help="16 draws per group (one mesh)")
     ap.add_argument("--ws-test-groups", type=int, default=2)
     ap.add_argument("--ws-epochs", type=int, default=600)
+    ap.add_argument("--ws-eval-refine", type=int, default=8,
+                    help="also warm-start one group of 16 held-out draws at this refinement (0: skip)")
     ap.add_argument("--ws-seed", type=int, default=20261202)
     ap.add_argument("--ws-fallback", type=float, default=1e-2,
                     help="restart a warm-started instance cold if its predicted residual (relative density "
