@@ -62,10 +62,11 @@ class CarrierFE:
         GV[2 * nb:] = g
         return P, DIVP, VV, GV
 
-    def face_parts(self, f):
-        """Face tabulation: PF[i,q,2] (current), VF[i,q] (density)."""
+    def face_parts(self, f, ph=None):
+        """Face tabulation: PF[i,q,2] (current), VF[i,q] (density); ph
+        replaces the basis values at the face points (subface points)."""
         nb = self.nb
-        ph = self.fphi[f]
+        ph = self.fphi[f] if ph is None else ph
         nq = ph.shape[1]
         PF = np.zeros((3 * nb, nq, 2))
         VF = np.zeros((3 * nb, nq))
@@ -77,6 +78,27 @@ class CarrierFE:
 
 def _scatter(rows, cols, vals, n):
     return sp.coo_matrix((vals.ravel(), (rows.ravel(), cols.ravel())), shape=(n, n)).tocsr()
+
+
+def _flux_blocks(PFm, VFm, Pm_n, PFp, VFp, Pp_n, sig, jw):
+    """Local flux matrices (mm, mp, pm, pp) of one interior face
+    (assemble_local_flux_terms, LDG.cpp:437-626): minus side m assembles with
+    its outward normal n (Pm_n, Pp_n = p.n of each side's current basis)."""
+    bPm = PFm @ BETA
+    bPp = PFp @ BETA
+
+    def blk(Pa_n, Va, bPa, Pb_n, Vb, bPb, s_half, s_beta, s_pen):
+        # s_half * 1/2 (p_i.n u_j + v_i n.q_j) + s_beta*((b.p_i) u_j - v_i (b.q_j))
+        # + s_pen * sigma v_i u_j
+        t = 0.5 * s_half * (np.einsum("ciq,jq->cijq", Pa_n, Vb) + np.einsum("iq,cjq->cijq", Va, Pb_n))
+        t = t + s_beta * (np.einsum("iq,jq->ijq", bPa, Vb)[None] - np.einsum("iq,jq->ijq", Va, bPb)[None])
+        t = t + s_pen * np.einsum("iq,jq->ijq", Va, Vb)[None] * sig[:, None, None, :]
+        return np.einsum("cijq,cq->cij", t, jw)
+
+    return (blk(Pm_n, VFm, bPm, Pm_n, VFm, bPm, 1.0, 1.0, 1.0),
+            blk(Pm_n, VFm, bPm, Pp_n, VFp, bPp, 1.0, -1.0, -1.0),
+            blk(Pp_n, VFp, bPp, Pm_n, VFm, bPm, -1.0, -1.0, -1.0),
+            blk(Pp_n, VFp, bPp, Pp_n, VFp, bPp, -1.0, 1.0, 1.0))
 
 
 def assemble_ldg(region, cfe, mu1, mu2, dt, theta, penalty, flip_orientation=False):
@@ -146,27 +168,40 @@ def assemble_ldg(region, cfe, mu1, mu2, dt, theta, penalty, flip_orientation=Fal
         sig = (penalty / hmin)[:, None]
         Pm_n = np.einsum("iqd,cqd->ciq", PFm, nrm)
         Pp_n = np.einsum("iqd,cqd->ciq", PFp, nrm)
-        bPm = PFm @ BETA
-        bPp = PFp @ BETA
-
-        def blk(Pa_n, Va, bPa, Pb_n, Vb, bPb, s_half, s_beta, s_pen):
-            # s_half * 1/2 (p_i.n u_j + v_i n.q_j) + s_beta*((b.p_i) u_j - v_i (b.q_j))
-            # + s_pen * sigma v_i u_j
-            t = 0.5 * s_half * (np.einsum("ciq,jq->cijq", Pa_n, Vb) + np.einsum("iq,cjq->cijq", Va, Pb_n))
-            t = t + s_beta * (np.einsum("iq,jq->ijq", bPa, Vb)[None] - np.einsum("iq,jq->ijq", Va, bPb)[None])
-            t = t + s_pen * np.einsum("iq,jq->ijq", Va, Vb)[None] * sig[:, None, None, :]
-            return np.einsum("cijq,cq->cij", t, jw)
-
-        mm = blk(Pm_n, VFm, bPm, Pm_n, VFm, bPm, 1.0, 1.0, 1.0)
-        mp = blk(Pm_n, VFm, bPm, Pp_n, VFp, bPp, 1.0, -1.0, -1.0)
-        pm = blk(Pp_n, VFp, bPp, Pm_n, VFm, bPm, -1.0, -1.0, -1.0)
-        pp = blk(Pp_n, VFp, bPp, Pp_n, VFp, bPp, -1.0, 1.0, 1.0)
         dm, dp = dofs[cells], dofs[e]
-        for B, r_, c_ in ((mm, dm, dm), (mp, dm, dp), (pm, dp, dm), (pp, dp, dp)):
+        for B, r_, c_ in zip(_flux_blocks(PFm, VFm, Pm_n, PFp, VFp, Pp_n, sig, jw), (dm, dm, dp, dp), (dm, dp, dm, dp)):
             rows.append(np.repeat(r_[:, :, None], nl, 2))
             cols.append(np.repeat(c_[:, None, :], nl, 1))
             vals1.append(B)
             vals2.append(B)
+    # hanging faces (LDG.cpp:338-383): the coarse cell assembles each subface
+    # with its own outward normal, its basis at the subface's quadrature
+    # points and the fine cell's basis at its face's points (the same points),
+    # h = min(coarse diameter, diameter of the fine cell's parent)
+    hang = getattr(region, "hang", None)
+    for f in range(4):
+        for s in range(2):
+            if hang is None or len(hang) == 0:
+                break
+            sel = hang[(hang[:, 1] == f) & (hang[:, 2] == s)]
+            if len(sel) == 0:
+                continue
+            cells, e, fo = sel[:, 0], sel[:, 3], int(sel[0, 4])
+            sub = (s + cfe.fs) / 2.0
+            PFm, VFm = cfe.face_parts(f, cfe.dgq.tab(fe.face_points(f, sub))[0])
+            PFp, VFp = cfe.face_parts(fo)
+            _, nrm, ln = fe.face_geometry(V[cells], f, sub)
+            jw = cfe.fw[None, :] * ln / 2.0
+            sig = (penalty / np.minimum(h[cells], region.pdiam[e]))[:, None]
+            Pm_n = np.einsum("iqd,cqd->ciq", PFm, nrm)
+            Pp_n = np.einsum("iqd,cqd->ciq", PFp, nrm)
+            dm, dp = dofs[cells], dofs[e]
+            for B, r_, c_ in zip(_flux_blocks(PFm, VFm, Pm_n, PFp, VFp, Pp_n, sig, jw), (dm, dm, dp, dp),
+                                 (dm, dp, dm, dp)):
+                rows.append(np.repeat(r_[:, :, None], nl, 2))
+                cols.append(np.repeat(c_[:, None, :], nl, 1))
+                vals1.append(B)
+                vals2.append(B)
     R = np.concatenate([r.ravel() for r in rows])
     C = np.concatenate([c.ravel() for c in cols])
     A1 = sp.coo_matrix((np.concatenate([v.ravel() for v in vals1]), (R, C)), shape=(n, n)).tocsr()

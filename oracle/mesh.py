@@ -16,8 +16,10 @@ Restates:
     (semiconductor cell (N-1, j) face 1 <-> electrolyte cell (0, j) face 0)
     and the region -> Poisson cell maps, which for these meshes are index
     maps (the reference finds them by an O(N^2) search on cell centres).
-Boundary-layer local refinement (n_local_refine > 0, Grid.cpp:72-111) is not
-restated: none of the measured configurations uses it.
+  * boundary-layer local refinement (n_local_refine = 1, Grid.cpp:11-44,
+    72-111): LocalRefinedMesh below (deeper local refinement makes deal.II
+    refine bulk cells too to keep one level between face neighbours, and is
+    not restated).
 """
 import numpy as np
 
@@ -213,3 +215,226 @@ def poisson_dofs(mesh, deg):
         for l in range(nphi):
             idx[p, 2 * nx + l] = nD + p * nphi + l
     return idx, nD, nD + ncp * nphi
+
+
+# ---------------------------------------------------------------------------
+# Boundary-layer local refinement (Grid.cpp:11-44, 72-111, 135-336)
+# ---------------------------------------------------------------------------
+# With n_local_refine = 1 and a boundary layer of width bl > 0 each region is
+# two coarse quads side by side -- semiconductor: bulk [0, r-bl] then layer
+# [r-bl, r]; electrolyte: layer [r, r+bl] then bulk [r+bl, L] (r running
+# from r2 at y = 0 to r1 at y = H) -- both refined r times, after which the
+# layer cells (material ids 1 and 3) are refined once more.  The bulk/layer
+# face is then hanging: one bulk cell face meets two layer cell faces
+# (subface 0 = the half at the face's first vertex, i.e. the lower half).
+# deal.II lists active cells level by level, so a region's cells are its
+# level-r block (Morton) followed by its level-(r+1) block (Morton over the
+# 2N x 2N children, index 4*parent + child).  The merged Poisson mesh here
+# lists the semiconductor cells then the electrolyte cells, each in its
+# region's order, so the region -> Poisson maps stay index maps.
+#
+# Grid positions: every block is laid on one fine lattice (2 positions per
+# level-r cell width, 1 per level-(r+1) cell); a cell's position is its lower
+# left lattice point.  They are used by the nested-dissection ordering only.
+
+class BlockRegion:
+    """Structured blocks side by side (left to right), each one coarse quad
+    refined r + lev times; cells listed in the block order `order`."""
+
+    def __init__(self, blocks, r, order, classify):
+        # blocks: list of (coarse quad, lev in {0, 1}, material)
+        self.r = r
+        self.N = 1 << r
+        gscale = 2 if any(b[1] for b in blocks) else 1
+        self.blocks = []
+        Vs, mats, ijs, gij, levs, pd = [], [], [], [], [], []
+        base = {}
+        c0 = 0
+        for bi in order:
+            coarse, lev, mat = blocks[bi]
+            rr = r + lev
+            n = 1 << rr
+            G = vertex_grid(np.asarray(coarse, dtype=float), rr)
+            ij = np.array([morton_ij(c, rr) for c in range(n * n)], dtype=np.int64)
+            V = np.stack([G[ij[:, 0], ij[:, 1]], G[ij[:, 0] + 1, ij[:, 1]],
+                          G[ij[:, 0], ij[:, 1] + 1], G[ij[:, 0] + 1, ij[:, 1] + 1]], 1)
+            if lev:
+                I, J = 2 * (ij[:, 0] // 2), 2 * (ij[:, 1] // 2)
+                Vp = np.stack([G[I, J], G[I + 2, J], G[I, J + 2], G[I + 2, J + 2]], 1)
+                pd.append(_diam(Vp))
+            else:
+                pd.append(_diam(V))
+            sc = gscale >> lev
+            gx0 = bi * (gscale << r)
+            gij.append(np.stack([gx0 + sc * ij[:, 0], sc * ij[:, 1]], 1))
+            cell_of = np.empty((n, n), dtype=np.int64)
+            cell_of[ij[:, 0], ij[:, 1]] = c0 + np.arange(n * n)
+            base[bi] = dict(n=n, lev=lev, cell_of=cell_of, c0=c0)
+            Vs.append(V)
+            mats.append(np.full(n * n, mat))
+            ijs.append(ij)
+            levs.append(np.full(n * n, lev))
+            c0 += n * n
+        self.V = np.concatenate(Vs)
+        self.mat = np.concatenate(mats)
+        self.material = int(blocks[0][2])
+        self.lij = np.concatenate(ijs)       # (i, j) inside the cell's block
+        self.gij = np.concatenate(gij)       # lattice position
+        self.level = np.concatenate(levs)
+        self.pdiam = np.concatenate(pd)      # diameter of the level-r cell containing the cell
+        self.NXg = len(blocks) * (gscale << r)
+        self.NYg = gscale << r
+        self.ncell = len(self.V)
+        self.block_info = base
+        nc = self.ncell
+        self.nbr = np.full((nc, 4), -1, dtype=np.int64)
+        self.bid = np.full((nc, 4), INTERIOR, dtype=np.int64)
+        hang = []
+        for bi in range(len(blocks)):
+            B = base[bi]
+            n, co = B["n"], B["cell_of"]
+            for i in range(n):
+                for j in range(n):
+                    c = co[i, j]
+                    for f in range(4):
+                        di, dj = FACE_NBR[f]
+                        ii, jj = i + di, j + dj
+                        if 0 <= ii < n and 0 <= jj < n:
+                            self.nbr[c, f] = co[ii, jj]
+            # faces on the block's right edge meet block bi + 1
+            if bi + 1 < len(blocks):
+                R = base[bi + 1]
+                if R["lev"] == B["lev"]:
+                    for j in range(n):
+                        a, b = co[n - 1, j], R["cell_of"][0, j]
+                        self.nbr[a, 1], self.nbr[b, 0] = b, a
+                elif B["lev"] < R["lev"]:      # coarse left, fine right
+                    for j in range(n):
+                        for s in range(2):
+                            hang.append((co[n - 1, j], 1, s, R["cell_of"][0, 2 * j + s], 0))
+                else:                          # fine left, coarse right
+                    m = R["n"]
+                    for j in range(m):
+                        for s in range(2):
+                            hang.append((R["cell_of"][0, j], 0, s, co[n - 1, 2 * j + s], 1))
+        self.hang = np.array(hang, dtype=np.int64).reshape(-1, 5)
+        hanging = np.zeros((nc, 4), dtype=bool)
+        for c, f, s, e, fe_ in self.hang:
+            hanging[c, f] = True
+            hanging[e, fe_] = True
+        for c in range(nc):
+            for f in range(4):
+                if self.nbr[c, f] < 0 and not hanging[c, f]:
+                    a, b = FACE_VERTS[f]
+                    cen = (self.V[c, a] + self.V[c, b]) / 2.0
+                    self.bid[c, f] = classify(cen[0], cen[1])
+
+
+def _diam(V):
+    d1 = np.linalg.norm(V[:, 3] - V[:, 0], axis=-1)
+    d2 = np.linalg.norm(V[:, 2] - V[:, 1], axis=-1)
+    return np.maximum(d1, d2)
+
+
+class LocalRefinedMesh:
+    """SolarCellMesh with boundary-layer local refinement (n_local_refine = 1)."""
+
+    def __init__(self, r, geo, bl, insulated=False, schottky=False):
+        if not bl > 0.0:
+            raise ValueError("local refinement needs a boundary layer of positive width")
+        r1, r2, H, L = geo["r1"], geo["r2"], geo["H"], geo["L"]
+        self.geo = geo
+        self.r = r
+        self.N = 1 << r
+        self.local = 1
+        sb = [(0.0, 0.0), (r2 - bl, 0.0), (0.0, H), (r1 - bl, H)]
+        sl = [(r2 - bl, 0.0), (r2, 0.0), (r1 - bl, H), (r1, H)]
+        el = [(r2, 0.0), (r2 + bl, 0.0), (r1, H), (r1 + bl, H)]
+        eb = [(r2 + bl, 0.0), (L, 0.0), (r1 + bl, H), (L, H)]
+        cs = lambda cx, cy: classify_face(cx, cy, geo, insulated, schottky)
+        ce = lambda cx, cy: classify_face(cx, cy, geo, insulated, False)
+        cp = lambda cx, cy: classify_face(cx, cy, geo, insulated, schottky)
+        self.semi = BlockRegion([(sb, 0, SEMI_ID), (sl, 1, SEMI_BL_ID)], r, [0, 1], cs)
+        self.elec = BlockRegion([(el, 1, ELEC_BL_ID), (eb, 0, ELEC_ID)], r, [1, 0], ce)
+        P = BlockRegion([(sb, 0, SEMI_ID), (sl, 1, SEMI_BL_ID), (el, 1, ELEC_BL_ID), (eb, 0, ELEC_ID)],
+                        r, [0, 1, 3, 2], cp)
+        self.P = P
+        self.pV = P.V
+        self.pmat = P.mat
+        self.pbid = P.bid
+        self.pnbr = P.nbr
+        self.phang = P.hang
+        self.pIJ = P.gij
+        self.NX, self.NY = P.NXg, P.NYg
+        # interface pairs: semiconductor layer cells of the last column with
+        # the electrolyte layer cells of the first (both level r+1)
+        bs = self.semi.block_info[1]
+        be = self.elec.block_info[0]
+        n = bs["n"]
+        self.sigma_semi, self.sigma_elec = [], []
+        for j in range(n):
+            s, e = bs["cell_of"][n - 1, j], be["cell_of"][0, j]
+            if self.semi.bid[s, 1] == INTERFACE:
+                self.sigma_semi.append(int(s))
+                self.sigma_elec.append(int(e))
+        self.semi_sigma_nb = {s: e for s, e in zip(self.sigma_semi, self.sigma_elec)}
+        self.elec_sigma_nb = {e: s for s, e in zip(self.sigma_semi, self.sigma_elec)}
+        self.s2p = np.arange(self.semi.ncell)
+        self.e2p = self.semi.ncell + np.arange(self.elec.ncell)
+
+
+def poisson_dofs_general(mesh, deg):
+    """[D | Phi] DoF indices of every Poisson cell on a mesh with hanging faces.
+
+    Face blocks (deg+1 DoFs) are numbered on first encounter, cells in order
+    and faces 0..3, a conforming interior face sharing its neighbour's block;
+    the two sides of a hanging face keep their own blocks (the coarse face,
+    and each fine face).  Interior RT DoFs and Phi DoFs follow per cell as in
+    poisson_dofs.  Returns (idx, nD, ntotal, face_base[nc, 4]).
+    """
+    nc = len(mesh.pV)
+    nf = deg + 1
+    fb = np.full((nc, 4), -1, dtype=np.int64)
+    cnt = 0
+    for c in range(nc):
+        for f in range(4):
+            e = mesh.pnbr[c, f]
+            if e >= 0 and fb[e, OPP_FACE[f]] >= 0:
+                fb[c, f] = fb[e, OPP_FACE[f]]
+            else:
+                fb[c, f] = cnt
+                cnt += nf
+    nint = 2 * deg * (deg + 1)
+    nD = cnt + nc * nint
+    nphi = (deg + 1) ** 2
+    nx = (deg + 2) * (deg + 1)
+    nloc = 2 * nx + nphi
+    idx = np.empty((nc, nloc), dtype=np.int64)
+    for p in range(nc):
+        ib = cnt + p * nint
+        for b in range(deg + 1):
+            for a in range(deg + 2):
+                li = a + (deg + 2) * b
+                if a == 0:
+                    g = fb[p, 0] + b
+                elif a == deg + 1:
+                    g = fb[p, 1] + b
+                else:
+                    g = ib + (a - 1) + deg * b
+                idx[p, li] = g
+        for b in range(deg + 2):
+            for a in range(deg + 1):
+                li = nx + a + (deg + 1) * b
+                if b == 0:
+                    g = fb[p, 2] + a
+                elif b == deg + 1:
+                    g = fb[p, 3] + a
+                else:
+                    g = ib + deg * (deg + 1) + a + (deg + 1) * (b - 1)
+                idx[p, li] = g
+        for l in range(nphi):
+            idx[p, 2 * nx + l] = nD + p * nphi + l
+    return idx, nD, nD + nc * nphi, fb
+
+
+OPP_FACE = {0: 1, 1: 0, 2: 3, 3: 2}

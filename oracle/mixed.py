@@ -18,7 +18,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import fe
-from .mesh import DIRICHLET, NEUMANN, SCHOTTKY, SEMI_ID, poisson_dofs
+from .mesh import DIRICHLET, NEUMANN, SCHOTTKY, SEMI_BL_ID, SEMI_ID, poisson_dofs, poisson_dofs_general
 
 
 class PoissonFE:
@@ -36,7 +36,10 @@ class PoissonFE:
         self.phi_q, _ = self.dg.tab(self.qpts)                 # [nphi,nq]
         self.fs, self.fw = fe.gauss01(self.nq1)
         self.psi_f = {f: self.rt.tab(fe.face_points(f, self.fs))[0] for f in range(4)}
-        self.idx, self.nD, self.n = poisson_dofs(mesh, self.deg)
+        if getattr(mesh, "local", 0):
+            self.idx, self.nD, self.n, self.fb = poisson_dofs_general(mesh, self.deg)
+        else:
+            self.idx, self.nD, self.n = poisson_dofs(mesh, self.deg)
 
     def cell_parts(self, V):
         x, J = fe.bilinear(V, self.qpts)
@@ -69,12 +72,58 @@ def constrained_dofs(mesh, pfe):
     return np.array(sorted(cons), dtype=np.int64)
 
 
+def hanging_constraints(mesh, pfe):
+    """D DoFs of the fine faces of hanging faces as combinations of the coarse
+    face's DoFs (DoFTools::make_hanging_node_constraints, Poisson.cpp:33-34).
+
+    The constrained RT space has a single-valued normal flux: on subface s of
+    a coarse face, the fine face's flux is the coarse face's flux restricted
+    there.  Face DoFs here are reference normal components at the face's Gauss
+    nodes; the Piola map gives flux density = reference component / face
+    length, and a fine face is half as long, so fine DoF m = 1/2 * the coarse
+    reference component at the subface point (s + G_m) / 2, i.e.
+    sum_j G_j((s + G_m)/2) / 2 * coarse DoF j (G_j: Lagrange basis on the
+    Gauss nodes).  Both sides' reference normal axes point the same physical
+    way (faces 1 | 0 across vertical faces), so no sign.  Returns a list of
+    (slave, masters[K], coefs[K]).
+    """
+    out = []
+    if not getattr(mesh, "local", 0):
+        return out
+    G = pfe.rt.Gn
+    K = len(G)
+    for c, f, s, e, fe_ in mesh.phang:
+        for m in range(K):
+            Lv, _ = fe.lagrange1d(G, [(s + G[m]) / 2.0])
+            out.append((int(pfe.fb[e, fe_] + m), pfe.fb[c, f] + np.arange(K), 0.5 * Lv[:, 0]))
+    return out
+
+
+def reduction(mesh, pfe):
+    """Sparse T (n x nfree) with x = T y: Neumann D.n DoFs are zero and
+    hanging fine-face DoFs follow their coarse face; y runs over the others."""
+    cons = set(constrained_dofs(mesh, pfe).tolist())
+    hang = hanging_constraints(mesh, pfe)
+    slaves = {h[0] for h in hang}
+    free = np.array([i for i in range(pfe.n) if i not in cons and i not in slaves], dtype=np.int64)
+    col = -np.ones(pfe.n, dtype=np.int64)
+    col[free] = np.arange(len(free))
+    R, C, V = list(free), list(range(len(free))), [1.0] * len(free)
+    for sl, ms, cs in hang:
+        for mm, cc in zip(ms, cs):
+            assert col[mm] >= 0
+            R.append(sl)
+            C.append(col[mm])
+            V.append(cc)
+    return sp.csr_matrix((V, (R, C)), shape=(pfe.n, len(free)))
+
+
 def assemble_mixed(mesh, pfe, eps_s, eps_e, lam2):
     V = mesh.pV
     nc = len(V)
     x, JxW, psi, div = pfe.cell_parts(V)
     nrt, nphi, nl = pfe.nrt, pfe.nphi, pfe.nloc
-    eps = np.where(mesh.pmat == SEMI_ID, eps_s, eps_e)
+    eps = np.where((mesh.pmat == SEMI_ID) | (mesh.pmat == SEMI_BL_ID), eps_s, eps_e)
     K = np.zeros((nc, nl, nl))
     K[:, :nrt, :nrt] = np.einsum("icqd,jcqd,cq->cij", psi, psi, JxW) / eps[:, None, None]
     B = np.einsum("icq,jq,cq->cji", div, pfe.phi_q, JxW)  # B[c, phi_j, psi_i] = (phi_j, div psi_i)

@@ -25,8 +25,8 @@ import numpy as np
 import scipy.sparse.linalg as spla
 
 from .ldg import CarrierFE, assemble_ldg, carrier_rhs
-from .mesh import SolarCellMesh
-from .mixed import PoissonFE, assemble_mixed, constrained_dofs, poisson_rhs
+from .mesh import LocalRefinedMesh, SolarCellMesh
+from .mixed import PoissonFE, assemble_mixed, constrained_dofs, poisson_rhs, reduction
 
 THERMAL_VOLTAGE = 0.02585
 ELECTRON_CHARGE = 1.62e-19
@@ -101,7 +101,12 @@ class OracleSolarCell:
         p = self.prm
         k, r = p["k"], p["r"]
         geo = dict(r1=p["r1"], r2=p["r2"], H=p["H"], L=p["L"])
-        self.mesh = SolarCellMesh(r, geo, p["insulated"], p["schottky"])
+        if p.get("local_refine", 0):
+            if p["local_refine"] != 1:
+                raise ValueError("only one level of boundary-layer refinement is restated")
+            self.mesh = LocalRefinedMesh(r, geo, p["bl"], p["insulated"], p["schottky"])
+        else:
+            self.mesh = SolarCellMesh(r, geo, p["insulated"], p["schottky"])
         self.cfe = CarrierFE(k)
         self.pfe = PoissonFE(k, self.mesh)
         m = self.mesh
@@ -110,6 +115,11 @@ class OracleSolarCell:
         self.A_P = assemble_mixed(m, self.pfe, p["eps_s"], p["eps_e"], p["lam2"])
         self.cons = constrained_dofs(m, self.pfe)
         self.free = np.setdiff1d(np.arange(self.pfe.n), self.cons)
+        # with hanging faces the constraints are x = T y (Neumann D.n = 0,
+        # hanging fine faces following their coarse face)
+        self.mesh_local = bool(getattr(m, "local", 0))
+        if self.mesh_local:
+            self.T = reduction(m, self.pfe)
         self.lu = {}
         self.factor()
         nb = self.cfe.nb
@@ -124,12 +134,17 @@ class OracleSolarCell:
         for name in ("n", "p", "r", "o"):
             self.lu[name] = spla.splu(getattr(self, "A_" + name).tocsc(), permc_spec="MMD_AT_PLUS_A",
                                       diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
-        Af = self.A_P[self.free][:, self.free]
+        if self.mesh_local:
+            Af = (self.T.T @ self.A_P @ self.T).tocsr()
+        else:
+            Af = self.A_P[self.free][:, self.free]
         self.lu["P"] = spla.splu(Af.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.1,
                                  options=dict(SymmetricMode=True))
 
     def solve(self, which, b):
         if which == "P":
+            if self.mesh_local:
+                return self.T @ self.lu["P"].solve(self.T.T @ b)
             x = np.zeros(self.pfe.n)
             x[self.free] = self.lu["P"].solve(b[self.free])
             return x
