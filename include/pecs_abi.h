@@ -34,12 +34,21 @@ typedef struct pecs_ctx pecs_ctx;
 
 /* Geometry of the half cell (Grid_Maker::Grid, source/Grid.cpp:11-44,
  * 135-336): semiconductor trapezoid (0,0),(r2,0),(0,H),(r1,H); electrolyte
- * (r2,0),(L,0),(r1,H),(L,H); both refined n_refine times. */
+ * (r2,0),(L,0),(r1,H),(L,H); both refined n_refine times.
+ * local_refine = 1 ("local refinements" of the parameter file,
+ * Grid.cpp:72-111): each region is split at boundary_layer from the r line
+ * into a bulk quad and a boundary-layer quad, and the layer cells are refined
+ * once more, leaving hanging faces between bulk and layer (n_refine 1..7).
+ * Cell order is then deal.II's active-cell order per region: the level-r
+ * bulk cells (Morton), then the level-(r+1) layer cells (Morton); the Poisson
+ * vector uses the face-block layout documented at pecs_get_state. */
 typedef struct {
   double r1, r2, H, L;
   int n_refine;
   int insulated; /* make_Neumann_boundaries, Grid.cpp:376-430 */
   int schottky;  /* make_Schottky_boundaries, Grid.cpp:432-461 */
+  double boundary_layer; /* "boundary layer" width (Grid.cpp:19-40) */
+  int local_refine;      /* 0 or 1 */
 } pecs_mesh;
 
 /* Scaled parameters (ParameterSpace::Parameters, include/Parameters.hpp:25-80,

@@ -36,7 +36,8 @@ ABI_FUNCTIONS = [
 
 class CMesh(ctypes.Structure):
     _fields_ = [("r1", ctypes.c_double), ("r2", ctypes.c_double), ("H", ctypes.c_double), ("L", ctypes.c_double),
-                ("n_refine", ctypes.c_int), ("insulated", ctypes.c_int), ("schottky", ctypes.c_int)]
+                ("n_refine", ctypes.c_int), ("insulated", ctypes.c_int), ("schottky", ctypes.c_int),
+                ("boundary_layer", ctypes.c_double), ("local_refine", ctypes.c_int)]
 
 
 class CParams(ctypes.Structure):
@@ -117,7 +118,8 @@ def default_params():
     geometry), insulated, schottky (boundary passes), dt, theta, penalty,
     mu_{n,p,r,o}, eps_s, eps_e, lam2 (Debye length^2), k_et, k_ht, v_rec_{n,p},
     ne, pe, rinf, oinf (equilibria / doping), phi_bi, phi_app, phi_sch,
-    gen_mode ('ref' | 'config' | 'off'), G0, alpha, srh, ni, tau_n, tau_p.
+    gen_mode ('ref' | 'config' | 'off'), G0, alpha, srh, ni, tau_n, tau_p;
+    optional: local_refine (0 | 1) with bl (boundary-layer width).
     """
     return dict(
         k=1, r=4, r1=0.5, r2=0.5, H=1.0, L=1.0, insulated=False, schottky=False,
@@ -144,7 +146,8 @@ def workload_params(refine, degree, dt):
 
 def to_c(prm):
     """Build (CMesh, CParams) from a parameter dict (keys as oracle.solarcell.default_params)."""
-    m = CMesh(prm["r1"], prm["r2"], prm["H"], prm["L"], prm["r"], int(prm["insulated"]), int(prm["schottky"]))
+    m = CMesh(prm["r1"], prm["r2"], prm["H"], prm["L"], prm["r"], int(prm["insulated"]), int(prm["schottky"]),
+              float(prm.get("bl", 0.0)), int(prm.get("local_refine", 0)))
     p = CParams()
     p.degree = prm["k"]
     p.dt, p.theta, p.penalty = prm["dt"], prm.get("theta", 1.0), prm["penalty"]

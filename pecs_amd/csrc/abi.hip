@@ -237,6 +237,8 @@ GeometrySpec to_geo(const pecs_mesh* m) {
   g.L = m->L;
   g.insulated = m->insulated != 0;
   g.schottky = m->schottky != 0;
+  g.bl = m->boundary_layer;
+  g.local = m->local_refine;
   return g;
 }
 
@@ -244,6 +246,14 @@ void validate(const pecs_mesh* m, const pecs_params* q) {
   if (!m || !q) throw std::runtime_error("null mesh or params");
   if (q->degree < 1 || q->degree > KMAX) throw std::runtime_error("degree must be 1..3");
   if (m->n_refine < 1 || m->n_refine > 10) throw std::runtime_error("n_refine must be 1..10");
+  if (m->local_refine < 0 || m->local_refine > 1)
+    throw std::runtime_error("local_refine must be 0 or 1 (one level of boundary-layer refinement)");
+  if (m->local_refine) {
+    if (m->n_refine > 7) throw std::runtime_error("with local refinement n_refine must be 1..7");
+    if (!(m->boundary_layer > 0 && m->boundary_layer < std::min(m->r1, m->r2) &&
+          std::max(m->r1, m->r2) + m->boundary_layer < m->L))
+      throw std::runtime_error("boundary_layer must be positive and fit inside both regions");
+  }
   if (!(q->dt > 0)) throw std::runtime_error("dt must be positive");
   if (!(m->r1 > 0 && m->r2 > 0 && m->L > m->r1 && m->L > m->r2 && m->H > 0)) throw std::runtime_error("bad geometry");
   for (int i = 0; i < 4; ++i)
@@ -639,8 +649,39 @@ DevRegion make_region_dev(pecs_ctx* c, int reg, const Region& R, const LdgRegion
   d.stride = O.type_stride;
   d.sigma_nb = upload(c, sigma);
   std::vector<int> pb(R.ncell);
-  for (int i = 0; i < R.ncell; ++i) pb[i] = F.cell_base[R.ci[i] + R.N * R.cj[i]];
+  for (int i = 0; i < R.ncell; ++i) pb[i] = F.cell_base[R.ci[i] + (size_t)R.NXg * R.cj[i]];
   d.perm_base = upload(c, pb);
+  if (!O.hang.empty()) {  // couplings across hanging faces, by target cell and by row cell
+    std::vector<double> blk;
+    std::vector<std::vector<std::pair<int, int>>> byx(R.ncell), bym(R.ncell);
+    for (size_t h = 0; h < O.hang.size(); ++h) {
+      const LdgHangBlock& hb = O.hang[h];
+      blk.insert(blk.end(), hb.B.begin(), hb.B.end());
+      byx[hb.X].push_back({hb.M, (int)h});
+      bym[hb.M].push_back({hb.X, (int)h});
+    }
+    auto csr = [&](const std::vector<std::vector<std::pair<int, int>>>& L, const int*& cell, const int*& ptr,
+                   const int*& other, const int*& bi) {
+      std::vector<int> vc, vp{0}, vo, vb;
+      for (int x = 0; x < R.ncell; ++x) {
+        if (L[x].empty()) continue;
+        vc.push_back(x);
+        for (const auto& pr : L[x]) {
+          vo.push_back(pr.first);
+          vb.push_back(pr.second);
+        }
+        vp.push_back((int)vo.size());
+      }
+      cell = upload(c, vc);
+      ptr = upload(c, vp);
+      other = upload(c, vo);
+      bi = upload(c, vb);
+      return (int)vc.size();
+    };
+    d.nhx = csr(byx, d.hx_cell, d.hx_ptr, d.hx_src, d.hx_blk);
+    d.nhm = csr(bym, d.hm_cell, d.hm_ptr, d.hm_dst, d.hm_blk);
+    d.hblk = upload(c, blk);
+  }
   // Wave-uniform cell lists for the Schur kernels, kept when the padding to
   // whole waves adds at most max(ncell, 2048) cells (meshes with few
   // operator types; on conic meshes every cell has its own type and the
@@ -1147,6 +1188,33 @@ int create_impl(const pecs_mesh* mesh, const pecs_params* params, int B, pecs_ct
           mcoef[2 * id + e] = sf * gw[mm];
         }
       }
+    P.own = upload(c.get(), op.own);
+    if (!op.hang.empty()) {  // hanging faces, grouped by coarse cell
+      std::vector<PoissonOps::HangMult> hs = op.hang;
+      std::stable_sort(hs.begin(), hs.end(), [](const PoissonOps::HangMult& a, const PoissonOps::HangMult& b) {
+        return a.coarse < b.coarse;
+      });
+      std::vector<int> hc, hf, hsub, hmb, cells, ptr{0};
+      for (size_t q = 0; q < hs.size(); ++q) {
+        hc.push_back(hs[q].coarse);
+        hf.push_back(hs[q].face);
+        hsub.push_back(hs[q].sub);
+        hmb.push_back(hs[q].mbase);
+        if (q + 1 == hs.size() || hs[q + 1].coarse != hs[q].coarse) {
+          cells.push_back(hs[q].coarse);
+          ptr.push_back((int)q + 1);
+        }
+      }
+      P.nhang = (int)hs.size();
+      P.nhc = (int)cells.size();
+      P.h_cell = upload(c.get(), hc);
+      P.h_face = upload(c.get(), hf);
+      P.h_sub = upload(c.get(), hsub);
+      P.h_mbase = upload(c.get(), hmb);
+      P.hc_cell = upload(c.get(), cells);
+      P.hc_ptr = upload(c.get(), ptr);
+      P.hE = upload(c.get(), op.hE);
+    }
     P.mult_perm = upload(c.get(), mperm);
     P.mcell = upload(c.get(), mcell);
     P.mloc = upload(c.get(), mloc);
@@ -1264,6 +1332,8 @@ int pecs_set_dof_layout(pecs_ctx* ctx, int layout) {
   try {
     if (layout != PECS_LAYOUT_OWN && layout != PECS_LAYOUT_DEALII)
       throw std::runtime_error("unknown DoF layout (PECS_LAYOUT_OWN or PECS_LAYOUT_DEALII)");
+    if (layout == PECS_LAYOUT_DEALII && ctx->mesh.local)
+      throw std::runtime_error("the deal.II layout is not available on locally refined meshes");
     if (layout == PECS_LAYOUT_DEALII && !ctx->dii)
       ctx->dii = std::make_unique<DealiiPoisson>(dealii_poisson(ctx->mesh, ctx->L, *ctx->T));
     ctx->layout = layout;
@@ -1279,6 +1349,7 @@ int pecs_poisson_layout_convert(const pecs_mesh* mesh, const pecs_params* params
   try {
     if (!mesh || !params || !in || !out) throw std::runtime_error("null argument");
     validate(mesh, params);
+    if (mesh->local_refine) throw std::runtime_error("the deal.II layout is not available on locally refined meshes");
     const SolarCellMesh m = make_solarcell_mesh(mesh->n_refine, to_geo(mesh));
     const RefTables T(params->degree);
     const PoissonLayout L = poisson_layout(m, params->degree - 1);
@@ -1670,6 +1741,7 @@ double* grid_buffer(pecs_ctx* c, int G) {
 
 int pecs_sample_grid(pecs_ctx* ctx, int i, int G, double* out) {
   PECS_TRY(ctx, {
+    if (ctx->mesh.local) throw std::runtime_error("the density grid needs a uniformly refined mesh");
     if (i < 0 || i >= ctx->B) throw std::runtime_error("instance index out of range");
     if (!out) throw std::runtime_error("pecs_sample_grid: null output");
     double* g = grid_buffer(ctx, G);
@@ -1685,6 +1757,7 @@ int pecs_sample_grid(pecs_ctx* ctx, int i, int G, double* out) {
 
 int pecs_set_state_from_grid(pecs_ctx* ctx, int i, int G, const double* grid) {
   PECS_TRY(ctx, {
+    if (ctx->mesh.local) throw std::runtime_error("the density grid needs a uniformly refined mesh");
     if (i < 0 || i >= ctx->B) throw std::runtime_error("instance index out of range");
     if (!grid) throw std::runtime_error("pecs_set_state_from_grid: null grid");
     double* g = grid_buffer(ctx, G);

@@ -25,10 +25,37 @@ struct NDBuilder {
   std::vector<NDNode> nodes;  // postorder
   NDBuilder(const BlockSPD& s, int l) : S(s), leaf(l) {}
 
+  // Width of a separator starting at column (row) `mid` of the rectangle
+  // that leaves no coupling between the part before it and the part after
+  // it: the farthest reach past mid of any coupling from a cell before mid.
+  int reach(int i0, int i1, int j0, int j1, int mid, bool vertical) const {
+    const int no = S.noff();
+    int maxo = 0;
+    for (int o = 0; o < no; ++o) maxo = std::max(maxo, std::abs(vertical ? S.offi[o] : S.offj[o]));
+    int w = 1;
+    const int lo = std::max(vertical ? i0 : j0, mid - maxo);
+    for (int t = lo; t < mid; ++t) {
+      const int u0 = vertical ? j0 : i0, u1 = vertical ? j1 : i1;
+      for (int u = u0; u < u1; ++u) {
+        const int xi = vertical ? t : u, xj = vertical ? u : t;
+        const int X = xi + S.NXc * xj;
+        if (S.ndof[X] == 0) continue;
+        for (int o = 0; o < no; ++o) {
+          const int yi = xi + S.offi[o], yj = xj + S.offj[o];
+          if (yi < i0 || yi >= i1 || yj < j0 || yj >= j1) continue;
+          if (S.blk[(size_t)X * no + o].empty() || S.ndof[yi + S.NXc * yj] == 0) continue;
+          const int p = vertical ? yi : yj;
+          if (p >= mid) w = std::max(w, p - mid + 1);
+        }
+      }
+    }
+    return w;
+  }
+
   int build(int i0, int i1, int j0, int j1) {
-    const int ni = i1 - i0, nj = j1 - j0, w = S.sep_width;
+    const int ni = i1 - i0, nj = j1 - j0, w0 = S.sep_width > 0 ? S.sep_width : 2;
     if (ni <= 0 || nj <= 0) return -1;
-    if (ni * nj <= leaf || (ni <= w && nj <= w)) {
+    if (ni * nj <= leaf || (ni <= w0 && nj <= w0)) {
       NDNode n;
       for (int j = j0; j < j1; ++j)
         for (int i = i0; i < i1; ++i) n.cells.push_back(i + S.NXc * j);
@@ -37,21 +64,23 @@ struct NDBuilder {
     }
     std::vector<int> kids;
     NDNode sep;
-    if (ni >= nj && ni > w) {
-      const int mid = i0 + (ni - w) / 2;
+    if (ni >= nj && ni > w0) {
+      const int mid = i0 + (ni - w0) / 2;
+      const int w = S.sep_width > 0 ? w0 : reach(i0, i1, j0, j1, mid, true);
       int a = build(i0, mid, j0, j1);
-      int b = build(mid + w, i1, j0, j1);
+      int b = build(std::min(mid + w, i1), i1, j0, j1);
       if (a >= 0) kids.push_back(a);
       if (b >= 0) kids.push_back(b);
       for (int j = j0; j < j1; ++j)
-        for (int i = mid; i < mid + w; ++i) sep.cells.push_back(i + S.NXc * j);
+        for (int i = mid; i < std::min(mid + w, i1); ++i) sep.cells.push_back(i + S.NXc * j);
     } else {
-      const int mid = j0 + (nj - w) / 2;
+      const int mid = j0 + (nj - w0) / 2;
+      const int w = S.sep_width > 0 ? w0 : reach(i0, i1, j0, j1, mid, false);
       int a = build(i0, i1, j0, mid);
-      int b = build(i0, i1, mid + w, j1);
+      int b = build(i0, i1, std::min(mid + w, j1), j1);
       if (a >= 0) kids.push_back(a);
       if (b >= 0) kids.push_back(b);
-      for (int j = mid; j < mid + w; ++j)
+      for (int j = mid; j < std::min(mid + w, j1); ++j)
         for (int i = i0; i < i1; ++i) sep.cells.push_back(i + S.NXc * j);
     }
     sep.children = kids;

@@ -1154,9 +1154,7 @@ __device__ __forceinline__ int face_of_local(int i, int& m) {
 
 template <int K>
 __device__ __forceinline__ bool owns_face(const DevPoisson& P, int Kc, int f) {
-  if (f == 0 || f == 2) return true;
-  if (f == 1) return P.pI[Kc] == P.NX - 1;
-  return P.pJ[Kc] == P.NY - 1;
+  return (P.own[Kc] >> f) & 1;
 }
 
 template <int K>
@@ -1500,6 +1498,166 @@ inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
     default: return (int)hipErrorInvalidValue;      \
   }
 
+// ------------------------------------------------------------------------
+// Hanging faces of boundary-layer refinement
+// ------------------------------------------------------------------------
+// The Schur kernels leave out the couplings across hanging faces (those faces
+// are absent from the neighbour lists); these kernels add them afterwards, in
+// FP64, one thread per cell that has such a coupling.  Both are linear in the
+// terms they add, so the sum equals the one over all couplings.
+template <int K, int NRHS>
+__global__ __launch_bounds__(64) void k_schur_rhs_hang(DevRegion R, const double* __restrict__ b1,
+                                                       const double* __restrict__ b2, double mu1, double mu2,
+                                                       double* __restrict__ g1, double* __restrict__ g2, int64_t sb,
+                                                       int64_t sg) {
+  constexpr int NB = Sz<K>::NB;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R.nhx) return;
+  {
+    const int64_t i = blockIdx.y;
+    b1 += i * sb;
+    b2 += i * sb;
+    g1 += i * sg;
+    g2 += i * sg;
+  }
+  double a1[NB], a2[NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) a1[a] = a2[a] = 0.0;
+  for (int e = R.hx_ptr[t]; e < R.hx_ptr[t + 1]; ++e) {
+    const int M = R.hx_src[e];
+    const double* B = R.hblk + (size_t)R.hx_blk[e] * 2 * NB * NB;
+    const int ty = R.type[M];
+    double w1[2 * NB], w2[2 * NB];
+    schur_w<K>(R, b1, mu1, M, ty, w1);
+    if (NRHS > 1) schur_w<K>(R, b2, mu2, M, ty, w2);
+    for (int i = 0; i < 2 * NB; ++i) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        a1[a] += B[i * NB + a] * w1[i];
+        if (NRHS > 1) a2[a] += B[i * NB + a] * w2[i];
+      }
+    }
+  }
+  const int pb = R.perm_base[R.hx_cell[t]];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    g1[pb + a] += a1[a];
+    if (NRHS > 1) g2[pb + a] += a2[a];
+  }
+}
+
+template <int K, int NRHS>
+__global__ __launch_bounds__(64) void k_schur_recover_hang(DevRegion R, const double* __restrict__ x1,
+                                                           const double* __restrict__ x2, double mu1, double mu2,
+                                                           double* __restrict__ o1, double* __restrict__ o2,
+                                                           int64_t sx, int64_t so) {
+  constexpr int NB = Sz<K>::NB;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= R.nhm) return;
+  {
+    const int64_t i = blockIdx.y;
+    x1 += i * sx;
+    x2 += i * sx;
+    o1 += i * so;
+    o2 += i * so;
+  }
+  const int M = R.hm_cell[t];
+  const size_t nc = R.ncell;
+  double s1[2 * NB], s2[2 * NB];
+#pragma unroll
+  for (int i = 0; i < 2 * NB; ++i) s1[i] = s2[i] = 0.0;
+  for (int e = R.hm_ptr[t]; e < R.hm_ptr[t + 1]; ++e) {
+    const double* B = R.hblk + (size_t)R.hm_blk[e] * 2 * NB * NB;
+    const int px = R.perm_base[R.hm_dst[e]];
+    double v1[NB], v2[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      v1[a] = x1[px + a];
+      v2[a] = (NRHS > 1) ? x2[px + a] : 0.0;
+    }
+    for (int i = 0; i < 2 * NB; ++i) {
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        s1[i] += B[i * NB + a] * v1[a];
+        if (NRHS > 1) s2[i] += B[i * NB + a] * v2[a];
+      }
+    }
+  }
+  const double* Minv = R.table + (size_t)R.type[M] * R.stride;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        t1 += Minv[a * NB + q] * s1[h * NB + q];
+        if (NRHS > 1) t2 += Minv[a * NB + q] * s2[h * NB + q];
+      }
+      const size_t off = (size_t)h * nc * NB + (size_t)M * NB + a;
+      o1[off] -= mu1 * t1;
+      if (NRHS > 1) o2[off] -= mu2 * t2;
+    }
+}
+
+// H right-hand side: the coarse side's contribution to the fine faces'
+// multipliers, one thread per (hanging pair, multiplier DoF)
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_face_hang(DevPoisson P, const double* __restrict__ y,
+                                                          double* __restrict__ r, int64_t sy, int64_t sr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.nhang * K) return;
+  y += blockIdx.y * sy;
+  r += blockIdx.y * sr;
+  const int h = t / K, m = t % K;
+  const int Kc = P.h_cell[h], f = P.h_face[h], sub = P.h_sub[h];
+  const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) s += P.hE[(sub * K + m) * K + j] * y[(size_t)Kc * P.nloc + rt_face<K>(f, j)];
+  r[P.mult_perm[P.h_mbase[h] + m]] += sf * s;
+}
+
+// Recovery: x_K -= A_K^{-1} E_K^T lambda over the coarse cell's hanging faces
+// (with the same write rules as k_poisson_recover)
+template <int K>
+__global__ __launch_bounds__(64) void k_poisson_recover_hang(DevPoisson P, const double* __restrict__ lam,
+                                                             double* __restrict__ out, int64_t sl, int64_t so) {
+  constexpr int NL = Sz<K>::NL, NRT = Sz<K>::NRT;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.nhc) return;
+  lam += blockIdx.y * sl;
+  out += blockIdx.y * so;
+  const int Kc = P.hc_cell[t];
+  double e[4 * K];
+#pragma unroll
+  for (int i = 0; i < 4 * K; ++i) e[i] = 0.0;
+  for (int q = P.hc_ptr[t]; q < P.hc_ptr[t + 1]; ++q) {
+    const int f = P.h_face[q], sub = P.h_sub[q], mb = P.h_mbase[q];
+    const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+    for (int m = 0; m < K; ++m) {
+      const double l = sf * lam[P.mult_perm[mb + m]];
+#pragma unroll
+      for (int j = 0; j < K; ++j) e[f * K + j] += P.hE[(sub * K + m) * K + j] * l;
+    }
+  }
+  const double* tb = P.table + (size_t)P.type[Kc] * P.stride;
+  const int* id = P.idx + (size_t)Kc * NL;
+  for (int i = 0; i < NL; ++i) {
+    if (i < NRT) {
+      int m;
+      const int fc = face_of_local<K>(i, m);
+      if (fc >= 0 && (!owns_face<K>(P, Kc, fc) || P.pbid[4 * (size_t)Kc + fc] == 2)) continue;
+    }
+    double c = 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int j = 0; j < K; ++j) c += tb[i * NL + rt_face<K>(f, j)] * e[f * K + j];
+    out[id[i]] -= c;
+  }
+}
+
 int launch_rhs_carrier(int k, bool semi, const DevRegion& R, const DevRegion& O, const DevPoisson& P,
                        const DevTabs& T, const DevPhys* phs, const double* u1, const double* u2, const double* ou1,
                        const double* ou2, const double* pvec, double* b1, double* b2, hipStream_t s, int nb,
@@ -1516,7 +1674,7 @@ int launch_rhs_carrier(int k, bool semi, const DevRegion& R, const DevRegion& O,
   return (int)hipGetLastError();
 }
 
-int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, double mu1, double mu2,
+static int launch_schur_rhs_main(int k, const DevRegion& R, const double* b1, const double* b2, double mu1, double mu2,
                      double* g1, double* g2, int nrhs, hipStream_t s, int nb, int64_t sb, int64_t sg) {
 #define PECS_SCHUR_RHS(UU, SS, NCELL)                                                                       \
   {                                                                                                         \
@@ -1544,7 +1702,7 @@ int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* 
 #undef PECS_SCHUR_RHS
 }
 
-int launch_schur_recover(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
+static int launch_schur_recover_main(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
                          const double* xs2, double mu1, double mu2, double* out1, double* out2, int nrhs,
                          hipStream_t s, int nb, int64_t sb, int64_t sx, int64_t so) {
 #define PECS_RECOVER(UU, NCELL)                                                                                 \
@@ -1741,13 +1899,13 @@ int launch_poisson_cell_from_b(const DevPoisson& P, const double* b, double* y, 
   return (int)hipGetLastError();
 }
 
-int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s, int nb, int64_t sy,
+static int launch_poisson_face_main(const DevPoisson& P, const double* y, double* r, hipStream_t s, int nb, int64_t sy,
                         int64_t sr) {
   k_poisson_face<<<dim3(nblk(P.nmult, 256), nb), 256, 0, s>>>(P, y, r, sy, sr);
   return (int)hipGetLastError();
 }
 
-int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
+static int launch_poisson_recover_main(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
                            hipStream_t s, int nb, int64_t sl, int64_t sy, int64_t so) {
   if (P.table_f && P.ucell) {  // mixed precision
     PECS_DISPATCH_K(k, (k_poisson_recover_f32<KK><<<dim3(nblk(P.nucell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl, sy,
@@ -1761,6 +1919,50 @@ int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const 
     PECS_DISPATCH_K(k, (k_poisson_recover<KK, false><<<dim3(nblk(P.ncell, 64), nb), 64, 0, s>>>(P, lam, y, out, sl,
                                                                                                   sy, so)));
   }
+  return (int)hipGetLastError();
+}
+
+int launch_schur_rhs(int k, const DevRegion& R, const double* b1, const double* b2, double mu1, double mu2,
+                     double* g1, double* g2, int nrhs, hipStream_t s, int nb, int64_t sb, int64_t sg) {
+  const int rc = launch_schur_rhs_main(k, R, b1, b2, mu1, mu2, g1, g2, nrhs, s, nb, sb, sg);
+  if (rc || R.nhx == 0) return rc;
+  const dim3 g(nblk(R.nhx, 64), nb);
+  if (nrhs == 2) {
+    PECS_DISPATCH_K(k, (k_schur_rhs_hang<KK, 2><<<g, 64, 0, s>>>(R, b1, b2, mu1, mu2, g1, g2, sb, sg)));
+  } else {
+    PECS_DISPATCH_K(k, (k_schur_rhs_hang<KK, 1><<<g, 64, 0, s>>>(R, b1, b1, mu1, mu1, g1, g1, sb, sg)));
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_schur_recover(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
+                         const double* xs2, double mu1, double mu2, double* out1, double* out2, int nrhs,
+                         hipStream_t s, int nb, int64_t sb, int64_t sx, int64_t so) {
+  const int rc = launch_schur_recover_main(k, R, b1, b2, xs1, xs2, mu1, mu2, out1, out2, nrhs, s, nb, sb, sx, so);
+  if (rc || R.nhm == 0) return rc;
+  const dim3 g(nblk(R.nhm, 64), nb);
+  if (nrhs == 2) {
+    PECS_DISPATCH_K(k, (k_schur_recover_hang<KK, 2><<<g, 64, 0, s>>>(R, xs1, xs2, mu1, mu2, out1, out2, sx, so)));
+  } else {
+    PECS_DISPATCH_K(k, (k_schur_recover_hang<KK, 1><<<g, 64, 0, s>>>(R, xs1, xs1, mu1, mu1, out1, out1, sx, so)));
+  }
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_face(const DevPoisson& P, const double* y, double* r, hipStream_t s, int nb, int64_t sy,
+                        int64_t sr) {
+  const int rc = launch_poisson_face_main(P, y, r, s, nb, sy, sr);
+  if (rc || P.nhang == 0) return rc;
+  const int k = (P.nphi == 1) ? 1 : (P.nphi == 4 ? 2 : 3);
+  PECS_DISPATCH_K(k, (k_poisson_face_hang<KK><<<dim3(nblk(P.nhang * k, 64), nb), 64, 0, s>>>(P, y, r, sy, sr)));
+  return (int)hipGetLastError();
+}
+
+int launch_poisson_recover(int k, const DevPoisson& P, const double* lam, const double* y, double* out,
+                           hipStream_t s, int nb, int64_t sl, int64_t sy, int64_t so) {
+  const int rc = launch_poisson_recover_main(k, P, lam, y, out, s, nb, sl, sy, so);
+  if (rc || P.nhc == 0) return rc;
+  PECS_DISPATCH_K(k, (k_poisson_recover_hang<KK><<<dim3(nblk(P.nhc, 64), nb), 64, 0, s>>>(P, lam, out, sl, so)));
   return (int)hipGetLastError();
 }
 

@@ -41,6 +41,16 @@ struct DevRegion {
   const double* msrc;
   const double* mdir;
   const double* mflux;
+  // hanging faces of boundary-layer refinement (nhx = nhm = 0: none).  Their
+  // couplings B_{M,X} (2nb x nb blocks in hblk: cell M's current rows, cell
+  // X's density columns) are not in nbr / table; they are applied after the
+  // kernels above by target cell X (right-hand side: g_X += B^T W_M; entries
+  // hx_ptr[t] .. hx_ptr[t+1] of target hx_cell[t] name M in hx_src and the
+  // block in hx_blk) and by row cell M (recovery: q_M -= mu Minv_M B u_X).
+  int nhx, nhm;
+  const int *hx_cell, *hx_ptr, *hx_src, *hx_blk;
+  const int *hm_cell, *hm_ptr, *hm_dst, *hm_blk;
+  const double* hblk;
 };
 
 struct DevPoisson {
@@ -69,6 +79,18 @@ struct DevPoisson {
   // as -<psi.n, g> in place of the bias data
   const double* msrc;
   const double* mdir;
+  // per cell: bit f set if the cell writes face f's D DoFs
+  const unsigned char* own;
+  // hanging faces: the coarse side's D.n couples to the fine faces'
+  // multipliers.  Per hanging pair h (nhang): coarse cell, coarse face,
+  // subface and first multiplier of the fine face; the coupling of
+  // multiplier (s, m) to the coarse face's local DoF j is
+  // sign(face) * hE[(s k + m) k + j].  hc_cell / hc_ptr group the pairs by
+  // coarse cell (nhc cells).
+  int nhang, nhc;
+  const int *h_cell, *h_face, *h_sub, *h_mbase;
+  const int *hc_cell, *hc_ptr;
+  const double* hE;
 };
 
 struct DevPhys {

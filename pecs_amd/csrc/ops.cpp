@@ -97,6 +97,15 @@ LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P)
   O.cell.resize(R.ncell);
   std::vector<double> hdiam(R.ncell);
   for (int c = 0; c < R.ncell; ++c) hdiam[c] = cell_diameter(&R.V[8 * c]);
+  // hanging faces: the penalty uses h = min(coarse diameter, diameter of the
+  // fine cell's parent) on both sides (LDG.cpp:366-367: cell->neighbor() is
+  // the fine cells' level-r parent)
+  std::vector<double> hsig(4 * (size_t)R.ncell, 0.0);
+  for (const HangFace& hf : R.hang) {
+    const double sg = P.penalty / std::min(hdiam[hf.coarse], R.pdiam[hf.fine]);
+    hsig[4 * hf.coarse + hf.face] = sg;
+    hsig[4 * hf.fine + hf.fine_face] = sg;
+  }
   int singular = 0;  // the cells are independent: one OpenMP loop, errors raised after it
 #pragma omp parallel for schedule(static) reduction(| : singular)
   for (int c = 0; c < R.ncell; ++c) {
@@ -131,14 +140,20 @@ LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P)
       const int bid = R.bid[4 * c + f];
       const double* ps = &T.fphi[(size_t)f * nb * nfq];
       const double* po = &T.fphi[(size_t)OPP[f] * nb * nfq];
-      const double sig = (e >= 0) ? P.penalty / std::min(h, hdiam[e]) : P.penalty / h;
+      const bool hanging = e < 0 && bid == INTERIOR;
+      const double sig =
+          hanging ? hsig[4 * c + f] : ((e >= 0) ? P.penalty / std::min(h, hdiam[e]) : P.penalty / h);
       for (int q = 0; q < nfq; ++q) {
         const double jw = T.fw[q] * len;
         for (int a = 0; a < nb; ++a) {
           const double pa = ps[a * nfq + q];
           for (int b = 0; b < nb; ++b) {
             const double pbs = ps[b * nfq + q];
-            if (e >= 0) {
+            if (hanging) {  // own-side terms of an interior face, over the whole face
+              L.Bself[a * nb + b] += jw * (0.5 * pa * nx + BETA * pa) * pbs;
+              L.Bself[(nb + a) * nb + b] += jw * (0.5 * pa * ny + BETA * pa) * pbs;
+              L.Dself[a * nb + b] += sig * jw * pa * pbs;
+            } else if (e >= 0) {
               const double pbo = po[b * nfq + q];
               L.Bself[a * nb + b] += jw * (0.5 * pa * nx + BETA * pa) * pbs;
               L.Bself[(nb + a) * nb + b] += jw * (0.5 * pa * ny + BETA * pa) * pbs;
@@ -160,6 +175,54 @@ LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P)
     if (!dense_inverse(nb, L.Mk.data(), L.Minv.data())) singular = 1;
   }
   if (singular) throw std::runtime_error("singular cell mass matrix");
+  // hanging couplings: quadrature on each fine face, the coarse basis at the
+  // matching subface points (s + t) / 2 of its face
+  auto dgq_at = [&](double xi, double eta, std::vector<double>& out) {
+    const auto nodes = equispaced(T.k);
+    out.resize(nb);
+    for (int bb = 0; bb < T.n1; ++bb)
+      for (int aa = 0; aa < T.n1; ++aa) {
+        double vx, dx, vy, dy;
+        lagrange(nodes, aa, xi, vx, dx);
+        lagrange(nodes, bb, eta, vy, dy);
+        out[aa + T.n1 * bb] = vx * vy;
+      }
+  };
+  for (const HangFace& hf : R.hang) {
+    double nx, ny, len;
+    face_normal(&R.V[8 * hf.coarse], hf.face, nx, ny, len);
+    const double sg = hsig[4 * hf.coarse + hf.face];
+    LdgHangBlock cf, fc;  // coarse rows / fine columns, fine rows / coarse columns
+    cf.M = hf.coarse;
+    cf.X = hf.fine;
+    fc.M = hf.fine;
+    fc.X = hf.coarse;
+    for (LdgHangBlock* hb : {&cf, &fc}) {
+      hb->B.assign(2 * nb * nb, 0.0);
+      hb->D.assign(nb * nb, 0.0);
+    }
+    std::vector<double> pc, pf;
+    for (int q = 0; q < nfq; ++q) {
+      double xi, eta;
+      face_point(hf.face, 0.5 * (hf.sub + T.fs[q]), xi, eta);
+      dgq_at(xi, eta, pc);
+      face_point(hf.fine_face, T.fs[q], xi, eta);
+      dgq_at(xi, eta, pf);
+      const double jw = T.fw[q] * 0.5 * len;
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b) {
+          // coarse rows, outward normal n; fine rows, outward normal -n
+          cf.B[a * nb + b] += jw * (0.5 * pc[a] * nx - BETA * pc[a]) * pf[b];
+          cf.B[(nb + a) * nb + b] += jw * (0.5 * pc[a] * ny - BETA * pc[a]) * pf[b];
+          cf.D[a * nb + b] -= sg * jw * pc[a] * pf[b];
+          fc.B[a * nb + b] += jw * (-0.5 * pf[a] * nx - BETA * pf[a]) * pc[b];
+          fc.B[(nb + a) * nb + b] += jw * (-0.5 * pf[a] * ny - BETA * pf[a]) * pc[b];
+          fc.D[a * nb + b] -= sg * jw * pf[a] * pc[b];
+        }
+    }
+    O.hang.push_back(std::move(cf));
+    O.hang.push_back(std::move(fc));
+  }
   O.type_stride = nb * nb + 5 * 2 * nb * nb;
   std::vector<double> rec((size_t)R.ncell * O.type_stride);
 #pragma omp parallel for schedule(static)
@@ -180,7 +243,107 @@ static int offset_index(const BlockSPD& S, int di, int dj) {
   return -1;
 }
 
+// The per-cell coupling lists of the LDG operators: for cell M, (Y, B_{M,Y},
+// D_{M,Y}) over Y = M, its conforming face neighbours and its hanging
+// partners.
+struct Coupling {
+  int Y;
+  const double* B;
+  const double* D;
+};
+static std::vector<std::vector<Coupling>> ldg_couplings(const Region& R, const LdgRegionOps& ops) {
+  const int nb = ops.nb;
+  std::vector<std::vector<Coupling>> adj(R.ncell);
+  for (int M = 0; M < R.ncell; ++M) {
+    const LdgCellOps& L = ops.cell[M];
+    adj[M].push_back({M, L.Bself.data(), L.Dself.data()});
+    for (int f = 0; f < 4; ++f) {
+      const int e = R.nbr[4 * M + f];
+      if (e >= 0) adj[M].push_back({e, &L.Bf[(size_t)f * 2 * nb * nb], &L.Df[(size_t)f * nb * nb]});
+    }
+  }
+  for (const LdgHangBlock& hb : ops.hang) adj[hb.M].push_back({hb.X, hb.B.data(), hb.D.data()});
+  return adj;
+}
+
+// S = D + mu B^T blockdiag(Minv, Minv) B on the region's lattice (positions
+// without a cell hold no DoFs), for meshes with hanging faces.
+static BlockSPD schur_matrix_lattice(const Region& R, const LdgRegionOps& ops, double mu) {
+  const int nb = ops.nb;
+  const auto adj = ldg_couplings(R, ops);
+  BlockSPD S;
+  S.NXc = R.NXg;
+  S.NYc = R.NYg;
+  S.sep_width = 0;
+  const size_t NG = (size_t)S.NXc * S.NYc;
+  S.ndof.assign(NG, 0);
+  auto gid = [&](int c) { return (size_t)R.ci[c] + (size_t)S.NXc * R.cj[c]; };
+  for (int c = 0; c < R.ncell; ++c) S.ndof[gid(c)] = nb;
+  // couplings X ~ Y through a shared M, and their lattice offsets
+  std::map<std::pair<int, int>, int> offs;
+  std::vector<std::vector<int>> nbh(R.ncell);
+  for (int M = 0; M < R.ncell; ++M)
+    for (const Coupling& a : adj[M])
+      for (const Coupling& b : adj[M]) nbh[a.Y].push_back(b.Y);
+  for (int X = 0; X < R.ncell; ++X) {
+    std::sort(nbh[X].begin(), nbh[X].end());
+    nbh[X].erase(std::unique(nbh[X].begin(), nbh[X].end()), nbh[X].end());
+    for (int Y : nbh[X]) offs.emplace(std::make_pair(R.ci[Y] - R.ci[X], R.cj[Y] - R.cj[X]), 0);
+  }
+  for (auto& kv : offs) {
+    kv.second = S.noff();
+    S.offi.push_back(kv.first.first);
+    S.offj.push_back(kv.first.second);
+  }
+  const int no = S.noff();
+  S.blk.assign(NG * no, {});
+  auto block = [&](int X, int Y) -> double* {
+    const int o = offs.at(std::make_pair(R.ci[Y] - R.ci[X], R.cj[Y] - R.cj[X]));
+    return S.blk[gid(X) * no + o].data();
+  };
+  for (int X = 0; X < R.ncell; ++X)
+    for (int Y : nbh[X]) {
+      const int o = offs.at(std::make_pair(R.ci[Y] - R.ci[X], R.cj[Y] - R.cj[X]));
+      S.blk[gid(X) * no + o].assign((size_t)nb * nb, 0.0);
+    }
+  // row X: sum over the cells M coupled to X of B_{M,X}^T Minv_M B_{M,Y}
+  std::vector<std::vector<std::pair<int, const double*>>> into(R.ncell);  // (M, B_{M,X})
+  for (int M = 0; M < R.ncell; ++M)
+    for (const Coupling& a : adj[M]) into[a.Y].push_back({M, a.B});
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int X = 0; X < R.ncell; ++X) {
+    std::vector<double> Pm(2 * nb * nb);
+    for (const auto& mb : into[X]) {
+      const int M = mb.first;
+      const double* BX = mb.second;
+      const LdgCellOps& L = ops.cell[M];
+      for (const Coupling& y : adj[M]) {
+        for (int half = 0; half < 2; ++half)
+          for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b) {
+              double s = 0;
+              for (int t = 0; t < nb; ++t) s += L.Minv[a * nb + t] * y.B[(half * nb + t) * nb + b];
+              Pm[(half * nb + a) * nb + b] = s;
+            }
+        double* blk = block(X, y.Y);
+        for (int a = 0; a < nb; ++a)
+          for (int b = 0; b < nb; ++b) {
+            double s = 0;
+            for (int i = 0; i < 2 * nb; ++i) s += BX[i * nb + a] * Pm[i * nb + b];
+            blk[a * nb + b] += mu * s;
+          }
+      }
+    }
+    for (const Coupling& y : adj[X]) {
+      double* blk = block(X, y.Y);
+      for (int t = 0; t < nb * nb; ++t) blk[t] += y.D[t];
+    }
+  }
+  return S;
+}
+
 BlockSPD schur_matrix(const Region& R, const LdgRegionOps& ops, double mu) {
+  if (!R.hang.empty()) return schur_matrix_lattice(R, ops, mu);
   const int nb = ops.nb, N = R.N;
   BlockSPD S;
   S.NXc = N;
@@ -318,7 +481,82 @@ void ldg_csr(const Region& R, const LdgRegionOps& ops, double mu, double dt, std
         for (int b = 0; b < nb; ++b) C.add(uu(M, a), uu(e, b), L.Df[(size_t)(f * nb + a) * nb + b]);
     }
   }
+  if (!mass_only)
+    for (const LdgHangBlock& hb : ops.hang) {
+      const int M = hb.M, X = hb.X;
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b) {
+          C.add(jx(M, a), uu(X, b), hb.B[a * nb + b]);
+          C.add(jy(M, a), uu(X, b), hb.B[(nb + a) * nb + b]);
+          C.add(uu(X, b), jx(M, a), -hb.B[a * nb + b]);
+          C.add(uu(X, b), jy(M, a), -hb.B[(nb + a) * nb + b]);
+          C.add(uu(M, a), uu(X, b), hb.D[a * nb + b]);
+        }
+    }
   C.finish(rowptr, col, val);
+}
+
+// Multipliers of a mesh with hanging faces: one block of K = deg + 1 on every
+// conforming interior face, Neumann face and fine side of a hanging face
+// (none on the coarse side), owned by the cell that writes the face's D
+// DoFs (O.own) and numbered by the owners' lattice positions.
+static void poisson_multipliers_local(const SolarCellMesh& m, const RefTables& T, PoissonOps& O) {
+  static const int OPPF[4] = {1, 0, 3, 2};
+  const int nc = m.ncellP, kf = T.deg + 1;
+  std::vector<char> coarse_hang(4 * (size_t)nc, 0), fine_hang(4 * (size_t)nc, 0);
+  for (const HangFace& h : m.phang) {
+    coarse_hang[4 * h.coarse + h.face] = 1;
+    fine_hang[4 * h.fine + h.fine_face] = 1;
+  }
+  auto is_mult = [&](int K, int f) {
+    const int b = m.pbid[4 * K + f];
+    if (coarse_hang[4 * K + f]) return false;
+    return fine_hang[4 * K + f] || (b == INTERIOR && m.pnbr[4 * K + f] >= 0) || b == NEUMANN;
+  };
+  const size_t NG = (size_t)m.NX * m.NY;
+  std::vector<int> cell_at(NG, -1);
+  for (int K = 0; K < nc; ++K) cell_at[m.pI[K] + (size_t)m.NX * m.pJ[K]] = K;
+  O.ndof_owner.assign(NG, 0);
+  O.owner_base.assign(NG, 0);
+  O.mult_face.assign(4 * (size_t)nc, -1);
+  int cnt = 0;
+  for (size_t G = 0; G < NG; ++G) {
+    O.owner_base[G] = cnt;
+    const int K = cell_at[G];
+    if (K < 0) continue;
+    const int order[4] = {0, 2, 1, 3};
+    for (int t = 0; t < 4; ++t) {
+      const int f = order[t];
+      if ((O.own[K] >> f & 1) && is_mult(K, f)) {
+        O.mult_face[4 * K + f] = cnt;
+        cnt += kf;
+      }
+    }
+    O.ndof_owner[G] = cnt - O.owner_base[G];
+  }
+  O.nmult = cnt;
+  for (int K = 0; K < nc; ++K)
+    for (int f = 0; f < 4; ++f) {
+      if (!is_mult(K, f) || O.mult_face[4 * K + f] >= 0) continue;
+      const int e = m.pnbr[4 * K + f];
+      if (e < 0 || O.mult_face[4 * e + OPPF[f]] < 0) throw std::runtime_error("multiplier ownership");
+      O.mult_face[4 * K + f] = O.mult_face[4 * e + OPPF[f]];
+    }
+  for (const HangFace& h : m.phang) {
+    const int mb = O.mult_face[4 * h.fine + h.fine_face];
+    if (mb < 0) throw std::runtime_error("hanging face without multipliers");
+    O.hang.push_back({h.coarse, h.face, h.sub, mb});
+  }
+  std::vector<double> gx, gw;
+  gauss01(kf, gx, gw);
+  O.hE.assign(2 * (size_t)kf * kf, 0.0);
+  for (int s = 0; s < 2; ++s)
+    for (int mm = 0; mm < kf; ++mm)
+      for (int j = 0; j < kf; ++j) {
+        double v, d;
+        lagrange(gx, j, 0.5 * (s + gx[mm]), v, d);
+        O.hE[((size_t)s * kf + mm) * kf + j] = 0.5 * gw[mm] * v;
+      }
 }
 
 PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, const RefTables& T, const Params& P) {
@@ -392,8 +630,15 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
     }
   }
   if (singular) throw std::runtime_error("singular local mixed matrix");
-  // multipliers: interior and Neumann faces
   const int kf = T.deg + 1;
+  O.own.assign(nc, 0);
+  for (int K = 0; K < nc; ++K)
+    for (int f = 0; f < 4; ++f)
+      if (f == 0 || f == 2 || m.pnbr[4 * K + f] < 0) O.own[K] |= (unsigned char)(1u << f);
+  if (m.local) {
+    poisson_multipliers_local(m, T, O);
+  } else {
+  // multipliers: interior and Neumann faces
   auto is_mult = [&](int K, int f) {
     int b = m.pbid[4 * K + f];
     return b == INTERIOR || b == NEUMANN;
@@ -436,6 +681,7 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
       O.mult_face[4 * K + f] = base;
     }
   }
+  }
   O.constrained.assign(Lay.n, 0);
   for (int K = 0; K < nc; ++K)
     for (int f = 0; f < 4; ++f)
@@ -452,7 +698,78 @@ PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& Lay, c
   return O;
 }
 
+// H = sum_K E_K A_K^{-1} E_K^T on the lattice of multiplier owners, for
+// meshes with hanging faces (the coarse side of a hanging face couples to the
+// fine faces' multipliers through the dense hE rows).
+static BlockSPD multiplier_matrix_local(const SolarCellMesh& m, const PoissonOps& O, const RefTables& T) {
+  const int nl = O.nloc, kf = T.deg + 1, nc = m.ncellP;
+  std::vector<double> gx, gw;
+  gauss01(kf, gx, gw);
+  BlockSPD S;
+  S.NXc = m.NX;
+  S.NYc = m.NY;
+  S.ndof = O.ndof_owner;
+  S.sep_width = 0;
+  const size_t NG = (size_t)m.NX * m.NY;
+  std::vector<int> owner_of(O.nmult);
+  for (size_t G = 0; G < NG; ++G)
+    for (int t = 0; t < O.ndof_owner[G]; ++t) owner_of[O.owner_base[G] + t] = (int)G;
+  // per cell: its multipliers and their E rows (local DoF, coefficient)
+  struct Row {
+    int mult;
+    std::vector<std::pair<int, double>> e;
+  };
+  std::vector<std::vector<Row>> rows(nc);
+  for (int K = 0; K < nc; ++K)
+    for (int f = 0; f < 4; ++f) {
+      const int bf = O.mult_face[4 * K + f];
+      if (bf < 0) continue;
+      const double sf = (f == 1 || f == 3) ? 1.0 : -1.0;
+      for (int a = 0; a < kf; ++a) rows[K].push_back({bf + a, {{T.rt_face_dof(f, a), sf * gw[a]}}});
+    }
+  for (const PoissonOps::HangMult& h : O.hang) {
+    const double sf = (h.face == 1 || h.face == 3) ? 1.0 : -1.0;
+    for (int a = 0; a < kf; ++a) {
+      Row r{h.mbase + a, {}};
+      for (int j = 0; j < kf; ++j)
+        r.e.push_back({T.rt_face_dof(h.face, j), sf * O.hE[((size_t)h.sub * kf + a) * kf + j]});
+      rows[h.coarse].push_back(r);
+    }
+  }
+  std::map<std::pair<int, int>, int> offs;
+  for (int K = 0; K < nc; ++K)
+    for (const Row& a : rows[K])
+      for (const Row& b : rows[K]) {
+        const int Ga = owner_of[a.mult], Gb = owner_of[b.mult];
+        offs.emplace(std::make_pair(Gb % m.NX - Ga % m.NX, Gb / m.NX - Ga / m.NX), 0);
+      }
+  for (auto& kv : offs) {
+    kv.second = S.noff();
+    S.offi.push_back(kv.first.first);
+    S.offj.push_back(kv.first.second);
+  }
+  const int no = S.noff();
+  S.blk.assign(NG * no, {});
+  for (int K = 0; K < nc; ++K) {
+    const double* Ai = &O.Ainv[(size_t)K * nl * nl];
+    for (const Row& a : rows[K])
+      for (const Row& b : rows[K]) {
+        const int Ga = owner_of[a.mult], Gb = owner_of[b.mult];
+        const int o = offs.at(std::make_pair(Gb % m.NX - Ga % m.NX, Gb / m.NX - Ga / m.NX));
+        auto& blk = S.blk[(size_t)Ga * no + o];
+        const int ra = a.mult - O.owner_base[Ga], cb = b.mult - O.owner_base[Gb], rg = S.ndof[Gb];
+        if (blk.empty()) blk.assign((size_t)S.ndof[Ga] * rg, 0.0);
+        double v = 0.0;
+        for (const auto& x : a.e)
+          for (const auto& y : b.e) v += x.second * y.second * Ai[(size_t)x.first * nl + y.first];
+        blk[(size_t)ra * rg + cb] += v;
+      }
+  }
+  return S;
+}
+
 BlockSPD multiplier_matrix(const SolarCellMesh& m, const PoissonOps& O, const RefTables& T) {
+  if (m.local) return multiplier_matrix_local(m, O, T);
   BlockSPD S;
   S.NXc = m.NX;
   S.NYc = m.NY;

@@ -59,9 +59,19 @@ struct LdgCellOps {
   std::vector<double> Mk;     // nb x nb mass
 };
 
+// Coupling of cell M's rows to cell X's density across a hanging face
+// (LDG.cpp:338-383, written per cell like the blocks above): B (2nb x nb,
+// M's current rows) and D (nb x nb, M's density rows).  Each hanging pair
+// gives two: coarse -> fine and fine -> coarse.
+struct LdgHangBlock {
+  int M = -1, X = -1;
+  std::vector<double> B, D;
+};
+
 struct LdgRegionOps {
   int nb = 0, ncell = 0;
-  std::vector<LdgCellOps> cell;  // per Morton cell
+  std::vector<LdgCellOps> cell;  // per cell (Bf / Df zero on hanging faces)
+  std::vector<LdgHangBlock> hang;
   // de-duplicated device table: per type Minv | Bself | Bf[4]
   std::vector<int> type;
   std::vector<double> table;
@@ -71,6 +81,9 @@ struct LdgRegionOps {
 LdgRegionOps build_ldg_ops(const Region& R, const RefTables& T, const Params& P);
 
 // Block-sparse SPD matrix on a structured cell grid (input to the Cholesky).
+// sep_width = 0: every nested-dissection separator is made as wide as the
+// couplings across it require (meshes embedded in a lattice with empty
+// positions, whose coupling offsets differ from place to place).
 struct BlockSPD {
   int NXc = 0, NYc = 0;
   std::vector<int> ndof;          // per grid cell gid = i + NXc*j
@@ -108,6 +121,20 @@ struct PoissonOps {
   std::vector<double> table;
   int ntype = 0, type_stride = 0;
   std::vector<char> constrained;  // per global Poisson DoF: Neumann D.n constraint
+  // per cell: bit f set if the cell writes face f's D DoFs (faces 0 and 2,
+  // and faces 1 and 3 without a conforming neighbour)
+  std::vector<unsigned char> own;
+  // hanging faces: the coarse side has no multipliers of its own; its D.n is
+  // tied to the multipliers of the fine faces.  Per hanging pair: coarse
+  // cell, coarse face, subface, first multiplier of the fine face.  The
+  // coupling of multiplier (s, m) to the coarse face's local DoF j is
+  // sign(face) * hE[(s K + m) K + j], hE = w_m / 2 * G_j((s + x_m) / 2) with
+  // the K-point Gauss rule (x, w) and its Lagrange basis G.
+  struct HangMult {
+    int coarse, face, sub, mbase;
+  };
+  std::vector<HangMult> hang;
+  std::vector<double> hE;
 };
 
 PoissonOps build_poisson_ops(const SolarCellMesh& m, const PoissonLayout& L, const RefTables& T, const Params& P);

@@ -626,11 +626,16 @@ int main(int argc, char** argv) {
   int k = argc > 2 ? std::atoi(argv[2]) : 1;
   int leaf = argc > 3 ? std::atoi(argv[3]) : 16;
   int conic = argc > 4 ? std::atoi(argv[4]) : 0;
+  int local = argc > 5 ? std::atoi(argv[5]) : 0;
   GeometrySpec g;
   if (conic) {
     g.r1 = 0.5;
     g.r2 = 0.5 + std::tan(10.0 * M_PI / 180.0);
     g.L = 1.5;
+  }
+  if (local) {  // boundary-layer refinement: lattice-embedded matrices, adaptive separators
+    g.local = 1;
+    g.bl = 0.1;
   }
   Params P;
   P.k = k;

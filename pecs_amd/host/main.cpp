@@ -81,11 +81,14 @@ int run(const Cli& cli) {
   ParameterFile file;
   file.read(cli.prm);
   RunSettings rs = settings_from(file, cli.degree);
-  if (rs.local_refinements > 0 && !cli.ignore_local)
-    throw std::runtime_error("local refinements > 0 (boundary-layer refinement) is not supported; "
+  // one level of boundary-layer refinement runs as the reference's Grid.cpp:72-111;
+  // deeper local refinement is refused
+  if (rs.local_refinements > 1 && !cli.ignore_local)
+    throw std::runtime_error("local refinements > 1 are not supported (one level of boundary-layer refinement is); "
                              "pass --ignore-local-refinement to run on the globally refined mesh");
   Parameters prm;
   prm.mesh = rs.mesh;
+  if (cli.ignore_local) prm.mesh.local_refine = 0;
   prm.p = rs.params;
   prm.char_length = rs.char_length;
   prm.char_density = rs.char_density;

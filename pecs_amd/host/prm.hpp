@@ -183,6 +183,8 @@ inline RunSettings settings_from(const ParameterFile& f, int degree) {
   r.mesh.insulated = f.get_bool("physical", "insulated");
   r.mesh.schottky = f.get_bool("physical", "schottky status");
   r.local_refinements = f.get_integer("computational", "local refinements");
+  r.mesh.boundary_layer = f.get_double("mesh", "boundary layer");
+  r.mesh.local_refine = r.local_refinements > 0 ? r.local_refinements : 0;
   r.t_end = f.get_double("computational", "end time");
   r.t_end_2 = f.get_double("computational", "end time 2");
   r.time_stamps = f.get_integer("computational", "time stamps");

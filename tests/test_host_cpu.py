@@ -166,7 +166,8 @@ def test_dofs_format(tmp_path):
 
 def test_runner_rejects_bad_input(tmp_path):
     f = tmp_path / "in.prm"
-    f.write_text("subsection computational\n  set local refinements = 1\nend\n")
+    # one level of boundary-layer refinement runs; deeper local refinement is refused
+    f.write_text("subsection computational\n  set local refinements = 2\nend\n")
     r = subprocess.run([RUNNER, str(f)], capture_output=True, text=True)
     assert r.returncode == 1 and "local refinements" in r.stderr and "Aborting!" in r.stderr
     r = subprocess.run([RUNNER, str(tmp_path / "missing.prm")], capture_output=True, text=True)

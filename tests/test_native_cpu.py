@@ -40,6 +40,11 @@ CASES = [
     dict(k=2, r=3, r2=0.5 + np.tan(np.radians(10.0)), L=1.5, insulated=True, schottky=True, mu_n=1.3, mu_p=0.7,
          mu_r=2.0, mu_o=0.5, eps_s=2.0, eps_e=3.0, lam2=0.5),
     dict(k=3, r=2, insulated=True, dt=1e-4, penalty=2.0),
+    # boundary-layer local refinement: hanging-face fluxes and the face-block Poisson layout
+    dict(k=1, r=3, local_refine=1, bl=0.05),
+    dict(k=2, r=2, r1=0.3, r2=0.6, L=1.2, local_refine=1, bl=0.1, insulated=True, schottky=True, mu_n=1.3,
+         mu_p=0.7, eps_s=2.0, eps_e=3.0, lam2=0.5, penalty=2.0),
+    dict(k=3, r=2, local_refine=1, bl=0.1, dt=1e-4),
 ]
 
 
@@ -74,7 +79,8 @@ def test_scaling_matches_oracle():
     assert a["recombo_v"][1] == pytest.approx(b["hole_recombo_v"], rel=1e-15)
 
 
-@pytest.mark.parametrize("args", [["4", "1"], ["4", "2", "16", "1"], ["5", "2", "4"]])
+@pytest.mark.parametrize("args", [["4", "1"], ["4", "2", "16", "1"], ["5", "2", "4"], ["3", "2", "16", "1", "1"],
+                                  ["4", "1", "16", "0", "1"]])
 def test_native_selftest(args):
     exe = os.path.join(ROOT, "pecs_amd", "_build", "pecs_selftest")
     if not os.path.exists(exe):
