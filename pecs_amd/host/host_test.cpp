@@ -25,8 +25,9 @@ int main(int argc, char** argv) {
       f.read(argv[2]);
       const RunSettings r = settings_from(f, std::atoi(argv[3]));
       const pecs_params& p = r.params;
-      std::printf("r1=%.17g\nr2=%.17g\nH=%.17g\nL=%.17g\nn_refine=%d\ninsulated=%d\nschottky=%d\n", r.mesh.r1,
-                  r.mesh.r2, r.mesh.H, r.mesh.L, r.mesh.n_refine, r.mesh.insulated, r.mesh.schottky);
+      std::printf("r1=%.17g\nr2=%.17g\nH=%.17g\nL=%.17g\nn_refine=%d\ninsulated=%d\nschottky=%d\nbl=%.17g\n",
+                  r.mesh.r1, r.mesh.r2, r.mesh.H, r.mesh.L, r.mesh.n_refine, r.mesh.insulated, r.mesh.schottky,
+                  r.mesh.boundary_layer);
       std::printf("degree=%d\ndt=%.17g\nt_end=%.17g\nt_end_2=%.17g\ntime_stamps=%d\nrestart=%d\nlocal=%d\n", p.degree,
                   p.dt, r.t_end, r.t_end_2, r.time_stamps, (int)r.restart, r.local_refinements);
       for (int i = 0; i < 4; ++i) std::printf("mu%d=%.17g\n", i, p.mu[i]);

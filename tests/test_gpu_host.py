@@ -94,7 +94,7 @@ def _oracle_params(prm_file, degree):
              eps_e=kv["eps_e"], lam2=kv["lambda2"], k_et=kv["k_et"], k_ht=kv["k_ht"], v_rec_n=kv["v_rec_n"],
              v_rec_p=kv["v_rec_p"], ne=kv["n_e"], pe=kv["p_e"], rinf=kv["r_inf"], oinf=kv["o_inf"],
              phi_bi=kv["phi_bi"], phi_app=kv["phi_app"], phi_sch=kv["phi_sch"], gen_mode="ref", G0=kv["G0"],
-             alpha=kv["alpha"], srh=False)
+             alpha=kv["alpha"], srh=False, local_refine=int(kv["local"]), bl=kv["bl"])
     assert kv["gen_mode"] == 0
     return p
 
@@ -165,3 +165,25 @@ def test_cli_steady_stop(tmp_path):
     _say("[gpu] pecs_run steady-state stop")
     out = _run([prm, "--degree", 1, "--steps", 40, "--steady-tol", 1e30, "--check-every", 3, "--out", tmp_path])
     assert "steady state reached" in out and "3 steps in" in out
+
+
+def test_cli_local_refinement(tmp_path):
+    """The reference's own input_file.prm sets `local refinements = 1` with a
+    boundary layer of 0.1 (input_file.prm:14, 64): pecs_run runs such a file
+    as is, on the locally refined mesh, and matches the oracle there."""
+    degree = 2
+    prm = tmp_path / "input_file.prm"
+    text = PRM.format(restart="false")
+    text = text.replace("  set global refinements = 3\n", "  set global refinements = 3\n  set local refinements  = 1\n")
+    text = text.replace("  set mesh length = 1.2\n", "  set mesh length = 1.2\n  set boundary layer = 0.1\n")
+    prm.write_text(text)
+    _say("[gpu] pecs_run with local refinements = 1")
+    out = _run([prm, "--degree", degree, "--out", tmp_path])
+    steps = int(out.split("output 2: step")[1].split(",")[0])
+    assert (tmp_path / "Poisson-002.vtu").exists()
+    p = _oracle_params(prm, degree)
+    assert p["local_refine"] == 1 and p["bl"] == 0.1
+    st = OracleSolarCell(p).run(steps)
+    for c, name in NAMES.items():
+        d = _read_dofs(tmp_path / (name + ".dofs"))
+        assert _rel(d, st[c]) < 1e-10, (name, _rel(d, st[c]))
