@@ -16,6 +16,10 @@
  *   Poisson vector: [D | Phi] (Poisson.cpp:18-69); D uses a nodal RT basis
  *   (face DoFs: vertical faces j*(NX+1)+i then horizontal faces j*NX+i, k DoFs
  *   each; then 2(k-1)k interior DoFs per cell), Phi is FE_DGQ(k-1) per cell;
+ *   on a locally refined mesh the face blocks are numbered on first encounter
+ *   instead (cells in order, faces 0..3; a conforming face shares its
+ *   neighbour's block, the coarse face and each fine face of a hanging face
+ *   have their own, the fine ones equal to the coarse flux restricted);
  *   pecs_set_dof_layout switches the host-pointer calls to deal.II's basis
  *   and numbering of D.
  * Host-pointer functions copy in and out; *_dev functions take device
@@ -41,7 +45,7 @@ typedef struct pecs_ctx pecs_ctx;
  * once more, leaving hanging faces between bulk and layer (n_refine 1..7).
  * Cell order is then deal.II's active-cell order per region: the level-r
  * bulk cells (Morton), then the level-(r+1) layer cells (Morton); the Poisson
- * vector uses the face-block layout documented at pecs_get_state. */
+ * vector uses the face-block layout described at the top of this file. */
 typedef struct {
   double r1, r2, H, L;
   int n_refine;
