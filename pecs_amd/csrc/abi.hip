@@ -430,9 +430,25 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
             bk = ks;
           }
         }
-        if (trace)
-          std::fprintf(stderr, "[setup] tune n=%d level %d %s: W %d MAXR %d rows %d segment %d (%.3f ms)\n", P.n, l,
-                       fwd ? "fwd" : "bwd", bw, bm, br, bk, best);
+        if (trace) {
+          // the level's factor bytes (MFMA-blocked copy when batched) and nodes
+          double bytes = 0.0;
+          int maxns = 0;
+          for (int q = P.lnoff[l]; q < P.lnoff[l + 1]; ++q) {
+            const int sn = P.lnodes[q];
+            maxns = std::max(maxns, P.ns[sn]);
+            if (nrhs > 2) {
+              const std::vector<int>& b0 = fwd ? P.fblk0 : P.bblk0;
+              const std::vector<int64_t>& bo = fwd ? P.fblk_off : P.bblk_off;
+              bytes += 8.0 * (double)(bo[b0[sn + 1]] - bo[b0[sn]]);
+            }
+          }
+          std::fprintf(stderr,
+                       "[setup] tune n=%d level %d %s: W %d MAXR %d rows %d segment %d (%.3f ms) nodes %d max ns %d"
+                       " factor %.1f MB (%.2f TB/s)\n",
+                       P.n, l, fwd ? "fwd" : "bwd", bw, bm, br, bk, best, P.lnoff[l + 1] - P.lnoff[l], maxns,
+                       bytes / 1e6, bytes / (best * 1e-3) / 1e12);
+        }
         if (fwd) {
           P.fw[l] = bw;
           P.fm[l] = bm;

@@ -895,7 +895,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
   const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
   const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
   const int VL = FWD ? ns : ns + ms;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index through readfirstlane: everything derived from it (rows,
+  // column range, step count) is then scalar and the step loop's control flow
+  // is uniform, so the compiler counts outstanding loads exactly
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ri = lane & 15, kq = lane >> 4;
   const int hg0 = (wave % WPR) * HPW;                    // this wave's first right-hand-side group
   const int first = row0 + (wave / WPR) * MB * 16;       // wave-uniform
@@ -926,12 +929,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
   for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
     for (int hg = 0; hg < NH; ++hg) acc[mb][hg] = v4d{0.0, 0.0, 0.0, 0.0};
-  // vector entry at column col of right-hand side h (interleaved storage)
+  // vector entry at column col of right-hand side h (interleaved storage).
+  // Columns past the rows' last one (col >= VL, in the last step) read the
+  // last column's entry instead: the factor's entries there are zero and the
+  // vector entries are finite, so the products vanish.  The loads then carry
+  // no per-lane branch, and the compiler issues them back to back instead of
+  // waiting on each one to select between it and zero.
+  const int64_t zrow = FWD ? 0 : F.rows_off[s] - ns;
   auto vecv = [&](int col, int h) -> double {
-    if (col >= VL) return 0.0;
-    if (FWD) return F.X[(int64_t)(p0 + col) * NRP + h];
-    if (col < ns) return Y[(int64_t)(p0 + col) * NRP + h];
-    return F.Z[(F.rows_off[s] + (col - ns)) * NRP + h];
+    const int c = min(col, VL - 1);
+    if (FWD) return F.X[(int64_t)(p0 + c) * NRP + h];
+    const double* src = c < ns ? Y + (int64_t)(p0 + c) * NRP : F.Z + (zrow + c) * NRP;
+    return src[h];
   };
   int c0 = (lo >> 4) << 4, cend = hi;
   if (ks > 0) {  // this tile's column segment (ks and the segment start are multiples of 16)
@@ -939,15 +948,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
     cend = min(cend, (tl.w + 1) * ks);
   }
   const int nstep = cend > c0 ? (cend - c0 + 15) >> 4 : 0;
-  auto load = [&](int st, double (&L)[MB][4], double (&Bv)[4][NH]) {
+  // Step st's operands.  Every load is unconditional: a block outside its
+  // stored columns at this step (only with MB > 1) loads its nearest stored
+  // step and is masked to zero where the MFMA uses it (inb), so no load waits
+  // behind a select.
+  auto load = [&](int st, double (&L)[MB][4], double (&Bv)[4][NH], bool (&inb)[MB]) {
     const int cc = c0 + 16 * st;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
       const int idx = (cc - bclo[mb]) >> 4;  // the block's own step (wave-uniform)
-      const bool in = cc >= bclo[mb] && idx < bsteps[mb];
+      inb[mb] = cc >= bclo[mb] && idx < bsteps[mb];
+      // a block past the tile's rows (bsteps 0) reads block 0's first step,
+      // which exists: the wave has rows, so block 0 has at least one step
+      const int ic = min(max(idx, 0), max(bsteps[mb] - 1, 0));
+      const double* src = blk + (bsteps[mb] > 0 ? bbase[mb] + (int64_t)ic * 256 : bbase[0]) + lane;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        L[mb][q] = in ? __builtin_nontemporal_load(blk + bbase[mb] + ((int64_t)idx * 4 + q) * 64 + lane) : 0.0;
+      for (int q = 0; q < 4; ++q) L[mb][q] = __builtin_nontemporal_load(src + q * 64);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -956,23 +972,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
       for (int hg = 0; hg < NH; ++hg) Bv[q][hg] = vecv(col, (hg0 + hg) * 16 + ri);
     }
   };
-  auto mma = [&](const double (&L)[MB][4], const double (&Bv)[4][NH]) {
+  auto mma = [&](const double (&L)[MB][4], const double (&Bv)[4][NH], const bool (&inb)[MB]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int hg = 0; hg < NH; ++hg)
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb)
-          acc[mb][hg] = __builtin_amdgcn_mfma_f64_16x16x4f64(L[mb][q], Bv[q][hg], acc[mb][hg], 0, 0, 0);
+          acc[mb][hg] = __builtin_amdgcn_mfma_f64_16x16x4f64(MB == 1 || inb[mb] ? L[mb][q] : 0.0, Bv[q][hg],
+                                                             acc[mb][hg], 0, 0, 0);
   };
-  // loads run one step ahead of the MFMAs that use them (two register sets)
+  // loads run one step ahead of the MFMAs that use them (two register sets);
+  // past the last step they repeat it (unused), so the loop body has no
+  // conditional loads
   double LA[MB][4], LB[MB][4], BA[4][NH], BB[4][NH];
-  if (nstep > 0) load(0, LA, BA);
-  for (int st = 0; st < nstep; st += 2) {
-    if (st + 1 < nstep) load(st + 1, LB, BB);
-    mma(LA, BA);
-    if (st + 2 < nstep) load(st + 2, LA, BA);
-    if (st + 1 < nstep) mma(LB, BB);
+  bool IA[MB], IB[MB];
+  // (sched_barrier keeps each step's loads ahead of the previous step's MFMAs:
+  // left alone, the scheduler sinks them next to their use)
+  if (nstep > 0) {
+    load(0, LA, BA, IA);
+    for (int st = 0; st < nstep; st += 2) {
+      load(min(st + 1, nstep - 1), LB, BB, IB);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(LA, BA, IA);
+      __builtin_amdgcn_sched_barrier(0);
+      load(min(st + 2, nstep - 1), LA, BA, IA);
+      __builtin_amdgcn_sched_barrier(0);
+      if (st + 1 < nstep) mma(LB, BB, IB);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
