@@ -287,7 +287,8 @@ def batched_leg(args, workload):
                                    "flops": flops, "avg_ms": cms, "achieved": tf, "peak": FP64_PEAK_TFS,
                                    "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS},
            "sweep_tiles_semi": [list(t) for t in g.sweep_config(0)]}
-    pv = profiled_values("carrier_solves_r8k2_batch16_counters.md", out["workload"],
+    prof = f"carrier_solves_r8k2_batch{B}_counters.md"
+    pv = profiled_values(prof, out["workload"],
                          ["mfma_flops_per_solve", "mfma_busy_cycles_per_solve", "batch_traffic_bytes_per_solve"])
     if pv and cms > 0:
         cf = pv["mfma_flops_per_solve"] / (cms * 1e-3) / 1e12
@@ -296,7 +297,7 @@ def batched_leg(args, workload):
             "mfma_busy_frac": pv["mfma_busy_cycles_per_solve"] / (1024 * cms * 1e-3 * 2.4e9),
             "traffic_bytes": pv["batch_traffic_bytes_per_solve"],
             "hbm_frac": pv["batch_traffic_bytes_per_solve"] / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "source": "profiles/carrier_solves_r8k2_batch16_counters.md (rocprofv3 --pmc "
+            "source": f"profiles/{prof} (rocprofv3 --pmc "
                       "SQ_INSTS_VALU_MFMA_MOPS_F64, SQ_VALU_MFMA_BUSY_CYCLES, FETCH_SIZE, WRITE_SIZE) over this "
                       "run's kernel time; busy cycles over 1024 SIMDs at 2.4 GHz"}
     if not args.no_steady:
@@ -551,7 +552,7 @@ def main():
     ap.add_argument("--steady-max-steps", type=int, default=8000)
     ap.add_argument("--steady-check-every", type=int, default=25)
     ap.add_argument("--no-steady", action="store_true", help="skip the time-to-steady-state legs")
-    ap.add_argument("--batch", type=int, default=16, help="also run an I-V sweep of this many bias points (0: skip)")
+    ap.add_argument("--batch", type=int, default=64, help="also run an I-V sweep of this many bias points (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-only", action="store_true", help="run only the batched leg")
     ap.add_argument("--sweep", type=int, default=16,
