@@ -420,6 +420,53 @@ def sweep_leg(args):
             "plain_ms_per_step": [round(m, 4) for m in plain_ms]}
 
 
+def local_refinement_leg(args):
+    """The reference input's mesh with its boundary-layer local refinement
+    (input_file.prm: radii 0.3 / 0.6, boundary layer 0.1, one local
+    refinement) at global refinement args.lr_refine, the measured physics:
+    ms per step (graph-replayed) and the phase times, beside the same
+    geometry refined uniformly one level more (similar DoF count)."""
+    import ctypes
+    import torch
+    import pecs_amd
+    L = pecs_amd.lib()
+    L.pecs_enable_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.pecs_phase_times.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int)]
+    names = ["rhs_semi", "rhs_elec", "schur_transforms", "carrier_cholesky", "recoveries", "poisson_cond",
+             "chol_poisson", "poisson_recover"]
+    res = {"geometry": "r1=0.3, r2=0.6, H=1, L=1 (input_file.prm)", "degree": args.degree, "dt": args.dt}
+    for key, r, local in (("local", args.lr_refine, 1), ("uniform", args.lr_refine + 1, 0)):
+        p = pecs_amd.workload_params(r, args.degree, args.dt)
+        p.update(r1=0.3, r2=0.6, H=1.0, L=1.0)
+        if local:
+            p.update(local_refine=1, bl=0.1)
+        t0 = time.perf_counter()
+        g = pecs_amd.SolarCellGPU(p)
+        setup = time.perf_counter() - t0
+        g.initial_state()
+        g.step(args.warmup)
+        assert L.pecs_enable_timing(g.ctx, 1) == 0
+        g.step(args.lr_steps)
+        ph = (ctypes.c_double * 8)()
+        nst = ctypes.c_int()
+        L.pecs_phase_times(g.ctx, ph, 8, ctypes.byref(nst))
+        L.pecs_enable_timing(g.ctx, 0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.step(args.lr_steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.lr_steps * 1e3
+        U = 2 * g.n_semi + 2 * g.n_elec + g.n_poisson
+        res[key] = {"global_refinement": r, "local_refinement": local, "boundary_layer": 0.1 if local else 0.0,
+                    "n_carrier_semi": g.n_semi, "n_carrier_elec": g.n_elec, "n_poisson": g.n_poisson,
+                    "setup_s": round(setup, 2), "ms_per_step": ms, "dof_updates_per_s": U / (ms * 1e-3),
+                    "phase_ms_per_step": {n: round(ph[i] / max(nst.value, 1), 4) for i, n in enumerate(names)}}
+        g.close()
+        print(f"[bench] local refinement leg: {key}: {res[key]}", file=sys.stderr, flush=True)
+    return res
+
+
 def warm_start_leg(args):
     """Learned warm start: steady states of args.ws_train_groups x 16 draws
     (refinement and cone angle per group, drawn from --ws-refines; applied
@@ -581,6 +628,9 @@ def main():
     ap.add_argument("--ws-fallback", type=float, default=1e-2,
                     help="restart a warm-started instance cold if its predicted residual (relative density "
                          "change over the first step) is above this")
+    ap.add_argument("--lr-refine", type=int, default=6,
+                    help="global refinement of the local-refinement leg (0: skip)")
+    ap.add_argument("--lr-steps", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernel by kernel instead of replaying captured step graphs (counter collection)")
     ap.add_argument("--copy-calibration-gib", type=float, default=0.0,
@@ -751,6 +801,8 @@ def main():
         out["sweep"] = sweep_leg(args)
     if not args.no_warm_start:
         out["warm_start"] = warm_start_leg(args)
+    if args.lr_refine > 0:
+        out["local_refinement"] = local_refinement_leg(args)
     # every speed-up against the plain path: one instance alone, graph-replayed
     plain = out.get("time_to_steady_state", {}).get("gpu", {})
     if plain.get("reached"):
