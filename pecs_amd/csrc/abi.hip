@@ -292,10 +292,10 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
     const int nrp = mfma_rhs_pad(nrhs);
     if ((nf != 3 && nf != 4) || (m != 1 && m != 2 && m != 4) || (split != 0 && split != 1) || rows < 16 ||
         rows > mfma_tile_rows(nrp, m, split) || rows % 16 != 0 || ks < 0 || (ks > 0 && (ks < 16 || ks % 16 != 0)) ||
-        !mfma_shape_ok(nrp, m))
+        !mfma_shape_ok(nrp, m, split))
       throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks[,split] (m 1, 2 or 4 -- 1 or 2 above 64 "
-                               "right-hand sides -- rows a multiple of 16 up to the rows a workgroup covers, 64 m "
-                               "or with split 1 fewer, ks 0 or a multiple of 16)");
+                               "right-hand sides unless split -- rows a multiple of 16 up to the rows a workgroup "
+                               "covers, 64 m or with split 1 fewer, ks 0 or a multiple of 16)");
     const int w = split && nrp > 16 ? -1 : 0;
     for (int l = 0; l < P.nlevels; ++l) {
       P.fw[l] = P.bw[l] = w;
@@ -372,7 +372,7 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
               const int tr = mfma_tile_rows(nrp, m, split);
               for (int rows : {tr, tr / 2})
                 for (int ks : {0, 4096, 2048, 1024, 512, 256, 128})
-                  if (rows >= 16 && mfma_shape_ok(nrp, m)) cand.push_back({split ? -1 : 0, m, rows, ks});
+                  if (rows >= 16 && mfma_shape_ok(nrp, m, split)) cand.push_back({split ? -1 : 0, m, rows, ks});
             }
         } else {
           for (int w : {8, 16, 32, 64})

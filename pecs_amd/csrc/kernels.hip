@@ -882,7 +882,7 @@ __global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restri
 // fewer registers per wave and so more waves in flight.
 constexpr int mfma_waves_per_eu(int nrp, int mb, int hpw) {
   if (hpw == nrp / 16) return mb == 1 && nrp <= 32 ? 4 : 1;
-  return hpw == 1 ? (mb == 1 ? 6 : mb == 2 ? 4 : 2) : 1;
+  return hpw == 1 ? (mb == 1 ? 6 : mb == 2 ? 4 : 2) : (mb == 4 ? 2 : 1);
 }
 
 template <int NRP, int MB, bool FWD, int HPW = NRP / 16>
@@ -1800,7 +1800,7 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
   PECS_MFMA_CASE(128, 1) PECS_MFMA_CASE(128, 2)
   PECS_MFMA_SPLIT(32, 1, 1) PECS_MFMA_SPLIT(32, 2, 1) PECS_MFMA_SPLIT(32, 4, 1)
   PECS_MFMA_SPLIT(64, 1, 1) PECS_MFMA_SPLIT(64, 2, 1) PECS_MFMA_SPLIT(64, 4, 1)
-  PECS_MFMA_SPLIT(128, 1, 2) PECS_MFMA_SPLIT(128, 2, 2)
+  PECS_MFMA_SPLIT(128, 1, 2) PECS_MFMA_SPLIT(128, 2, 2) PECS_MFMA_SPLIT(128, 4, 2)
 #undef PECS_MFMA_SPLIT
 #undef PECS_MFMA_CASE
   return (int)hipErrorInvalidValue;
@@ -1808,7 +1808,10 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
 
 int mfma_rhs_pad(int nrhs) { return nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : nrhs <= 64 ? 64 : 128; }
 
-bool mfma_shape_ok(int nrp, int mb) { return mb == 1 || mb == 2 || (mb == 4 && nrp <= 64); }
+// 4 blocks per wave at 128 right-hand sides only with the groups split over
+// waves (2 groups per wave): unsplit, their accumulators and operands would
+// not fit the register file
+bool mfma_shape_ok(int nrp, int mb, bool split) { return mb == 1 || mb == 2 || (mb == 4 && (nrp <= 64 || split)); }
 
 int mfma_tile_rows(int nrp, int mb, bool split) {
   if (!split || nrp <= 16) return 64 * mb;

@@ -193,7 +193,7 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
 // sweep is built for nrp padded columns with mb 16-row blocks per wave.
 constexpr int MAX_MFMA_RHS = 128;
 int mfma_rhs_pad(int nrhs);
-bool mfma_shape_ok(int nrp, int mb);
+bool mfma_shape_ok(int nrp, int mb, bool split = false);
 // rows a workgroup of the MFMA sweep covers: 64 mb, or with the right-hand
 // sides split over waves (sweep width -1) 64 mb / (waves per row block)
 int mfma_tile_rows(int nrp, int mb, bool split);

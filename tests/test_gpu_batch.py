@@ -104,9 +104,10 @@ SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,
           "1,32,0,1", "2,64,128,1", "4,128,0,1"]  # split: right-hand-side groups over waves
 
 
-# m = 4 is not built for 128 padded right-hand sides (the accumulators and
-# operands of 4 blocks would not fit the register file)
-WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512", "1,16,0,1", "2,32,256,1"]
+# m = 4 is built for 128 padded right-hand sides only with the groups split
+# over waves (unsplit, the accumulators and operands of 4 blocks would not fit
+# the register file)
+WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512", "1,16,0,1", "2,32,256,1", "4,64,0,1", "4,64,256,1"]
 
 
 @pytest.mark.parametrize("B,r", [(3, 4), (9, 3), (40, 3)])
