@@ -101,13 +101,15 @@ def test_batch_rejects_mismatched_matrix_parameters(seed):
 # tiles and their reduction) run here against the oracle.  Split tiles add
 # their segments in a fixed order: they agree with unsplit ones to rounding.
 SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,128,256", "1,32,512",
-          "1,32,0,1", "2,64,128,1", "4,128,0,1"]  # split: right-hand-side groups over waves
+          "1,32,0,1", "2,64,128,1", "4,128,0,1",  # split 1: right-hand-side groups over waves
+          "1,64,0,2", "1,64,256,2"]  # split 2: the vector operand staged in LDS (more than 16 right-hand sides)
 
 
 # m = 4 is built for 128 padded right-hand sides only with the groups split
 # over waves (unsplit, the accumulators and operands of 4 blocks would not fit
 # the register file)
-WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512", "1,16,0,1", "2,32,256,1", "4,64,0,1", "4,64,256,1"]
+WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512", "1,16,0,1", "2,32,256,1", "4,64,0,1", "4,64,256,1",
+        "1,64,0,2", "1,64,128,2"]
 
 
 @pytest.mark.parametrize("B,r", [(3, 4), (9, 3), (40, 3)])
@@ -127,7 +129,9 @@ def test_forced_sweep_shapes(B, r, seed):
             del os.environ["PECS_SWEEP_FORCE"]
         f = [int(x) for x in shape.split(",")]
         m, rows = f[0], f[1]
-        w = -1 if len(f) > 3 and f[3] and 2 * B > 16 else 0
+        w = -1 if len(f) > 3 and f[3] == 1 and 2 * B > 16 else 0
+        if len(f) > 3 and f[3] == 2 and 2 * B > 16:
+            w = -2
         cfg = g.sweep_config(0)
         assert all(t[0] == w and t[1] == m and t[2] == rows and t[3] == w and t[4] == m and t[5] == rows
                    for t in cfg), cfg
@@ -147,7 +151,8 @@ def test_forced_sweep_shapes(B, r, seed):
 def test_forced_sweep_shape_rejected():
     import os
     import pecs_amd
-    for B, shape in ((3, "3,64,0"), (40, "4,256,0"), (9, "1,64,0,1")):  # split covers 32 rows at 32 columns
+    # split covers 32 rows at 32 columns; the LDS-staged sweep takes m 1 and at most 64 rows
+    for B, shape in ((3, "3,64,0"), (40, "4,256,0"), (9, "1,64,0,1"), (9, "2,64,0,2"), (9, "1,128,0,2")):
         os.environ["PECS_SWEEP_FORCE"] = shape
         try:
             with pytest.raises(RuntimeError, match="PECS_SWEEP_FORCE"):
