@@ -149,7 +149,9 @@ int pecs_stats(pecs_ctx* ctx, double* out, int n);
  * host from the local operators (SolarCell.cpp:377-417, 822-932): call with
  * rowptr == NULL to get sizes.  which: PECS_ELECTRONS..PECS_OXIDANTS (system
  * matrices), PECS_POISSON (unconstrained mixed matrix), PECS_MASS_*.
- * `constrained` (may be NULL, PECS_POISSON only) flags Neumann D.n DoFs. */
+ * `constrained` (may be NULL, PECS_POISSON only) flags Neumann D.n DoFs; on
+ * a locally refined mesh the hanging-node constraints (each fine face's D
+ * DoFs equal to the coarse face's flux restricted to it) are not flagged. */
 int pecs_export_matrix(const pecs_mesh* mesh, const pecs_params* params, int which, int64_t* n, int64_t* nnz,
                        int64_t* rowptr, int32_t* col, double* val, int8_t* constrained);
 
