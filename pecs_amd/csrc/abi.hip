@@ -42,15 +42,17 @@ thread_local std::string g_error;
 
 struct FactorDev {
   DevFactor d{};
-  std::vector<int> foff, boff, goff, xoff, fw, bw, fm, bm, frows, brows, fks, bks, froff, broff;  // per level
+  std::vector<int> foff, boff, goff, xoff, fw, bw, fm, bm, frows, brows, fks, bks, froff, broff, fpf,
+      bpf;  // per level
   int nlevels = 0, n = 0;
   double nnz = 0.0;
   double tri_bytes = 0.0;      // algorithmic bytes of a forward + backward sweep
   double device_seconds = 0.0, device_flops = 0.0;  // numeric factorisation on the device (0: on the host)
   std::vector<int> cell_base;  // host copy: grid cell -> permuted base
   FactorLevels lv() const {
-    return FactorLevels{foff.data(), boff.data(), goff.data(), xoff.data(), fw.data(), bw.data(), fm.data(),
-                        bm.data(),   fks.data(),  bks.data(),  froff.data(), broff.data(), nlevels};
+    return FactorLevels{foff.data(), boff.data(), goff.data(),  xoff.data(),  fw.data(), bw.data(),
+                        fm.data(),   bm.data(),   fks.data(),   bks.data(),   froff.data(), broff.data(),
+                        nlevels,     fpf.empty() ? nullptr : fpf.data(), bpf.empty() ? nullptr : bpf.data()};
   }
 };
 
@@ -359,7 +361,7 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
         const bool fwd = dir == 0;
         float best = 1e30f;
         int bw = fwd ? P.fw[l] : P.bw[l], bm = fwd ? P.fm[l] : P.bm[l], br = fwd ? P.frows[l] : P.brows[l];
-        int bk = 0;
+        int bk = 0, bpf = 0;
         if (nrhs > 2) {
           bw = 0;
           bm = 1;
@@ -368,7 +370,7 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
         // candidates: (w, m, rows, ks); w = 0 is the MFMA sweep (m 16-row
         // blocks per wave, rows <= 64 m, column segments of ks), used for more
         // than 2 right-hand sides
-        std::vector<std::array<int, 4>> cand;
+        std::vector<std::array<int, 5>> cand;  // (w, m, rows, ks, prefetching scalar variant)
         if (nrhs > 2) {
           const int nrp = mfma_rhs_pad(nrhs);
           for (int split = 0; split < (nrp > 16 ? 2 : 1); ++split)
@@ -376,23 +378,27 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
               const int tr = mfma_tile_rows(nrp, m, split);
               for (int rows : {tr, tr / 2})
                 for (int ks : {0, 4096, 2048, 1024, 512, 256, 128})
-                  if (rows >= 16 && mfma_shape_ok(nrp, m, split)) cand.push_back({split ? -1 : 0, m, rows, ks});
+                  if (rows >= 16 && mfma_shape_ok(nrp, m, split)) cand.push_back({split ? -1 : 0, m, rows, ks, 0});
             }
           if (nrp >= 32)  // the LDS-staged vector operand (w = -2): one block per wave, 64 rows per tile
-            for (int ks : {0, 4096, 2048, 1024, 512, 256, 128}) cand.push_back({-2, 1, 64, ks});
+            for (int ks : {0, 4096, 2048, 1024, 512, 256, 128}) cand.push_back({-2, 1, 64, ks, 0});
         } else {
-          for (int w : {8, 16, 32, 64})
-            for (int m : {1, 2, 4, 8}) {
-              if (m == 8 && w < 32) continue;
-              for (int half = 0; half < 2; ++half) {
-                const int R = 256 / w;
-                const int rows = (R * m) >> half;
-                if (rows >= R) cand.push_back({w, m, rows, 0});
+          // the prefetching variant is a candidate on levels with few nodes,
+          // where each lane's chain of load latencies sets the time
+          const bool few = P.lnoff[l + 1] - P.lnoff[l] <= 64;
+          for (int pf = 0; pf < (few ? 2 : 1); ++pf)
+            for (int w : {8, 16, 32, 64})
+              for (int m : {1, 2, 4, 8}) {
+                if (m == 8 && w < 32) continue;
+                for (int half = 0; half < 2; ++half) {
+                  const int R = 256 / w;
+                  const int rows = (R * m) >> half;
+                  if (rows >= R) cand.push_back({w, m, rows, 0, pf});
+                }
               }
-            }
         }
         for (const auto& cd : cand) {
-          const int w = cd[0], m = cd[1], rows = cd[2], ks = cd[3];
+          const int w = cd[0], m = cd[1], rows = cd[2], ks = cd[3], pf = cd[4];
           tl.clear();
           rtl.clear();
           if (!level_tiles_split(P, l, fwd, rows, ks, tl, rtl)) continue;
@@ -421,7 +427,7 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
           for (int rep = 0; rep < 3; ++rep) {
             HIPCHECK(hipMemsetAsync(scrub, rep, scrub_bytes, s));
             HIPCHECK(hipEventRecord(e0, s));
-            HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, s, ks, rtbuf, nrt));
+            HIPCHECK((hipError_t)launch_sweep(d, tbuf, nt, w, m, fwd, nrhs, Y, s, ks, rtbuf, nrt, nullptr, pf != 0));
             HIPCHECK(hipEventRecord(e1, s));
             HIPCHECK(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -434,6 +440,7 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
             bm = m;
             br = rows;
             bk = ks;
+            bpf = pf;
           }
         }
         if (trace) {
@@ -450,21 +457,23 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
             }
           }
           std::fprintf(stderr,
-                       "[setup] tune n=%d level %d %s: W %d MAXR %d rows %d segment %d (%.3f ms) nodes %d max ns %d"
+                       "[setup] tune n=%d level %d %s: W %d MAXR %d rows %d segment %d%s (%.3f ms) nodes %d max ns %d"
                        " factor %.1f MB (%.2f TB/s)\n",
-                       P.n, l, fwd ? "fwd" : "bwd", bw, bm, br, bk, best, P.lnoff[l + 1] - P.lnoff[l], maxns,
-                       bytes / 1e6, bytes / (best * 1e-3) / 1e12);
+                       P.n, l, fwd ? "fwd" : "bwd", bw, bm, br, bk, bpf ? " prefetch" : "", best,
+                       P.lnoff[l + 1] - P.lnoff[l], maxns, bytes / 1e6, bytes / (best * 1e-3) / 1e12);
         }
         if (fwd) {
           P.fw[l] = bw;
           P.fm[l] = bm;
           P.frows[l] = br;
           P.fks[l] = bk;
+          P.fpf[l] = bpf;
         } else {
           P.bw[l] = bw;
           P.bm[l] = bm;
           P.brows[l] = br;
           P.bks[l] = bk;
+          P.bpf[l] = bpf;
         }
       }
     }
@@ -623,6 +632,8 @@ FactorDev make_factor(pecs_ctx* c, const BlockSPD& S, int leaf, int nrhs) {
   D.bw = P.bw;
   D.fm = P.fm;
   D.bm = P.bm;
+  D.fpf = P.fpf;
+  D.bpf = P.bpf;
   D.frows = P.frows;
   D.brows = P.brows;
   D.fks = P.fks;

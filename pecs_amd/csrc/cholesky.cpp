@@ -650,6 +650,8 @@ PackedFactor pack_layout(const SupernodalFactor& F) {
   P.bw.assign(F.nlevels, 64);
   P.fm.assign(F.nlevels, 1);
   P.bm.assign(F.nlevels, 1);
+  P.fpf.assign(F.nlevels, 0);
+  P.bpf.assign(F.nlevels, 0);
   P.frows.assign(F.nlevels, 4);
   P.brows.assign(F.nlevels, 4);
   P.lnoff.assign(F.nlevels + 1, 0);

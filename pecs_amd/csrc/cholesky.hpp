@@ -97,6 +97,7 @@ struct PackedFactor {
   std::vector<int> fm, bm;            // row passes per tile per level
   std::vector<int> frows, brows;      // tile height per level (<= (256 / W) * MAXR)
   std::vector<int> fks, bks;          // per level column-segment length of split tiles (0: no split)
+  std::vector<int> fpf, bpf;          // per level: the scalar sweep's prefetching variant (0 / 1)
   std::vector<int> frtiles, brtiles;  // split levels: row tiles (node, row0, nrows, 64 * first tile + nseg)
   std::vector<int> froff, broff;      // per level offsets into frtiles / brtiles
   std::vector<int> lnodes, lnoff;     // nodes of each level + offsets

@@ -687,7 +687,12 @@ __global__ __launch_bounds__(256) void k_fwd_gather(DevFactor F, const int* __re
   }
 }
 
-template <int NRHS, int W, int MAXR_, bool FWD>
+// PF: the factor loads of the next pair of positions are issued before this
+// pair's products (unconditional loads from clamped positions, masked at use,
+// two register sets in turn).  Costlier in registers and issue slots, it
+// pays on the levels with few tiles, whose time is the latency of each
+// lane's chain of loads; the tuner picks it per level.
+template <int NRHS, int W, int MAXR_, bool FWD, bool PF = false>
 __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restrict__ tiles,
                                                const double* __restrict__ b0, const double* __restrict__ b1,
                                                double* __restrict__ Y0, double* __restrict__ Y1) {
@@ -762,6 +767,50 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
     }
     __syncthreads();
     const int e = min(hi, c0 + cl);
+    if constexpr (PF) {
+      auto ld = [&](int p, double (&L)[MAXR]) {
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) {
+          const int pc = re[q] > rs[q] ? min(max(p, rs[q]), re[q] - 1) : 0;
+          L[q] = __builtin_nontemporal_load(base + off[q] + pc);
+        }
+      };
+      auto fma_pair = [&](int pa, const double (&La)[MAXR], const double (&Lb)[MAXR]) {
+        const int pb = pa + W;
+        const bool okb = pb < e;
+        const int ib = okb ? pb - c0 : 0;
+        const double va0 = vec[0][pa - c0];
+        const double vb0 = okb ? vec[0][ib] : 0.0;
+        const double va1 = NRHS > 1 ? vec[NRHS - 1][pa - c0] : 0.0;
+        const double vb1 = NRHS > 1 && okb ? vec[NRHS - 1][ib] : 0.0;
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) {
+          const double a = (pa >= rs[q] && pa < re[q]) ? La[q] : 0.0;
+          const double b = (pb >= rs[q] && pb < re[q]) ? Lb[q] : 0.0;
+          acc[q][0] += a * va0 + b * vb0;
+          if (NRHS > 1) acc[q][NRHS - 1] += a * va1 + b * vb1;
+        }
+      };
+      int pa = max(lo, c0) + sl;
+      if (pa < e) {
+        double La[MAXR], Lb[MAXR], Ma[MAXR], Mb[MAXR];
+        ld(pa, La);
+        ld(pa + W, Lb);
+        for (; pa < e; pa += 4 * W) {
+          ld(pa + 2 * W, Ma);
+          ld(pa + 3 * W, Mb);
+          __builtin_amdgcn_sched_barrier(0);
+          fma_pair(pa, La, Lb);
+          __builtin_amdgcn_sched_barrier(0);
+          ld(pa + 4 * W, La);
+          ld(pa + 5 * W, Lb);
+          __builtin_amdgcn_sched_barrier(0);
+          if (pa + 2 * W < e) fma_pair(pa + 2 * W, Ma, Mb);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      continue;
+    }
     for (int pa = max(lo, c0) + sl; pa < e; pa += 2 * W) {
       const int pb = pa + W;
       const bool okb = pb < e;
@@ -1902,10 +1951,11 @@ static int launch_schur_recover_main(int k, const DevRegion& R, const double* b1
 
 template <int NRHS, bool FWD>
 static int sweep_dispatch(const DevFactor& F, const int4* tl, int nt, int w, int maxr, const double* v0,
-                          const double* v1, double* t0, double* t1, hipStream_t s) {
+                          const double* v1, double* t0, double* t1, hipStream_t s, bool pf) {
 #define PECS_SWEEP_CASE(WW, MM)                                                          \
   if (w == WW && maxr == MM) {                                                           \
-    k_sweep<NRHS, WW, MM, FWD><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1);                \
+    if (pf) k_sweep<NRHS, WW, MM, FWD, true><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1);  \
+    else k_sweep<NRHS, WW, MM, FWD><<<nt, 256, 0, s>>>(F, tl, v0, v1, t0, t1);           \
     return 0;                                                                            \
   }
   PECS_SWEEP_CASE(8, 1) PECS_SWEEP_CASE(8, 2) PECS_SWEEP_CASE(8, 4)
@@ -1959,7 +2009,7 @@ int mfma_tile_rows(int nrp, int mb, bool split) {
 }
 
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
-                 double* t, hipStream_t s, int ks, const int4* rtiles, int nrt, const double* src) {
+                 double* t, hipStream_t s, int ks, const int4* rtiles, int nrt, const double* src, bool pf) {
   if (nt <= 0) return 0;
   if (w <= 0) {  // 0: MFMA sweep, -1: its right-hand-side groups split over waves, -2: LDS-staged vector
     if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
@@ -1989,11 +2039,11 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
   const double* v0 = fwd ? src : nullptr;
   double* t1 = nrhs == 2 ? t + F.n : t;
   if (fwd) {
-    return nrhs == 2 ? sweep_dispatch<2, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s)
-                     : sweep_dispatch<1, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s);
+    return nrhs == 2 ? sweep_dispatch<2, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s, pf)
+                     : sweep_dispatch<1, true>(F, tiles, nt, w, maxr, v0, v0, t, t1, s, pf);
   }
-  return nrhs == 2 ? sweep_dispatch<2, false>(F, tiles, nt, w, maxr, v0, v0, t, t1, s)
-                   : sweep_dispatch<1, false>(F, tiles, nt, w, maxr, v0, v0, t, t1, s);
+  return nrhs == 2 ? sweep_dispatch<2, false>(F, tiles, nt, w, maxr, v0, v0, t, t1, s, pf)
+                   : sweep_dispatch<1, false>(F, tiles, nt, w, maxr, v0, v0, t, t1, s, pf);
 }
 
 template <int NRP>
@@ -2047,13 +2097,14 @@ int launch_chol_solve(const DevFactor& F, const FactorLevels& L, double* v, doub
     }
     const int nt = L.foff[l + 1] - L.foff[l];
     const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s, 0, nullptr, 0,
-                               l == 0 ? v : nullptr);
+                               l == 0 ? v : nullptr, L.fpf && L.fpf[l]);
     if (e) return e;
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
     // the backward sweep reads the panel rows' x straight from X (no gather)
     const int nt = L.boff[l + 1] - L.boff[l];
-    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s);
+    const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s, 0, nullptr, 0,
+                               nullptr, L.bpf && L.bpf[l]);
     if (e) return e;
   }
   // a kernel, not a memcpy node, so that the solve can be captured in a graph

@@ -157,6 +157,8 @@ struct FactorLevels {
   const int* froff; // host: per level offsets into frtiles (nlevels+1)
   const int* broff; // host: per level offsets into brtiles
   int nlevels;
+  const int* fpf;   // host: scalar sweep, prefetching variant per forward level (null: none)
+  const int* bpf;   // host: the same for the backward sweep
 };
 
 // launchers (return hipError_t as int).  The cell-local launchers run nb
@@ -186,7 +188,7 @@ int launch_schur_recover(int k, const DevRegion& R, const double* b1, const doub
 // level, whose rows are owed no contributions).
 int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr, bool fwd, int nrhs,
                  double* t, hipStream_t s, int ks = 0, const int4* rtiles = nullptr, int nrt = 0,
-                 const double* src = nullptr);
+                 const double* src = nullptr, bool pf = false);
 // Padded right-hand-side count of the MFMA sweep (16 or 32).
 // Right-hand sides of the MFMA sweeps: padded to 16, 32, 64 or 128
 // interleaved columns; at most MAX_MFMA_RHS.  mfma_shape_ok: whether the
