@@ -454,6 +454,9 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
               const std::vector<int>& b0 = fwd ? P.fblk0 : P.bblk0;
               const std::vector<int64_t>& bo = fwd ? P.fblk_off : P.bblk_off;
               bytes += 8.0 * (double)(bo[b0[sn + 1]] - bo[b0[sn]]);
+            } else {  // packed triangle and panel of one copy
+              const double ns_ = P.ns[sn], ms_ = P.ms[sn];
+              bytes += 8.0 * (0.5 * ns_ * (ns_ + 1.0) + ms_ * ns_);
             }
           }
           std::fprintf(stderr,

@@ -870,29 +870,71 @@ __global__ __launch_bounds__(256) void k_sweep(DevFactor F, const int4* __restri
 // Accumulator lane l, register r holds row l/16 + 4r, right-hand side l%16.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// t_p = b_p - sum of the updates z owed to row p, interleaved (h fastest).
+// The interleaved vectors from and to one vector per right-hand side (entry
+// p of right-hand side h at [h * n + p]): a tile of IL_TP rows passes through
+// LDS, so that the per-right-hand-side side moves IL_TP consecutive rows
+// (256 bytes) per wave access and the interleaved side whole rows of NRP
+// entries.  The padding columns (h >= nrhs) are written as zeros.
+constexpr int IL_TP = 32;
+
+template <int NRP>
+__global__ __launch_bounds__(256) void k_pack_il(DevFactor F, const double* __restrict__ v, int nrhs) {
+  __shared__ double s[NRP * (IL_TP + 1)];
+  const int64_t n = F.n, p0 = (int64_t)blockIdx.x * IL_TP;
+  for (int e = threadIdx.x; e < NRP * IL_TP; e += 256) {
+    const int h = e / IL_TP, pp = e % IL_TP;
+    s[h * (IL_TP + 1) + pp] = (h < nrhs && p0 + pp < n) ? v[(int64_t)h * n + p0 + pp] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < IL_TP * NRP / 2; e += 256) {
+    const int pp = e / (NRP / 2), h = 2 * (e % (NRP / 2));
+    if (p0 + pp < n)
+      *reinterpret_cast<double2*>(F.X + (p0 + pp) * NRP + h) =
+          double2{s[h * (IL_TP + 1) + pp], s[(h + 1) * (IL_TP + 1) + pp]};
+  }
+}
+
+template <int NRP>
+__global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restrict__ v, int nrhs) {
+  __shared__ double s[NRP * (IL_TP + 1)];
+  const int64_t n = F.n, p0 = (int64_t)blockIdx.x * IL_TP;
+  for (int e = threadIdx.x; e < IL_TP * NRP / 2; e += 256) {
+    const int pp = e / (NRP / 2), h = 2 * (e % (NRP / 2));
+    if (p0 + pp < n) {
+      const double2 x = *reinterpret_cast<const double2*>(F.X + (p0 + pp) * NRP + h);
+      s[h * (IL_TP + 1) + pp] = x.x;
+      s[(h + 1) * (IL_TP + 1) + pp] = x.y;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nrhs * IL_TP; e += 256) {
+    const int h = e / IL_TP, pp = e % IL_TP;
+    if (p0 + pp < n) v[(int64_t)h * n + p0 + pp] = s[h * (IL_TP + 1) + pp];
+  }
+}
+
+// t_p = b_p - sum of the updates z owed to row p, in place on the packed
+// right-hand sides (X holds b on entry; rows owed nothing are left alone).
 // A thread takes two adjacent right-hand sides (16-byte loads and stores).
 template <int NRP>
 __global__ __launch_bounds__(256) void k_fwd_gather_il(DevFactor F, const int* __restrict__ lrows, int nrow,
-                                                       const double* __restrict__ b, int nrhs) {
+                                                       int nrhs) {
   constexpr int NP = NRP / 2;
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= (int64_t)nrow * NP) return;
   const int i = (int)(e / NP), h = 2 * (int)(e % NP);
   const int p = lrows[i];
-  double a0 = 0.0, a1 = 0.0;
-  if (h < nrhs) {
-    a0 = b[(int64_t)h * F.n + p];
-    if (h + 1 < nrhs) a1 = b[(int64_t)(h + 1) * F.n + p];
-    const int64_t c1 = F.cptr[p + 1];
-    for (int64_t c = F.cptr[p]; c < c1; ++c) {
-      const double2 z = *reinterpret_cast<const double2*>(F.Z + F.czidx[c] * NRP + h);
-      a0 -= z.x;
-      a1 -= z.y;
-    }
-    if (h + 1 >= nrhs) a1 = 0.0;
+  const int64_t c0 = F.cptr[p], c1 = F.cptr[p + 1];
+  if (c0 == c1 || h >= nrhs) return;
+  double2* x = reinterpret_cast<double2*>(F.X + (int64_t)p * NRP + h);
+  double2 a = *x;
+  for (int64_t c = c0; c < c1; ++c) {
+    const double2 z = *reinterpret_cast<const double2*>(F.Z + F.czidx[c] * NRP + h);
+    a.x -= z.x;
+    a.y -= z.y;
   }
-  *reinterpret_cast<double2*>(F.X + (int64_t)p * NRP + h) = double2{a0, a1};
+  if (h + 1 >= nrhs) a.y = 0.0;
+  *x = a;
 }
 
 template <int NRP>
@@ -904,16 +946,6 @@ __global__ __launch_bounds__(256) void k_bwd_gather_il(DevFactor F, const int64_
   const int64_t q = idx[i];
   *reinterpret_cast<double2*>(F.Z + q * NRP + h) =
       *reinterpret_cast<const double2*>(F.X + (int64_t)F.rows[q] * NRP + h);
-}
-
-// x (h-major, nrhs right-hand sides) from the interleaved result
-template <int NRP>
-__global__ __launch_bounds__(256) void k_unpack_il(DevFactor F, double* __restrict__ v, int nrhs) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)F.n * NRP) return;
-  const int64_t p = e / NRP;
-  const int h = (int)(e % NRP);
-  if (h < nrhs) v[(int64_t)h * F.n + p] = F.X[e];
 }
 
 // ks > 0: the tile is column segment tl.w (columns [tl.w ks, (tl.w + 1) ks))
@@ -2048,10 +2080,12 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
 
 template <int NRP>
 static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, double* t, int nrhs, hipStream_t s) {
+  k_pack_il<NRP><<<nblk(F.n, IL_TP), 256, 0, s>>>(F, v, nrhs);
+  // the leaf level's rows are owed nothing (pack_layout checks it)
   for (int l = 0; l < L.nlevels; ++l) {
     const int ng = L.goff[l + 1] - L.goff[l];
-    if (ng > 0)
-      k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP / 2, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, v, nrhs);
+    if (l > 0 && ng > 0)
+      k_fwd_gather_il<NRP><<<nblk((int64_t)ng * NRP / 2, 256), 256, 0, s>>>(F, F.lrows + L.goff[l], ng, nrhs);
     const int nt = L.foff[l + 1] - L.foff[l];
     if (L.fw[l] > 0) return (int)hipErrorInvalidValue;
     const int e = launch_sweep(F, F.ftiles + L.foff[l], nt, L.fw[l], L.fm[l], true, nrhs, t, s, L.fks[l],
@@ -2067,7 +2101,7 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
                                F.brtiles + L.broff[l], L.broff[l + 1] - L.broff[l]);
     if (e) return e;
   }
-  k_unpack_il<NRP><<<nblk((int64_t)F.n * NRP, 256), 256, 0, s>>>(F, v, nrhs);
+  k_unpack_il<NRP><<<nblk(F.n, IL_TP), 256, 0, s>>>(F, v, nrhs);
   return (int)hipGetLastError();
 }
 

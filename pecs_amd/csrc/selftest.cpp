@@ -114,8 +114,8 @@ static void packed_solve_host(const PackedFactor& P, const double* b, double* x)
   for (int i = 0; i < P.n; ++i) x[i] = X[i];
 }
 
-// Host emulation of the FP64-MFMA solve (k_fwd_gather_il, k_sweep_mfma,
-// k_bwd_gather_il, k_unpack_il) lane by lane: the same tiles, the
+// Host emulation of the FP64-MFMA solve (k_pack_il, k_fwd_gather_il,
+// k_sweep_mfma, k_bwd_gather_il, k_unpack_il) lane by lane: the same tiles, the
 // RHS-interleaved vectors ([p * NRP + h]), the same per-lane loads and
 // v_mfma_f64_16x16x4f64 operand / result layouts (A: lane l = row l%16, k
 // l/16; B: lane l = k l/16, column l%16; D: lane l register r = row l/16 + 4r,
@@ -140,18 +140,16 @@ static void mfma_solve_emulate(const PackedFactor& P, int nrhs, int MB, int KS, 
     for (int li = 0; li < P.nlevels; ++li) {
       const int l = FWD ? li : P.nlevels - 1 - li;
       if (FWD) {
+        if (l == 0)  // k_pack_il: the right-hand sides interleaved, padding zero
+          for (int64_t p = 0; p < n; ++p)
+            for (int h = 0; h < NRP; ++h) X[p * NRP + h] = h < nrhs ? b[h * n + p] : 0.0;
         for (int i = P.goff[l]; i < P.goff[l + 1]; ++i)
-          for (int h = 0; h < NRP; ++h) {
+          for (int h = 0; h < nrhs && l > 0; ++h) {
             const int p = P.lrows[i];
-            double a = 0.0;
-            if (h < nrhs) {
-              a = b[h * n + p];
-              for (int64_t c = P.cptr[p]; c < P.cptr[p + 1]; ++c) {
-                EMU_CHECK(P.czidx[c] >= 0 && P.czidx[c] < zl, "gathered z");
-                a -= Z[P.czidx[c] * NRP + h];
-              }
+            for (int64_t c = P.cptr[p]; c < P.cptr[p + 1]; ++c) {
+              EMU_CHECK(P.czidx[c] >= 0 && P.czidx[c] < zl, "gathered z");
+              X[(int64_t)p * NRP + h] -= Z[P.czidx[c] * NRP + h];
             }
-            X[(int64_t)p * NRP + h] = a;
           }
       } else {
         for (int i = P.xoff[l]; i < P.xoff[l + 1]; ++i)
