@@ -292,14 +292,14 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
     int m = 0, rows = 0, ks = -1, split = 0;
     const int nf = std::sscanf(fs, "%d,%d,%d,%d", &m, &rows, &ks, &split);
     const int nrp = mfma_rhs_pad(nrhs);
-    const bool lds = split == 2;  // the LDS-staged vector operand: m 1, at most 64 rows
+    const bool lds = split == 2;  // the LDS-staged vector operand: m 1 or 2, at most 64 m rows
     if ((nf != 3 && nf != 4) || (m != 1 && m != 2 && m != 4) || split < 0 || split > 2 || rows < 16 ||
-        rows > (lds ? 64 : mfma_tile_rows(nrp, m, split)) || rows % 16 != 0 || ks < 0 ||
-        (ks > 0 && (ks < 16 || ks % 16 != 0)) || !mfma_shape_ok(nrp, m, split == 1) || (lds && m != 1))
+        rows > (lds ? 64 * m : mfma_tile_rows(nrp, m, split)) || rows % 16 != 0 || ks < 0 ||
+        (ks > 0 && (ks < 16 || ks % 16 != 0)) || !mfma_shape_ok(nrp, m, split == 1) || (lds && m == 4))
       throw std::runtime_error("PECS_SWEEP_FORCE: expected m,rows,ks[,split] (m 1, 2 or 4 -- 1 or 2 above 64 "
                                "right-hand sides unless split -- rows a multiple of 16 up to the rows a workgroup "
                                "covers, 64 m or with split 1 fewer, ks 0 or a multiple of 16; split 2: the "
-                               "LDS-staged sweep, m 1, rows up to 64)");
+                               "LDS-staged sweep, m 1 or 2, rows up to 64 m)");
     // (both split forms need more than 16 right-hand sides; a factor with 16
     // or fewer keeps the plain MFMA sweep)
     const int w = nrp <= 16 ? 0 : (lds ? -2 : (split ? -1 : 0));
@@ -380,8 +380,9 @@ void tune_factor(pecs_ctx* c, PackedFactor& P, DevFactor& d, int nrhs) {
                 for (int ks : {0, 4096, 2048, 1024, 512, 256, 128})
                   if (rows >= 16 && mfma_shape_ok(nrp, m, split)) cand.push_back({split ? -1 : 0, m, rows, ks, 0});
             }
-          if (nrp >= 32)  // the LDS-staged vector operand (w = -2): one block per wave, 64 rows per tile
-            for (int ks : {0, 4096, 2048, 1024, 512, 256, 128}) cand.push_back({-2, 1, 64, ks, 0});
+          if (nrp >= 32)  // the LDS-staged vector operand (w = -2): m blocks per wave, 64 m rows per tile
+            for (int m : {1, 2})
+              for (int ks : {0, 4096, 2048, 1024, 512, 256, 128}) cand.push_back({-2, m, 64 * m, ks, 0});
         } else {
           // the prefetching variant is a candidate on levels with few nodes,
           // where each lane's chain of load latencies sets the time

@@ -1105,17 +1105,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
 }
 
 // The MFMA sweep with the vector operand staged in LDS once per workgroup:
-// its 4 waves each own one 16-row block of the tile (64 rows) and all
-// right-hand-side groups, and step through the union of their column ranges
-// together.  Per 16-column step the workgroup loads the step's 16 x NRP
-// vector entries once (NRP / 16 per thread, contiguous), stores them in LDS
-// and every wave reads its B operands there: the vector bytes leaving L2
-// drop to a quarter of the per-wave loads.  The next step's entries are
-// loaded into registers while this step's MFMAs run, with one barrier per
-// step (double-buffered LDS).  A wave's block contributes zero on steps
-// outside its stored columns (masked factor operand); a wave without rows
-// still loads, stores and meets every barrier.
-template <int NRP, bool FWD>
+// its 4 waves each own MBL consecutive 16-row blocks of the tile (64 MBL
+// rows) and all right-hand-side groups, and step through the union of their
+// column ranges together.  Per 16-column step the workgroup loads the step's
+// 16 x NRP vector entries once (NRP / 16 per thread, contiguous), stores them
+// in LDS and every wave reads its B operands there, each B operand serving
+// the wave's MBL blocks: the vector bytes leaving L2 drop to 1 / (4 MBL) of
+// per-block loads.  The next step's entries are loaded into registers while
+// this step's MFMAs run, with one barrier per step (double-buffered LDS).  A
+// block contributes zero on steps outside its stored columns (masked factor
+// operand), and a wave skips the steps none of its blocks stores (wave-uniform
+// through readfirstlane); a wave without rows still loads, stores and meets
+// every barrier.
+template <int NRP, bool FWD, int MBL>
 __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4* __restrict__ tiles,
                                                         double* __restrict__ Y, int ks,
                                                         double* __restrict__ Part) {
@@ -1129,9 +1131,8 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
   const int VL = FWD ? ns : ns + ms;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ri = lane & 15, kq = lane >> 4;
-  const int first = row0 + wave * 16;
-  const bool rows = first < row0 + nr;  // wave-uniform
-  const int last_t = row0 + nr - 1;     // the tile's last row
+  const int first = row0 + wave * 16 * MBL;
+  const int last_t = row0 + nr - 1;  // the tile's last row
   // the workgroup's column range (the union of its blocks')
   const int lo = FWD ? 0 : row0;
   const int hi = FWD ? (last_t < ns ? last_t + 1 : ns) : ns + ms;
@@ -1141,25 +1142,34 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
     cend = min(cend, (tl.w + 1) * ks);
   }
   const int nstep = cend > c0 ? (cend - c0 + 15) >> 4 : 0;
-  // this wave's block in the MFMA-blocked copy
+  // this wave's blocks in the MFMA-blocked copy
   const double* __restrict__ blk = FWD ? F.fblk : F.bblk;
   const int64_t* __restrict__ boff = FWD ? F.fblk_off : F.bblk_off;
-  const int b = first / 16;
   const int bnode = FWD ? F.fblk0[s] : F.bblk0[s];
-  const int64_t bbase = rows ? boff[bnode + b] : boff[bnode];
-  const int bclo = rows ? (FWD ? 0 : 16 * b) : 0;
-  const int bsteps = rows ? (int)((boff[bnode + b + 1] - bbase) >> 8) : 0;
+  bool rows[MBL];
+  int64_t bbase[MBL];
+  int bclo[MBL], bsteps[MBL];
+#pragma unroll
+  for (int j = 0; j < MBL; ++j) {
+    const int b = first / 16 + j;
+    rows[j] = first + 16 * j < row0 + nr;  // wave-uniform
+    bbase[j] = rows[j] ? boff[bnode + b] : boff[bnode];
+    bclo[j] = rows[j] ? (FWD ? 0 : 16 * b) : 0;
+    bsteps[j] = rows[j] ? (int)((boff[bnode + b + 1] - bbase[j]) >> 8) : 0;
+  }
   const int64_t zrow = FWD ? 0 : F.rows_off[s] - ns;
   // staged entries of thread t at step st: column c = (t K) / NRP of the
   // step, right-hand sides h0 .. h0 + K - 1 (contiguous in the interleaved
   // vector); past the rows' last column the last column's entries (finite;
-  // the factor is zero there)
+  // the factor is zero there).  The backward sweep reads its panel columns
+  // (the ancestors' rows, solved by earlier launches) straight from x
+  // through the row index: no gathered copy.
   const int tf = threadIdx.x * K, sc = tf / NRP, sh = tf % NRP;
   auto stage_load = [&](int st, double (&R)[K]) {
     const int col = min(c0 + 16 * st + sc, VL - 1);
     const double* src;
     if (FWD) src = F.X + (int64_t)(p0 + col) * NRP + sh;
-    else src = col < ns ? Y + (int64_t)(p0 + col) * NRP + sh : F.Z + (zrow + col) * NRP + sh;
+    else src = col < ns ? Y + (int64_t)(p0 + col) * NRP + sh : F.X + (int64_t)F.rows[zrow + col] * NRP + sh;
 #pragma unroll
     for (int j = 0; j < K; j += 2) {
       const double2 v = *reinterpret_cast<const double2*>(src + j);
@@ -1172,33 +1182,46 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
     for (int j = 0; j < K; j += 2)
       *reinterpret_cast<double2*>(&sB[buf][sc * S + sh + j]) = double2{R[j], R[j + 1]};
   };
-  // the factor operand of step st (clamped into the block's stored steps;
-  // in = whether the step is one of them)
-  auto a_load = [&](int st, double (&L)[4], bool& in) {
-    const int idx = (c0 + 16 * st - bclo) >> 4;
-    in = rows && idx >= 0 && idx < bsteps;
-    const int ic = min(max(idx, 0), max(bsteps - 1, 0));
-    const double* src = blk + bbase + (int64_t)ic * 256 + lane;
-    // (a wave without rows reads its node's first block: a valid address,
-    // masked by in)
+  // the factor operands of step st (clamped into each block's stored steps;
+  // in[j] = whether the step is one of block j's, wave-uniform)
+  auto a_load = [&](int st, double (&L)[MBL][4], int (&in)[MBL]) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) L[q] = __builtin_nontemporal_load(src + q * 64);
+    for (int j = 0; j < MBL; ++j) {
+      const int idx = (c0 + 16 * st - bclo[j]) >> 4;
+      in[j] = __builtin_amdgcn_readfirstlane((int)(rows[j] && idx >= 0 && idx < bsteps[j]));
+      const int ic = min(max(idx, 0), max(bsteps[j] - 1, 0));
+      const double* src = blk + bbase[j] + (int64_t)ic * 256 + lane;
+      // (a block without rows reads its node's first block: a valid
+      // address, skipped by in)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) L[j][q] = __builtin_nontemporal_load(src + q * 64);
+    }
   };
-  v4d acc[NH];
+  v4d acc[MBL][NH];
 #pragma unroll
-  for (int hg = 0; hg < NH; ++hg) acc[hg] = v4d{0.0, 0.0, 0.0, 0.0};
-  auto mma = [&](int buf, const double (&L)[4], bool in) {
+  for (int j = 0; j < MBL; ++j)
+#pragma unroll
+    for (int hg = 0; hg < NH; ++hg) acc[j][hg] = v4d{0.0, 0.0, 0.0, 0.0};
+  auto mma = [&](int buf, const double (&L)[MBL][4], const int (&in)[MBL]) {
+    int any = 0;
+#pragma unroll
+    for (int j = 0; j < MBL; ++j) any |= in[j];
+    if (!any) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double a = in ? L[q] : 0.0;
       const double* row = &sB[buf][(4 * kq + q) * S + ri];
 #pragma unroll
-      for (int hg = 0; hg < NH; ++hg) acc[hg] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, row[hg * 16], acc[hg], 0, 0, 0);
+      for (int hg = 0; hg < NH; ++hg) {
+        const double bv = row[hg * 16];
+#pragma unroll
+        for (int j = 0; j < MBL; ++j)
+          acc[j][hg] = __builtin_amdgcn_mfma_f64_16x16x4f64(in[j] ? L[j][q] : 0.0, bv, acc[j][hg], 0, 0, 0);
+      }
     }
   };
   if (nstep > 0) {
-    double R[K], LA[4], LB[4];
-    bool IA, IB;
+    double R[K], LA[MBL][4], LB[MBL][4];
+    int IA[MBL], IB[MBL];
     stage_load(0, R);
     a_load(0, LA, IA);
     stage_store(0, R);
@@ -1208,7 +1231,7 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
       stage_load(min(st + 1, nstep - 1), R);
       a_load(min(st + 1, nstep - 1), LB, IB);
       __builtin_amdgcn_sched_barrier(0);
-      if (rows) mma(0, LA, IA);
+      mma(0, LA, IA);
       __builtin_amdgcn_sched_barrier(0);
       if (st + 1 < nstep) stage_store(1, R);
       __syncthreads();
@@ -1217,30 +1240,33 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
       stage_load(min(st + 2, nstep - 1), R);
       a_load(min(st + 2, nstep - 1), LA, IA);
       __builtin_amdgcn_sched_barrier(0);
-      if (rows) mma(1, LB, IB);
+      mma(1, LB, IB);
       __builtin_amdgcn_sched_barrier(0);
       if (st + 2 < nstep) stage_store(0, R);
       __syncthreads();
     }
   }
-  if (!rows) return;  // after the last barrier
 #pragma unroll
-  for (int hg = 0; hg < NH; ++hg)
+  for (int j = 0; j < MBL; ++j) {
+    if (!rows[j]) continue;  // after the last barrier
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int lr = first + kq + 4 * r;
-      const int h = hg * 16 + ri;
-      if (lr >= row0 + nr) continue;
-      const double v = acc[hg][r];
-      if (ks > 0) {
-        Part[((int64_t)blockIdx.x * 64 + (lr - row0)) * NRP + h] = v;
-      } else if (FWD) {
-        if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
-        else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
-      } else {
-        F.X[(int64_t)(p0 + lr) * NRP + h] = v;
+    for (int hg = 0; hg < NH; ++hg)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lr = first + 16 * j + kq + 4 * r;
+        const int h = hg * 16 + ri;
+        if (lr >= row0 + nr) continue;
+        const double v = acc[j][hg][r];
+        if (ks > 0) {
+          Part[((int64_t)blockIdx.x * (64 * MBL) + (lr - row0)) * NRP + h] = v;
+        } else if (FWD) {
+          if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
+          else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
+        } else {
+          F.X[(int64_t)(p0 + lr) * NRP + h] = v;
+        }
       }
-    }
+  }
 }
 
 // Sum of the column segments of split row tiles, in segment order: row tile
@@ -2047,19 +2073,20 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
     if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
     const int nrp = mfma_rhs_pad(nrhs);
     if (w == -2) {
-      if (maxr != 1 || nrp < 32 || (ks > 0 && (!rtiles || !F.Part))) return (int)hipErrorInvalidValue;
-#define PECS_MFMA_LDS(NN)                                                                        \
-  if (nrp == NN) {                                                                               \
+      if ((maxr != 1 && maxr != 2) || nrp < 32 || (ks > 0 && (!rtiles || !F.Part))) return (int)hipErrorInvalidValue;
+#define PECS_MFMA_LDS(NN, MM)                                                                    \
+  if (nrp == NN && maxr == MM) {                                                                 \
     if (fwd) {                                                                                   \
-      k_sweep_mfma_lds<NN, true><<<nt, 256, 0, s>>>(F, tiles, t, ks, F.Part);                    \
-      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, true><<<nrt, 256, 0, s>>>(F, rtiles, 64, F.Part, t); \
+      k_sweep_mfma_lds<NN, true, MM><<<nt, 256, 0, s>>>(F, tiles, t, ks, F.Part);                \
+      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, true><<<nrt, 256, 0, s>>>(F, rtiles, 64 * MM, F.Part, t); \
     } else {                                                                                     \
-      k_sweep_mfma_lds<NN, false><<<nt, 256, 0, s>>>(F, tiles, t, ks, F.Part);                   \
-      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, false><<<nrt, 256, 0, s>>>(F, rtiles, 64, F.Part, t); \
+      k_sweep_mfma_lds<NN, false, MM><<<nt, 256, 0, s>>>(F, tiles, t, ks, F.Part);               \
+      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, false><<<nrt, 256, 0, s>>>(F, rtiles, 64 * MM, F.Part, t); \
     }                                                                                            \
     return (int)hipGetLastError();                                                               \
   }
-      PECS_MFMA_LDS(32) PECS_MFMA_LDS(64) PECS_MFMA_LDS(128)
+      PECS_MFMA_LDS(32, 1) PECS_MFMA_LDS(64, 1) PECS_MFMA_LDS(128, 1)
+      PECS_MFMA_LDS(32, 2) PECS_MFMA_LDS(64, 2) PECS_MFMA_LDS(128, 2)
 #undef PECS_MFMA_LDS
       return (int)hipErrorInvalidValue;
     }
@@ -2094,7 +2121,9 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
     const int nx = L.xoff[l + 1] - L.xoff[l];
-    if (nx > 0) k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP / 2, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
+    // (the LDS-staged sweep reads the panel columns through the row index)
+    if (nx > 0 && L.bw[l] != -2)
+      k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP / 2, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     const int nt = L.boff[l + 1] - L.boff[l];
     if (L.bw[l] > 0) return (int)hipErrorInvalidValue;
     const int e = launch_sweep(F, F.btiles + L.boff[l], nt, L.bw[l], L.bm[l], false, nrhs, t, s, L.bks[l],

@@ -102,14 +102,15 @@ def test_batch_rejects_mismatched_matrix_parameters(seed):
 # their segments in a fixed order: they agree with unsplit ones to rounding.
 SHAPES = ["1,64,0", "1,32,0", "2,128,0", "4,256,0", "4,256,128", "1,64,128", "2,128,256", "1,32,512",
           "1,32,0,1", "2,64,128,1", "4,128,0,1",  # split 1: right-hand-side groups over waves
-          "1,64,0,2", "1,64,256,2"]  # split 2: the vector operand staged in LDS (more than 16 right-hand sides)
+          "1,64,0,2", "1,64,256,2",  # split 2: the vector operand staged in LDS (more than 16 right-hand sides)
+          "2,128,0,2", "2,64,0,2", "2,128,256,2"]  # ... two blocks per wave (a 64-row tile leaves 2 waves idle)
 
 
 # m = 4 is built for 128 padded right-hand sides only with the groups split
 # over waves (unsplit, the accumulators and operands of 4 blocks would not fit
 # the register file)
 WIDE = ["1,64,0", "2,128,0", "2,128,256", "1,32,512", "1,16,0,1", "2,32,256,1", "4,64,0,1", "4,64,256,1",
-        "1,64,0,2", "1,64,128,2"]
+        "1,64,0,2", "1,64,128,2", "2,128,0,2", "2,128,128,2"]
 
 
 @pytest.mark.parametrize("B,r", [(3, 4), (9, 3), (40, 3)])
@@ -151,8 +152,9 @@ def test_forced_sweep_shapes(B, r, seed):
 def test_forced_sweep_shape_rejected():
     import os
     import pecs_amd
-    # split covers 32 rows at 32 columns; the LDS-staged sweep takes m 1 and at most 64 rows
-    for B, shape in ((3, "3,64,0"), (40, "4,256,0"), (9, "1,64,0,1"), (9, "2,64,0,2"), (9, "1,128,0,2")):
+    # split covers 32 rows at 32 columns; the LDS-staged sweep takes m 1 or 2 and at most 64 m rows
+    for B, shape in ((3, "3,64,0"), (40, "4,256,0"), (9, "1,64,0,1"), (9, "4,128,0,2"), (9, "1,128,0,2"),
+                     (9, "2,144,0,2")):
         os.environ["PECS_SWEEP_FORCE"] = shape
         try:
             with pytest.raises(RuntimeError, match="PECS_SWEEP_FORCE"):
