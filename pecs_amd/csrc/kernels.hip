@@ -928,7 +928,19 @@ __global__ __launch_bounds__(256) void k_fwd_gather_il(DevFactor F, const int* _
   if (c0 == c1 || h >= nrhs) return;
   double2* x = reinterpret_cast<double2*>(F.X + (int64_t)p * NRP + h);
   double2 a = *x;
-  for (int64_t c = c0; c < c1; ++c) {
+  // four contributions' loads in flight at a time, subtracted in order
+  int64_t c = c0;
+  for (; c + 4 <= c1; c += 4) {
+    double2 z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) z[u] = *reinterpret_cast<const double2*>(F.Z + F.czidx[c + u] * NRP + h);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a.x -= z[u].x;
+      a.y -= z[u].y;
+    }
+  }
+  for (; c < c1; ++c) {
     const double2 z = *reinterpret_cast<const double2*>(F.Z + F.czidx[c] * NRP + h);
     a.x -= z.x;
     a.y -= z.y;
