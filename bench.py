@@ -283,7 +283,7 @@ def batched_leg(args, workload):
            "ms_per_step_all_instances": ms_step, "ms_per_step_per_instance": ms_step / B,
            "ms_per_step_timed_phases": ms_timed,
            "dof_updates_per_s": U * B * args.batch_steps / el, "phase_ms_per_step": phase,
-           "carrier_solves_mfma": {"kernel": "k_sweep_mfma (FP64 MFMA 16x16x4) + gathers, both regions, one step",
+           "carrier_solves_mfma": {"kernel": "k_sweep_mfma_lds / k_sweep_mfma (FP64 MFMA 16x16x4) + packing and gathers, both regions, one step",
                                    "flops": flops, "avg_ms": cms, "achieved": tf, "peak": FP64_PEAK_TFS,
                                    "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS},
            "sweep_tiles_semi": [list(t) for t in g.sweep_config(0)]}
