@@ -404,8 +404,60 @@ __global__ __launch_bounds__(64) void k_schur_rhs(DevRegion R, const double* __r
   }
 }
 
+// One right-hand side of k_schur_recover on a wave-uniform cell list (the
+// same operations as the kernel's per carrier; half the live registers, for
+// batches that fill the chip).
+template <int K>
+__device__ __forceinline__ void schur_recover_one(const DevRegion& R, const double* __restrict__ b,
+                                                  const double* __restrict__ x, double mu, double* __restrict__ o,
+                                                  int M) {
+  constexpr int NB = Sz<K>::NB;
+  const size_t nc = R.ncell;
+  const int ty = wave_type<true>(R, M);
+  const double* tb = R.table + (size_t)ty * R.stride;
+  double sv[2 * NB], u[NB];
+  const int pb = R.perm_base[M];
+#pragma unroll
+  for (int a = 0; a < NB; ++a) u[a] = x[pb + a];
+  for (int i = 0; i < 2 * NB; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) t += tb[NB * NB + i * NB + a] * u[a];
+    sv[i] = t;
+  }
+  // u is final: stored now, so that it is not live through the rest
+#pragma unroll
+  for (int a = 0; a < NB; ++a) o[2 * nc * NB + (size_t)M * NB + a] = u[a];
+  for (int f = 0; f < 4; ++f) {
+    const int X = R.nbr[4 * (size_t)M + f];
+    if (X < 0) continue;
+    const int px = R.perm_base[X];
+    double v[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) v[a] = x[px + a];
+    const double* B = tb + NB * NB + 2 * NB * NB + f * 2 * NB * NB;
+    for (int i = 0; i < 2 * NB; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) t += B[i * NB + a] * v[a];
+      sv[i] += t;
+    }
+  }
+  double w[2 * NB];
+  schur_w<K>(R, b, mu, M, ty, w);
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      double t = 0.0;
+#pragma unroll
+      for (int q = 0; q < NB; ++q) t += tb[a * NB + q] * sv[h * NB + q];
+      o[(size_t)h * nc * NB + (size_t)M * NB + a] = w[h * NB + a] - mu * t;
+    }
+}
+
 // q = W - A^{-1} B u; writes [Jx | Jy | u] (W recomputed from the RHS b)
-template <int K, int NRHS, bool UNI>
+template <int K, int NRHS, bool UNI, bool SPLIT>
 __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double* __restrict__ b1,
                                                       const double* __restrict__ b2, const double* __restrict__ x1,
                                                       const double* __restrict__ x2, double mu1, double mu2,
@@ -422,6 +474,11 @@ __global__ __launch_bounds__(64) void k_schur_recover(DevRegion R, const double*
     x2 += i * sx;
     o1 += i * so;
     o2 += i * so;
+  }
+  if constexpr (UNI && SPLIT) {
+    schur_recover_one<K>(R, b1, x1, mu1, o1, M);
+    if (NRHS > 1) schur_recover_one<K>(R, b2, x2, mu2, o2, M);
+    return;
   }
   const size_t nc = R.ncell;
   const int ty = wave_type<UNI>(R, M);
@@ -1991,15 +2048,15 @@ static int launch_schur_rhs_main(int k, const DevRegion& R, const double* b1, co
 static int launch_schur_recover_main(int k, const DevRegion& R, const double* b1, const double* b2, const double* xs1,
                          const double* xs2, double mu1, double mu2, double* out1, double* out2, int nrhs,
                          hipStream_t s, int nb, int64_t sb, int64_t sx, int64_t so) {
-#define PECS_RECOVER(UU, NCELL)                                                                                 \
+#define PECS_RECOVER(UU, SS, NCELL)                                                                             \
   {                                                                                                             \
     const dim3 g(nblk(NCELL, 64), nb);                                                                          \
     if (nrhs == 2) {                                                                                            \
-      PECS_DISPATCH_K(k, (k_schur_recover<KK, 2, UU><<<g, 64, 0, s>>>(R, b1, b2, xs1, xs2, mu1, mu2, out1, out2, \
-                                                                      sb, sx, so)));                            \
+      PECS_DISPATCH_K(k, (k_schur_recover<KK, 2, UU, SS><<<g, 64, 0, s>>>(R, b1, b2, xs1, xs2, mu1, mu2, out1,   \
+                                                                          out2, sb, sx, so)));                  \
     } else {                                                                                                    \
-      PECS_DISPATCH_K(k, (k_schur_recover<KK, 1, UU><<<g, 64, 0, s>>>(R, b1, b1, xs1, xs1, mu1, mu1, out1, out1, \
-                                                                      sb, sx, so)));                            \
+      PECS_DISPATCH_K(k, (k_schur_recover<KK, 1, UU, SS><<<g, 64, 0, s>>>(R, b1, b1, xs1, xs1, mu1, mu1, out1,   \
+                                                                          out1, sb, sx, so)));                  \
     }                                                                                                           \
     return (int)hipGetLastError();                                                                              \
   }
@@ -2014,8 +2071,10 @@ static int launch_schur_recover_main(int k, const DevRegion& R, const double* b1
     }
     return (int)hipGetLastError();
   }
-  if (R.ucell_own) PECS_RECOVER(true, R.nu_own)
-  PECS_RECOVER(false, R.ncell)
+  // a pass per right-hand side once a batch fills the chip (4 or more instances)
+  if (R.ucell_own && nb >= 4) PECS_RECOVER(true, true, R.nu_own)
+  if (R.ucell_own) PECS_RECOVER(true, false, R.nu_own)
+  PECS_RECOVER(false, false, R.ncell)
 #undef PECS_RECOVER
 }
 
