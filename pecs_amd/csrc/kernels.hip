@@ -2143,8 +2143,13 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
   if (w <= 0) {  // 0: MFMA sweep, -1: its right-hand-side groups split over waves, -2: LDS-staged vector
     if (nrhs > MAX_MFMA_RHS) return (int)hipErrorInvalidValue;
     const int nrp = mfma_rhs_pad(nrhs);
+    // a factor tuned for many right-hand sides and solved with 16 or fewer
+    // (single solves on a batched context): the LDS-staged shape runs as the
+    // plain MFMA sweep, whose tiles of 64 maxr rows and partial-sum slots
+    // are the same
+    if (w == -2 && nrp < 32) w = 0;
     if (w == -2) {
-      if ((maxr != 1 && maxr != 2) || nrp < 32 || (ks > 0 && (!rtiles || !F.Part))) return (int)hipErrorInvalidValue;
+      if ((maxr != 1 && maxr != 2) || (ks > 0 && (!rtiles || !F.Part))) return (int)hipErrorInvalidValue;
 #define PECS_MFMA_LDS(NN, MM)                                                                    \
   if (nrp == NN && maxr == MM) {                                                                 \
     if (fwd) {                                                                                   \
@@ -2192,8 +2197,9 @@ static int chol_solve_il(const DevFactor& F, const FactorLevels& L, double* v, d
   }
   for (int l = L.nlevels - 1; l >= 0; --l) {
     const int nx = L.xoff[l + 1] - L.xoff[l];
-    // (the LDS-staged sweep reads the panel columns through the row index)
-    if (nx > 0 && L.bw[l] != -2)
+    // (the LDS-staged sweep, 32 or more right-hand sides, reads the panel
+    // columns through the row index)
+    if (nx > 0 && !(L.bw[l] == -2 && NRP >= 32))
       k_bwd_gather_il<NRP><<<nblk((int64_t)nx * NRP / 2, 256), 256, 0, s>>>(F, F.xidx + L.xoff[l], nx);
     const int nt = L.boff[l + 1] - L.boff[l];
     if (L.bw[l] > 0) return (int)hipErrorInvalidValue;

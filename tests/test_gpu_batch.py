@@ -161,3 +161,34 @@ def test_forced_sweep_shape_rejected():
                 pecs_amd.SolarCellGPU(_batch(B, 5, r=2))
         finally:
             del os.environ["PECS_SWEEP_FORCE"]
+
+
+@pytest.mark.parametrize("shape", ["1,64,0,2", "2,128,256,2"])
+def test_single_solves_on_lds_shaped_factors(shape, seed):
+    """Single solves (one right-hand side) on a batched context whose factors
+    were shaped for the LDS-staged sweep (17 instances: 34 carrier and 17
+    multiplier right-hand sides per factor, so both take the shape): the
+    narrow solves run the plain MFMA sweep on the same tiles and match the
+    oracle's solves."""
+    import os
+    import pecs_amd
+    _say(f"[gpu] single solves on LDS-shaped factors {shape}: B=17 r=3 k=2")
+    prms = _batch(17, seed + 11, r=3)
+    os.environ["PECS_SWEEP_FORCE"] = shape
+    try:
+        g = pecs_amd.SolarCellGPU(prms)
+    finally:
+        del os.environ["PECS_SWEEP_FORCE"]
+    try:
+        for which in (0, 1, 2):
+            assert all(t[0] == -2 and t[3] == -2 for t in g.sweep_config(which)), (which, g.sweep_config(which))
+        o = OracleSolarCell(prms[0])
+        rng = np.random.default_rng(seed)
+        for which, key in ((pecs_amd.ELECTRONS, "n"), (pecs_amd.HOLES, "p"), (pecs_amd.REDUCTANTS, "r"),
+                           (pecs_amd.OXIDANTS, "o"), (pecs_amd.POISSON, "P")):
+            n = {"n": g.n_semi, "p": g.n_semi, "r": g.n_elec, "o": g.n_elec, "P": g.n_poisson}[key]
+            b = rng.standard_normal(n)
+            x = g.solve(which, b)
+            assert _rel(x, o.solve(key, b)) < 1e-11, (key, _rel(x, o.solve(key, b)))
+    finally:
+        g.close()
