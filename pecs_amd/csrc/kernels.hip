@@ -1035,13 +1035,23 @@ constexpr int mfma_waves_per_eu(int nrp, int mb, int hpw) {
   return hpw == 1 ? (mb == 1 ? 6 : mb == 2 ? 4 : 2) : (mb == 4 ? 2 : 1);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), each XCD with its own L2.  Tile of workgroup b such that every XCD
+// takes a contiguous run of the tile list: the tiles of one node (which
+// stage the same vector rows) then share an L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / 8, r = n % 8, x = b % 8;
+  return x * q + min(x, r) + b / 8;
+}
+
 template <int NRP, int MB, bool FWD, int HPW = NRP / 16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_per_eu(NRP, MB, HPW)))) void k_sweep_mfma(
     DevFactor F, const int4* __restrict__ tiles, double* __restrict__ Y, int ks, double* __restrict__ Part) {
   constexpr int NH = HPW;                  // right-hand-side groups of this wave
   constexpr int WPR = NRP / 16 / HPW;      // waves per row block
   static_assert(WPR >= 1 && 4 % WPR == 0, "a row block's waves must divide the workgroup");
-  const int4 tl = tiles[blockIdx.x];
+  const int ti = xcd_tile(blockIdx.x, gridDim.x);
+  const int4 tl = tiles[ti];
   const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
   const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
   const int VL = FWD ? ns : ns + ms;
@@ -1163,7 +1173,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
         if (lr >= row0 + nr) continue;
         const double v = acc[mb][hg][r];
         if (ks > 0) {
-          Part[((int64_t)blockIdx.x * (64 * MB) + (lr - row0)) * NRP + h] = v;
+          Part[((int64_t)ti * (64 * MB) + (lr - row0)) * NRP + h] = v;
         } else if (FWD) {
           if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
           else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
@@ -1194,7 +1204,8 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
   constexpr int S = NRP + 4;   // LDS row stride (doubles): the 4 column groups of a read fall in 2 bank halves
   constexpr int K = NRP / 16;  // staged entries per thread per step
   __shared__ __align__(16) double sB[2][16 * S];
-  const int4 tl = tiles[blockIdx.x];
+  const int ti = xcd_tile(blockIdx.x, gridDim.x);
+  const int4 tl = tiles[ti];
   const int s = tl.x, row0 = tl.y, nr = tl.z;  // row0 is a multiple of 16
   const int ns = F.ns[s], ms = F.ms[s], p0 = F.p0[s];
   const int VL = FWD ? ns : ns + ms;
@@ -1327,7 +1338,7 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
         if (lr >= row0 + nr) continue;
         const double v = acc[j][hg][r];
         if (ks > 0) {
-          Part[((int64_t)blockIdx.x * (64 * MBL) + (lr - row0)) * NRP + h] = v;
+          Part[((int64_t)ti * (64 * MBL) + (lr - row0)) * NRP + h] = v;
         } else if (FWD) {
           if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
           else F.Z[(F.z_off[s] + (lr - ns)) * NRP + h] = v;
