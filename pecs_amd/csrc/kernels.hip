@@ -1187,7 +1187,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
 // its 4 waves each own MBL consecutive 16-row blocks of the tile (64 MBL
 // rows) and all right-hand-side groups, and step through the union of their
 // column ranges together.  Per 16-column step the workgroup loads the step's
-// 16 x NRP vector entries once (NRP / 16 per thread, contiguous), stores them
+// 16 x NRP vector entries once (NRP / 16 per thread, 16-byte pieces), stores them
 // in LDS and every wave reads its B operands there, each B operand serving
 // the wave's MBL blocks: the vector bytes leaving L2 drop to 1 / (4 MBL) of
 // per-block loads.  The next step's entries are loaded into registers while
@@ -1238,21 +1238,23 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
     bsteps[j] = rows[j] ? (int)((boff[bnode + b + 1] - bbase[j]) >> 8) : 0;
   }
   const int64_t zrow = FWD ? 0 : F.rows_off[s] - ns;
-  // staged entries of thread t at step st: column c = (t K) / NRP of the
-  // step, right-hand sides h0 .. h0 + K - 1 (contiguous in the interleaved
-  // vector); past the rows' last column the last column's entries (finite;
-  // the factor is zero there).  The backward sweep reads its panel columns
-  // (the ancestors' rows, solved by earlier launches) straight from x
-  // through the row index: no gathered copy.
-  const int tf = threadIdx.x * K, sc = tf / NRP, sh = tf % NRP;
+  // staged entries of thread t at step st: column c = t / 16 of the step,
+  // right-hand-side pairs 2 (t % 16) + 32 jj (jj < K / 2), so that each
+  // 16-lane group moves 256 contiguous bytes per instruction, from the
+  // interleaved vector and into LDS (no bank conflicts on the stores); past
+  // the rows' last column the last column's entries (finite; the factor is
+  // zero there).  The backward sweep reads its panel columns (the
+  // ancestors' rows, solved by earlier launches) straight from x through the
+  // row index: no gathered copy.
+  const int sc = threadIdx.x >> 4, sl = 2 * (threadIdx.x & 15);
   auto stage_load = [&](int st, double (&R)[K]) {
     const int col = min(c0 + 16 * st + sc, VL - 1);
     const double* src;
-    if (FWD) src = F.X + (int64_t)(p0 + col) * NRP + sh;
-    else src = col < ns ? Y + (int64_t)(p0 + col) * NRP + sh : F.X + (int64_t)F.rows[zrow + col] * NRP + sh;
+    if (FWD) src = F.X + (int64_t)(p0 + col) * NRP + sl;
+    else src = col < ns ? Y + (int64_t)(p0 + col) * NRP + sl : F.X + (int64_t)F.rows[zrow + col] * NRP + sl;
 #pragma unroll
     for (int j = 0; j < K; j += 2) {
-      const double2 v = *reinterpret_cast<const double2*>(src + j);
+      const double2 v = *reinterpret_cast<const double2*>(src + 16 * j);
       R[j] = v.x;
       R[j + 1] = v.y;
     }
@@ -1260,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
   auto stage_store = [&](int buf, const double (&R)[K]) {
 #pragma unroll
     for (int j = 0; j < K; j += 2)
-      *reinterpret_cast<double2*>(&sB[buf][sc * S + sh + j]) = double2{R[j], R[j + 1]};
+      *reinterpret_cast<double2*>(&sB[buf][sc * S + sl + 16 * j]) = double2{R[j], R[j + 1]};
   };
   // the factor operands of step st (clamped into each block's stored steps;
   // in[j] = whether the step is one of block j's, wave-uniform)
