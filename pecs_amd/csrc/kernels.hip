@@ -1302,30 +1302,35 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
     }
   };
   if (nstep > 0) {
-    double R[K], LA[MBL][4], LB[MBL][4];
-    int IA[MBL], IB[MBL];
+    // the factor operands are loaded two steps ahead, into three register
+    // sets that rotate (step s uses set s % 3 and loads step s + 2 into the
+    // set step s - 1 used); the staged vector one step ahead, through the
+    // two LDS buffers (step s reads buffer s % 2)
+    double R[K], LA[MBL][4], LB[MBL][4], LC[MBL][4];
+    int IA[MBL], IB[MBL], IC[MBL];
+    auto step = [&](int st, int buf, const double (&Lu)[MBL][4], const int (&Iu)[MBL], double (&Ln)[MBL][4],
+                    int (&In)[MBL]) {
+      stage_load(min(st + 1, nstep - 1), R);
+      a_load(min(st + 2, nstep - 1), Ln, In);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(buf, Lu, Iu);
+      __builtin_amdgcn_sched_barrier(0);
+      if (st + 1 < nstep) stage_store(buf ^ 1, R);
+      __syncthreads();
+      return st + 1 < nstep;
+    };
     stage_load(0, R);
     a_load(0, LA, IA);
+    a_load(min(1, nstep - 1), LB, IB);
     stage_store(0, R);
     __syncthreads();
-    for (int st = 0; st < nstep; st += 2) {
-      // step st from buffer 0 (LA), loading step st + 1
-      stage_load(min(st + 1, nstep - 1), R);
-      a_load(min(st + 1, nstep - 1), LB, IB);
-      __builtin_amdgcn_sched_barrier(0);
-      mma(0, LA, IA);
-      __builtin_amdgcn_sched_barrier(0);
-      if (st + 1 < nstep) stage_store(1, R);
-      __syncthreads();
-      if (st + 1 >= nstep) break;
-      // step st + 1 from buffer 1 (LB), loading step st + 2
-      stage_load(min(st + 2, nstep - 1), R);
-      a_load(min(st + 2, nstep - 1), LA, IA);
-      __builtin_amdgcn_sched_barrier(0);
-      mma(1, LB, IB);
-      __builtin_amdgcn_sched_barrier(0);
-      if (st + 2 < nstep) stage_store(0, R);
-      __syncthreads();
+    for (int st = 0;; st += 6) {
+      if (!step(st, 0, LA, IA, LC, IC)) break;
+      if (!step(st + 1, 1, LB, IB, LA, IA)) break;
+      if (!step(st + 2, 0, LC, IC, LB, IB)) break;
+      if (!step(st + 3, 1, LA, IA, LC, IC)) break;
+      if (!step(st + 4, 0, LB, IB, LA, IA)) break;
+      if (!step(st + 5, 1, LC, IC, LB, IB)) break;
     }
   }
 #pragma unroll
