@@ -1190,8 +1190,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves_
 // 16 x NRP vector entries once (NRP / 16 per thread, 16-byte pieces), stores them
 // in LDS and every wave reads its B operands there, each B operand serving
 // the wave's MBL blocks: the vector bytes leaving L2 drop to 1 / (4 MBL) of
-// per-block loads.  The next step's entries are loaded into registers while
-// this step's MFMAs run, with one barrier per step (double-buffered LDS).  A
+// per-block loads.  The next step's vector entries (and the factor operands
+// of the step after it) are loaded into registers while this step's MFMAs
+// run, with one barrier per step (double-buffered LDS).  The tile order is
+// XCD-aware (xcd_tile).  A
 // block contributes zero on steps outside its stored columns (masked factor
 // operand), and a wave skips the steps none of its blocks stores (wave-uniform
 // through readfirstlane); a wave without rows still loads, stores and meets
