@@ -1358,6 +1358,10 @@ __global__ __launch_bounds__(256) void k_sweep_mfma_lds(DevFactor F, const int4*
   }
 }
 
+// workgroups per row tile in k_sweep_reduce (a level has a few hundred split
+// row tiles: one workgroup each would leave most SIMDs idle)
+constexpr int REDUCE_SPLIT = 8;
+
 // Sum of the column segments of split row tiles, in segment order: row tile
 // rt = (node, row0, nrows, 64 * first tile + nseg); partial sums of tile t at
 // Part[(t * cap + r) * NRP + h] (cap = 64 MB rows per slot).
@@ -1367,10 +1371,22 @@ __global__ __launch_bounds__(256) void k_sweep_reduce(DevFactor F, const int4* _
   const int4 rt = rtiles[blockIdx.x];
   const int s = rt.x, row0 = rt.y, nr = rt.z, t0 = rt.w >> 6, nseg = rt.w & 63;
   const int ns = F.ns[s], p0 = F.p0[s];
-  for (int e = threadIdx.x; e < nr * NRP; e += blockDim.x) {
+  // gridDim.y workgroups share a row tile's entries; each entry's segments
+  // are added in segment order, four loads in flight at a time
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < nr * NRP; e += gridDim.y * blockDim.x) {
     const int r = e / NRP, h = e % NRP;
+    const double* pp = Part + ((int64_t)t0 * cap + r) * NRP + h;
+    const int64_t seg = (int64_t)cap * NRP;
     double v = 0.0;
-    for (int k = 0; k < nseg; ++k) v += Part[((int64_t)(t0 + k) * cap + r) * NRP + h];
+    int k = 0;
+    for (; k + 4 <= nseg; k += 4) {
+      double q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = pp[(k + u) * seg];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v += q[u];
+    }
+    for (; k < nseg; ++k) v += pp[k * seg];
     const int lr = row0 + r;
     if (FWD) {
       if (lr < ns) Y[(int64_t)(p0 + lr) * NRP + h] = v;
@@ -2123,13 +2139,13 @@ static int sweep_mfma_dispatch(const DevFactor& F, const int4* tl, int nt, int m
 #define PECS_MFMA_CASE(NN, MM)                                                              \
   if (!split && nrp == NN && mb == MM) {                                                    \
     k_sweep_mfma<NN, MM, FWD><<<nt, 256, 0, s>>>(F, tl, Y, ks, F.Part);                     \
-    if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<nrt, 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
+    if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<dim3(nrt, REDUCE_SPLIT), 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
     return 0;                                                                               \
   }
 #define PECS_MFMA_SPLIT(NN, MM, HH)                                                         \
   if (split && nrp == NN && mb == MM) {                                                     \
     k_sweep_mfma<NN, MM, FWD, HH><<<nt, 256, 0, s>>>(F, tl, Y, ks, F.Part);                 \
-    if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<nrt, 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
+    if (ks > 0 && nrt > 0) k_sweep_reduce<NN, FWD><<<dim3(nrt, REDUCE_SPLIT), 256, 0, s>>>(F, rtl, 64 * MM, F.Part, Y); \
     return 0;                                                                               \
   }
   PECS_MFMA_CASE(16, 1) PECS_MFMA_CASE(16, 2) PECS_MFMA_CASE(16, 4)
@@ -2174,10 +2190,10 @@ int launch_sweep(const DevFactor& F, const int4* tiles, int nt, int w, int maxr,
   if (nrp == NN && maxr == MM) {                                                                 \
     if (fwd) {                                                                                   \
       k_sweep_mfma_lds<NN, true, MM><<<nt, 256, 0, s>>>(F, tiles, t, ks, F.Part);                \
-      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, true><<<nrt, 256, 0, s>>>(F, rtiles, 64 * MM, F.Part, t); \
+      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, true><<<dim3(nrt, REDUCE_SPLIT), 256, 0, s>>>(F, rtiles, 64 * MM, F.Part, t); \
     } else {                                                                                     \
       k_sweep_mfma_lds<NN, false, MM><<<nt, 256, 0, s>>>(F, tiles, t, ks, F.Part);               \
-      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, false><<<nrt, 256, 0, s>>>(F, rtiles, 64 * MM, F.Part, t); \
+      if (ks > 0 && nrt > 0) k_sweep_reduce<NN, false><<<dim3(nrt, REDUCE_SPLIT), 256, 0, s>>>(F, rtiles, 64 * MM, F.Part, t); \
     }                                                                                            \
     return (int)hipGetLastError();                                                               \
   }
